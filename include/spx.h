@@ -1,0 +1,196 @@
+/*
+ * spx.h -- C ABI of the MI355X-native spinsim propagator library (libspx.so).
+ *
+ * The reference (BekturMur/Spin_Systems) has no FFI: its boundary is the C++
+ * class API of namespace spinsim. This header is the thin C layer beneath a
+ * drop-in re-implementation of that API (spinsim_amd/host, namespace
+ * spinsim): each entry point below names the reference member it backs.
+ * Plain pointers and sizes only; every function returns an spx_status.
+ *
+ * Storage: a state of L spins is 2^L complex FP64 amplitudes, interleaved
+ * (re, im), amplitude k at [2k], [2k+1]; spin s owns bit s of k, set bit = up
+ * (include/spinsim/core/StateVector.hpp:53-60, include/spinsim/core/Bits.hpp:15-27).
+ * Device vectors live in HBM and are handed across as opaque handles; host
+ * buffers cross only through spx_state_upload / spx_state_download.
+ *
+ * Threading: one spx_ctx per host thread (each owns a HIP stream and its
+ * reduction scratch). Objects created from a context must be used from the
+ * thread that owns it. Calls on one context run in stream order.
+ */
+#ifndef SPX_H
+#define SPX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The C++ layer rethrows them as the reference's exception
+ * types (include/spinsim/core/*.hpp): */
+typedef enum {
+  SPX_OK = 0,
+  SPX_EINVAL = 1,     /* std::invalid_argument */
+  SPX_ERANGE = 2,     /* std::out_of_range */
+  SPX_EDOMAIN = 3,    /* std::domain_error (bound <= 0, zero norm) */
+  SPX_ECONVERGE = 4,  /* ChebyshevConvergenceError (include/spinsim/core/Chebyshev.hpp:21-36) */
+  SPX_EOVERFLOW = 5,  /* std::overflow_error (besselI, x > 709) */
+  SPX_ERUNTIME = 6,   /* std::runtime_error */
+  SPX_EDEVICE = 7,    /* HIP runtime failure */
+  SPX_ENOMEM = 8      /* std::bad_alloc / hipErrorOutOfMemory */
+} spx_status;
+
+typedef struct spx_ctx spx_ctx;
+typedef struct spx_state spx_state;
+typedef struct spx_ham spx_ham;
+typedef struct spx_prop spx_prop;
+typedef struct spx_program spx_program;
+typedef struct spx_batch spx_batch;
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* spx_last_error(void);
+/* Context of the last SPX_ECONVERGE on this thread: bound, tau, epsilon,
+ * attempted order (ChebyshevConvergenceError accessors, Chebyshev.hpp:27-31). */
+int spx_last_convergence(double* bound, double* tau, double* epsilon, size_t* order);
+
+/* ------------------------------------------------------------ context */
+int spx_device_count(int* count);
+int spx_ctx_create(int device, spx_ctx** out);
+int spx_ctx_destroy(spx_ctx* ctx);
+int spx_ctx_synchronize(spx_ctx* ctx);
+/* The context's hipStream_t, for interop and for timing on that stream. */
+int spx_ctx_stream(spx_ctx* ctx, void** stream);
+/* Timing on the context's stream: record event `slot` (0..15); elapsed ms
+ * between two recorded slots (synchronises on the later one). */
+int spx_ctx_event_record(spx_ctx* ctx, int slot);
+int spx_ctx_event_elapsed(spx_ctx* ctx, int slot0, int slot1, float* ms);
+
+/* ------------------------------------------------- StateVector (a1)
+ * include/spinsim/core/StateVector.hpp:82-140, src/core/StateVector.cpp:10-75 */
+int spx_state_create(spx_ctx* ctx, size_t nspins, spx_state** out); /* StateVector(L), zeroed */
+int spx_state_destroy(spx_state* s);
+int spx_state_nspins(const spx_state* s, size_t* nspins);
+int spx_state_upload(spx_state* s, const double* host);          /* 2*2^L doubles */
+int spx_state_download(const spx_state* s, double* host);
+int spx_state_copy(spx_state* dst, const spx_state* src);         /* operator= */
+int spx_state_set_zero(spx_state* s);                             /* setZero */
+int spx_state_set_basis(spx_state* s, size_t index);              /* setBasisState */
+int spx_state_scale(spx_state* s, double factor);                 /* scale */
+int spx_state_negate(spx_state* s);                               /* negate */
+int spx_state_add_scaled(spx_state* dst, double cr, double ci, const spx_state* src); /* addScaled */
+int spx_state_norm(const spx_state* s, double* norm);             /* norm */
+int spx_state_normalize(spx_state* s, double* norm_before);       /* normalize */
+int spx_state_inner(const spx_state* self, const spx_state* other, double* re, double* im); /* innerProduct: <other|self> */
+
+/* ------------------------------------------ Bessel / Chebyshev (a6, a7)
+ * Host FP64. src/core/Bessel.cpp:45-149, src/core/Chebyshev.cpp:54-107.
+ * kind 0 = J, 1 = I. mode 0 = TimeMode::Real, 1 = TimeMode::Imaginary. */
+int spx_bessel(int kind, double x, size_t count, double* values, size_t* nonzero);
+int spx_cheb_build(int mode, double bound, double tau, double epsilon, size_t max_order,
+                   double* coeffs, size_t capacity, size_t* order, double* effective_bound);
+
+/* ---------------------------------------------- Hamiltonian (a2-a5)
+ * include/spinsim/core/Hamiltonian.hpp:34-85, src/core/Hamiltonian.cpp:20-147 */
+int spx_ham_create(spx_ctx* ctx, size_t nspins, spx_ham** out);   /* Hamiltonian(L) */
+int spx_ham_destroy(spx_ham* h);
+int spx_ham_set_field(spx_ham* h, size_t spin, double x, double y, double z);  /* setField */
+int spx_ham_get_field(const spx_ham* h, size_t spin, double* xyz);             /* field */
+int spx_ham_set_coupling(spx_ham* h, size_t a, size_t b, double jx, double jy, double jz); /* setCoupling */
+int spx_ham_num_couplings(const spx_ham* h, size_t* n);                        /* couplings().size() */
+int spx_ham_get_coupling(const spx_ham* h, size_t i, uint32_t* lo, uint32_t* hi, double* jxyz);
+int spx_ham_spectral_bound(const spx_ham* h, double* bound);                   /* spectralBound */
+/* kind 0: out = H in (applyPhysical); 1: out = -2H/bound in
+ * (applyTwiceNormalized); 2: out += -2H/bound in (accumulateTwiceNormalized). */
+int spx_ham_apply(spx_ham* h, int kind, double bound, const spx_state* in, spx_state* out);
+/* The cached sigma_z diagonal (rebuildDiagonal), downloaded to host [2^L]. */
+int spx_ham_diagonal(spx_ham* h, double* host);
+
+/* ------------------------------------------------ Propagator (a8)
+ * include/spinsim/core/Propagator.hpp:21-55, src/core/Propagator.cpp:38-100 */
+int spx_prop_create(spx_ctx* ctx, size_t nspins, double epsilon, spx_prop** out);
+int spx_prop_destroy(spx_prop* p);
+/* Propagator::evolve on one state. */
+int spx_prop_evolve(spx_prop* p, spx_ham* h, int mode, spx_state* psi, double tau,
+                    size_t nsteps, int autonormalize);
+/* The same step applied to `nstates` states under one Hamiltonian (the
+ * state and its reference in Sequence::runEvolve, src/sequence/Sequence.cpp:35-48),
+ * sharing one expansion and one diagonal read per order. */
+int spx_prop_evolve_many(spx_prop* p, spx_ham* h, int mode, spx_state* const* states,
+                         size_t nstates, double tau, size_t nsteps, int autonormalize);
+int spx_prop_last_order(const spx_prop* p, size_t* order);                      /* lastOrder */
+int spx_prop_epsilon(const spx_prop* p, double* epsilon);                       /* epsilon */
+/* Hot loop with a prebuilt expansion: `nsteps` applyOneStep calls of
+ * `order` fused Chebyshev orders (coeffs[0..order]) at `bound`. */
+int spx_prop_steps(spx_prop* p, spx_ham* h, int mode, spx_state* psi, const double* coeffs,
+                   size_t order, double bound, size_t nsteps, int autonormalize);
+
+/* --------------------------------------------------- Gates (a9)
+ * include/spinsim/core/Gate.hpp:37-93, src/core/Gate.cpp:20-121 */
+/* SpinRotation(axis, angle, targets).apply: exp(-i angle/2 n.sigma) on each target. */
+int spx_rotate(spx_state* s, double ax, double ay, double az, double angle,
+               const uint32_t* targets, size_t ntargets);
+/* TwoSpinGate(matrix, pair).apply: u16 is 16 complex row-major (32 doubles),
+ * basis (down-down, low-up, high-up, up-up). */
+int spx_two_spin_gate(spx_state* s, const double* u16, size_t a, size_t b);
+
+/* ----------------------------------------- Kernels and observables (a5, a10)
+ * include/spinsim/core/SpinOperators.hpp:22-45, src/core/Observable.cpp:13-120.
+ * axis 0 = x, 1 = y, 2 = z. */
+/* accumulate=1: out += c sigma_axis(spin) in (axpySigma*);
+ * accumulate=0: out = sigma_axis(spin) in (applySigma*). */
+int spx_apply_sigma(int axis, double c, const spx_state* in, size_t spin, spx_state* out,
+                    int accumulate);
+/* out = sum_i sigma_axis(i) in: prepareReferenceState (src/ensemble/Ensemble.cpp:47-58). */
+int spx_apply_total_spin(int axis, const spx_state* in, spx_state* out);
+/* corr:<axis> = <ref| sum_i sigma_axis(i) |state>: TotalSpinCorrelator::measure. */
+int spx_measure_corr(int axis, const spx_state* state, const spx_state* ref, double* re, double* im);
+/* mag:<spin>:<axis> = Re <state|sigma_axis(spin)|state>: SingleSpinMagnetization. */
+int spx_measure_mag(int axis, size_t spin, const spx_state* state, double* value);
+/* energy = Re <state|H|state>: Energy::measure. */
+int spx_measure_energy(spx_ham* h, const spx_state* state, double* value);
+
+/* ------------------------------------------- Pulse program (a11)
+ * The Sequence AST (include/spinsim/sequence/Sequence.hpp:19-104) flattened
+ * for the device runner: Repeat is a begin/end bracket. */
+int spx_program_create(spx_program** out);
+int spx_program_destroy(spx_program* p);
+/* Evolve{tau, nsteps, mode, autonormalize, fields}; fields3L NULL = keep base. */
+int spx_program_evolve(spx_program* p, double tau, size_t nsteps, int mode, int autonormalize,
+                       const double* fields3L, size_t nspins);
+/* Pulse{SpinRotation(axis, angle, targets)}; ntargets 0 = every spin. */
+int spx_program_pulse(spx_program* p, double ax, double ay, double az, double angle,
+                      const uint32_t* targets, size_t ntargets);
+int spx_program_measure(spx_program* p);                        /* Measure{} */
+int spx_program_repeat(spx_program* p, size_t count);            /* Repeat{count, body...} */
+int spx_program_end(spx_program* p);                             /* ...end of body */
+int spx_program_measurements(const spx_program* p, size_t* count); /* measurementCount */
+
+/* ------------------------------------ Batched ensemble runner (a13-a15)
+ * Many realisations of one experiment in one persistent launch, each
+ * realisation's state and reference resident in LDS (L <= 12). Realisation r
+ * is fully described by the host: its L, fields, couplings and initial
+ * state vector (drawn on the host from Rng::forRealization(seed, index) in
+ * runRealization order, src/io/Experiment.cpp:10-55). Observables: codes
+ * 0..2 corr:x/y/z (2 values), 3..5 mag:x/y/z of obs_spin (1 value),
+ * 6 norm (2 values), 7 energy (1 value). Output rows are written per
+ * realisation, [r][point][value], and folded on the host in index order. */
+int spx_batch_create(spx_ctx* ctx, const spx_program* program, double epsilon,
+                     int reference_axis, const int* obs_codes, const int* obs_spins,
+                     size_t nobs, spx_batch** out);
+int spx_batch_destroy(spx_batch* b);
+/* Adds realisation `slot` (0-based, dense). psi0 is the normalised initial
+ * state (2*2^L doubles); fields3L the base fields after disorder; couplings
+ * as SoA. */
+int spx_batch_add(spx_batch* b, size_t nspins, const double* fields3L, size_t npairs,
+                  const uint32_t* lo, const uint32_t* hi, const double* jx, const double* jy,
+                  const double* jz, const double* psi0);
+/* Runs every added realisation; times [points], values [n][points][width]. */
+int spx_batch_run(spx_batch* b, double* times, double* values);
+int spx_batch_width(const spx_batch* b, size_t* width);
+int spx_batch_clear(spx_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPX_H */

@@ -1,0 +1,664 @@
+// The extern "C" surface of libspx (include/spx.h): contexts, device state
+// vectors, Hamiltonians, the Chebyshev propagator driver, gates,
+// observables, pulse programs and the batched ensemble runner.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "spx_batch.hpp"
+#include "spx_common.hpp"
+
+namespace spx {
+const char* last_error();
+void last_convergence(double* b, double* t, double* e, size_t* o);
+}  // namespace spx
+
+using namespace spx;
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+int check_state(const spx_state* s) {
+  if (!s || !s->d) return fail(SPX_EINVAL, "null state");
+  return SPX_OK;
+}
+
+int same_space(const spx_state* a, const spx_state* b) {
+  SPX_TRY(check_state(a));
+  SPX_TRY(check_state(b));
+  if (a->n != b->n) return fail(SPX_EINVAL, "state vectors span different spaces");
+  return SPX_OK;
+}
+
+int set_device(spx_ctx* c) {
+  SPX_HIP(hipSetDevice(c->device));
+  return SPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spx_last_error(void) { return spx::last_error(); }
+
+int spx_last_convergence(double* bound, double* tau, double* epsilon, size_t* order) {
+  spx::last_convergence(bound, tau, epsilon, order);
+  return SPX_OK;
+}
+
+// ------------------------------------------------------------------ context
+
+int spx_device_count(int* count) {
+  int n = 0;
+  SPX_HIP(hipGetDeviceCount(&n));
+  *count = n;
+  return SPX_OK;
+}
+
+int spx_ctx_create(int device, spx_ctx** out) {
+  if (!out) return fail(SPX_EINVAL, "null output");
+  int n = 0;
+  SPX_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(SPX_ERANGE, "device index out of range");
+  SPX_HIP(hipSetDevice(device));
+  auto* c = new spx_ctx();
+  c->device = device;
+  SPX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  SPX_HIP(hipMalloc(&c->d_partials, 2 * kReduceBlocks * sizeof(double)));
+  SPX_HIP(hipMalloc(&c->d_result, 16 * sizeof(double)));
+  SPX_HIP(hipHostMalloc(&c->h_result, 16 * sizeof(double), hipHostMallocDefault));
+  for (int i = 0; i < kMaxEvents; ++i) SPX_HIP(hipEventCreate(&c->events[i]));
+  hipDeviceProp_t prop;
+  SPX_HIP(hipGetDeviceProperties(&prop, device));
+  c->arch_cus = prop.multiProcessorCount;
+  *out = c;
+  return SPX_OK;
+}
+
+int spx_ctx_destroy(spx_ctx* c) {
+  if (!c) return SPX_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (int i = 0; i < kMaxEvents; ++i)
+    if (c->events[i]) hipEventDestroy(c->events[i]);
+  hipFree(c->d_partials);
+  hipFree(c->d_result);
+  hipHostFree(c->h_result);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return SPX_OK;
+}
+
+int spx_ctx_synchronize(spx_ctx* c) {
+  SPX_HIP(hipStreamSynchronize(c->stream));
+  return SPX_OK;
+}
+
+int spx_ctx_stream(spx_ctx* c, void** stream) {
+  *stream = static_cast<void*>(c->stream);
+  return SPX_OK;
+}
+
+int spx_ctx_event_record(spx_ctx* c, int slot) {
+  if (slot < 0 || slot >= kMaxEvents) return fail(SPX_ERANGE, "event slot out of range");
+  SPX_HIP(hipEventRecord(c->events[slot], c->stream));
+  return SPX_OK;
+}
+
+int spx_ctx_event_elapsed(spx_ctx* c, int s0, int s1, float* ms) {
+  if (s0 < 0 || s0 >= kMaxEvents || s1 < 0 || s1 >= kMaxEvents)
+    return fail(SPX_ERANGE, "event slot out of range");
+  SPX_HIP(hipEventSynchronize(c->events[s1]));
+  SPX_HIP(hipEventElapsedTime(ms, c->events[s0], c->events[s1]));
+  return SPX_OK;
+}
+
+// -------------------------------------------------------------- StateVector
+
+int spx_state_create(spx_ctx* c, size_t nspins, spx_state** out) {
+  if (!c || !out) return fail(SPX_EINVAL, "null argument");
+  if (nspins > kMaxSpins) return fail(SPX_ERANGE, "number of spins above kMaxSpins");
+  SPX_TRY(set_device(c));
+  auto* s = new spx_state();
+  s->ctx = c;
+  s->nspins = nspins;
+  s->n = static_cast<size_t>(1) << nspins;
+  hipError_t e = hipMalloc(&s->d, s->n * sizeof(double2));
+  if (e != hipSuccess) {
+    delete s;
+    return fail(SPX_ENOMEM, "hipMalloc of a state vector failed");
+  }
+  int rc = launch_fill(c, s->d, s->n);
+  if (rc != SPX_OK) {
+    hipFree(s->d);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return SPX_OK;
+}
+
+int spx_state_destroy(spx_state* s) {
+  if (!s) return SPX_OK;
+  hipStreamSynchronize(s->ctx->stream);
+  hipFree(s->d);
+  delete s;
+  return SPX_OK;
+}
+
+int spx_state_nspins(const spx_state* s, size_t* nspins) {
+  SPX_TRY(check_state(s));
+  *nspins = s->nspins;
+  return SPX_OK;
+}
+
+int spx_state_upload(spx_state* s, const double* host) {
+  SPX_TRY(check_state(s));
+  SPX_HIP(hipMemcpyAsync(s->d, host, s->n * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+  SPX_HIP(hipStreamSynchronize(s->ctx->stream));
+  return SPX_OK;
+}
+
+int spx_state_download(const spx_state* s, double* host) {
+  SPX_TRY(check_state(s));
+  SPX_HIP(hipMemcpyAsync(host, s->d, s->n * sizeof(double2), hipMemcpyDeviceToHost, s->ctx->stream));
+  SPX_HIP(hipStreamSynchronize(s->ctx->stream));
+  return SPX_OK;
+}
+
+int spx_state_copy(spx_state* dst, const spx_state* src) {
+  SPX_TRY(same_space(dst, src));
+  if (dst == src) return SPX_OK;
+  SPX_HIP(hipMemcpyAsync(dst->d, src->d, src->n * sizeof(double2), hipMemcpyDeviceToDevice,
+                         dst->ctx->stream));
+  return SPX_OK;
+}
+
+int spx_state_set_zero(spx_state* s) {
+  SPX_TRY(check_state(s));
+  return launch_fill(s->ctx, s->d, s->n);
+}
+
+int spx_state_set_basis(spx_state* s, size_t index) {
+  SPX_TRY(check_state(s));
+  if (index >= s->n) return fail(SPX_ERANGE, "basis state index outside the 2^L state space");
+  SPX_TRY(launch_fill(s->ctx, s->d, s->n));
+  static const double one[2] = {1.0, 0.0};
+  SPX_HIP(hipMemcpyAsync(s->d + index, one, sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
+  SPX_HIP(hipStreamSynchronize(s->ctx->stream));
+  return SPX_OK;
+}
+
+int spx_state_scale(spx_state* s, double f) {
+  SPX_TRY(check_state(s));
+  return launch_scale(s->ctx, s->d, f, s->n);
+}
+
+int spx_state_negate(spx_state* s) {
+  SPX_TRY(check_state(s));
+  return launch_negate(s->ctx, s->d, s->n);
+}
+
+int spx_state_add_scaled(spx_state* dst, double cr, double ci, const spx_state* src) {
+  SPX_TRY(same_space(dst, src));
+  return launch_add_scaled(dst->ctx, dst->d, cr, ci, src->d, dst->n);
+}
+
+int spx_state_norm(const spx_state* s, double* norm) {
+  SPX_TRY(check_state(s));
+  double sum = 0.0;
+  SPX_TRY(reduce_norm2(s->ctx, s->d, s->n, &sum));
+  *norm = std::sqrt(sum);
+  return SPX_OK;
+}
+
+int spx_state_normalize(spx_state* s, double* norm_before) {
+  double nrm = 0.0;
+  SPX_TRY(spx_state_norm(s, &nrm));
+  if (norm_before) *norm_before = nrm;
+  if (nrm == 0.0) return fail(SPX_EDOMAIN, "cannot normalize a zero state");
+  return launch_scale(s->ctx, s->d, 1.0 / nrm, s->n);
+}
+
+int spx_state_inner(const spx_state* self, const spx_state* other, double* re, double* im) {
+  SPX_TRY(same_space(self, other));
+  return reduce_inner(self->ctx, self->d, other->d, self->n, re, im);
+}
+
+// ------------------------------------------------------- Bessel / Chebyshev
+
+int spx_bessel(int kind, double x, size_t count, double* values, size_t* nonzero) {
+  if (!values && count) return fail(SPX_EINVAL, "null output");
+  return spx::bessel(kind, x, count, values, nonzero);
+}
+
+int spx_cheb_build(int mode, double bound, double tau, double epsilon, size_t max_order,
+                   double* coeffs, size_t capacity, size_t* order, double* effective_bound) {
+  std::vector<double> c;
+  double eb = 1.0;
+  SPX_TRY(cheb_build(mode, bound, tau, epsilon, max_order, c, eb));
+  if (c.size() > capacity) return fail(SPX_EINVAL, "coefficient buffer too small");
+  std::memcpy(coeffs, c.data(), c.size() * sizeof(double));
+  *order = c.size() - 1;
+  *effective_bound = eb;
+  return SPX_OK;
+}
+
+// -------------------------------------------------------------- Hamiltonian
+
+int spx_ham_create(spx_ctx* c, size_t nspins, spx_ham** out) {
+  if (!c || !out) return fail(SPX_EINVAL, "null argument");
+  if (nspins == 0 || nspins > kMaxSpins)
+    return fail(SPX_ERANGE, "number of spins must be in [1, kMaxSpins]");
+  auto* h = new spx_ham();
+  h->ctx = c;
+  h->nspins = nspins;
+  h->hx.assign(nspins, 0.0);
+  h->hy.assign(nspins, 0.0);
+  h->hz.assign(nspins, 0.0);
+  *out = h;
+  return SPX_OK;
+}
+
+int spx_ham_destroy(spx_ham* h) {
+  if (!h) return SPX_OK;
+  hipStreamSynchronize(h->ctx->stream);
+  if (h->d_diag) hipFree(h->d_diag);
+  if (h->d_terms) hipFree(h->d_terms);
+  if (h->d_tab) hipFree(h->d_tab);
+  delete h;
+  return SPX_OK;
+}
+
+int spx_ham_set_field(spx_ham* h, size_t spin, double x, double y, double z) {
+  if (spin >= h->nspins) return fail(SPX_ERANGE, "spin index out of range");
+  if (h->hx[spin] != x || h->hy[spin] != y) h->terms_valid = false;
+  h->hx[spin] = x;
+  h->hy[spin] = y;
+  h->hz[spin] = z;
+  return SPX_OK;
+}
+
+int spx_ham_get_field(const spx_ham* h, size_t spin, double* xyz) {
+  if (spin >= h->nspins) return fail(SPX_ERANGE, "spin index out of range");
+  xyz[0] = h->hx[spin];
+  xyz[1] = h->hy[spin];
+  xyz[2] = h->hz[spin];
+  return SPX_OK;
+}
+
+int spx_ham_set_coupling(spx_ham* h, size_t a, size_t b, double jx, double jy, double jz) {
+  if (a == b) return fail(SPX_EINVAL, "a spin cannot couple to itself");
+  const size_t lo = a < b ? a : b, hi = a < b ? b : a;
+  if (hi >= h->nspins) return fail(SPX_ERANGE, "coupled spin index out of range");
+  h->terms_valid = false;
+  for (size_t p = 0; p < h->lo.size(); ++p)
+    if (h->lo[p] == lo && h->hi[p] == hi) {
+      h->jx[p] = jx;
+      h->jy[p] = jy;
+      h->jz[p] = jz;
+      return SPX_OK;
+    }
+  h->lo.push_back(static_cast<uint32_t>(lo));
+  h->hi.push_back(static_cast<uint32_t>(hi));
+  h->jx.push_back(jx);
+  h->jy.push_back(jy);
+  h->jz.push_back(jz);
+  return SPX_OK;
+}
+
+int spx_ham_num_couplings(const spx_ham* h, size_t* n) {
+  *n = h->lo.size();
+  return SPX_OK;
+}
+
+int spx_ham_get_coupling(const spx_ham* h, size_t i, uint32_t* lo, uint32_t* hi, double* jxyz) {
+  if (i >= h->lo.size()) return fail(SPX_ERANGE, "coupling index out of range");
+  *lo = h->lo[i];
+  *hi = h->hi[i];
+  jxyz[0] = h->jx[i];
+  jxyz[1] = h->jy[i];
+  jxyz[2] = h->jz[i];
+  return SPX_OK;
+}
+
+// spectralBound (src/core/Hamiltonian.cpp:82-93), same summation order.
+int spx_ham_spectral_bound(const spx_ham* h, double* bound) {
+  double b = 0.0;
+  for (size_t s = 0; s < h->nspins; ++s)
+    b += 0.5 * (std::abs(h->hx[s]) + std::abs(h->hy[s]) + std::abs(h->hz[s]));
+  for (size_t p = 0; p < h->lo.size(); ++p)
+    b += 0.25 * (std::abs(h->jx[p]) + std::abs(h->jy[p]) + std::abs(h->jz[p]));
+  *bound = b;
+  return SPX_OK;
+}
+
+int spx_ham_apply(spx_ham* h, int kind, double bound, const spx_state* in, spx_state* out) {
+  SPX_TRY(same_space(in, out));
+  if (in->nspins != h->nspins) return fail(SPX_EINVAL, "state and Hamiltonian disagree on spin count");
+  if (in == out) return fail(SPX_EINVAL, "kernels require distinct input and output");
+  SPX_TRY(set_device(h->ctx));
+  if (kind == 0) return launch_apply(h, 1.0, in->d, out->d, false);
+  if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
+  return launch_apply(h, -2.0 / bound, in->d, out->d, kind == 2);
+}
+
+int spx_ham_diagonal(spx_ham* h, double* host) {
+  SPX_TRY(ham_prepare(h, h->terms_valid ? h->terms_scale : 1.0));
+  const size_t n = static_cast<size_t>(1) << h->nspins;
+  SPX_HIP(hipMemcpyAsync(host, h->d_diag, n * sizeof(double), hipMemcpyDeviceToHost, h->ctx->stream));
+  SPX_HIP(hipStreamSynchronize(h->ctx->stream));
+  return SPX_OK;
+}
+
+// --------------------------------------------------------------- Propagator
+
+int spx_prop_create(spx_ctx* c, size_t nspins, double epsilon, spx_prop** out) {
+  if (!c || !out) return fail(SPX_EINVAL, "null argument");
+  if (!(epsilon > 0.0)) return fail(SPX_EINVAL, "epsilon must be positive");
+  if (nspins > kMaxSpins) return fail(SPX_ERANGE, "number of spins above kMaxSpins");
+  auto* p = new spx_prop();
+  p->ctx = c;
+  p->nspins = nspins;
+  p->epsilon = epsilon;
+  *out = p;
+  return SPX_OK;
+}
+
+int spx_prop_destroy(spx_prop* p) {
+  if (!p) return SPX_OK;
+  hipStreamSynchronize(p->ctx->stream);
+  for (auto& d : p->d_scratch)
+    if (d) hipFree(d);
+  delete p;
+  return SPX_OK;
+}
+
+int spx_prop_last_order(const spx_prop* p, size_t* order) {
+  *order = p->last_order;
+  return SPX_OK;
+}
+
+int spx_prop_epsilon(const spx_prop* p, double* epsilon) {
+  *epsilon = p->epsilon;
+  return SPX_OK;
+}
+
+namespace {
+
+int ensure_scratch(spx_prop* p, size_t nvec) {
+  const size_t n = static_cast<size_t>(1) << p->nspins;
+  for (size_t i = 0; i < 2 * nvec; ++i)
+    if (!p->d_scratch[i]) {
+      hipError_t e = hipMalloc(&p->d_scratch[i], n * sizeof(double2));
+      if (e != hipSuccess) return fail(SPX_ENOMEM, "hipMalloc of propagator scratch failed");
+    }
+  return SPX_OK;
+}
+
+// applyOneStep (src/core/Propagator.cpp:46-73) for up to kMaxMany states:
+// T_1 into B; T_2 into A with psi = a0 T_0 + c1 T_1 + c2 T_2 (psi still holds
+// T_0 until then, so no copy of it is made); from there T_n overwrites
+// T_{n-2} in place and psi += c_n T_n in the same pass.
+int one_step(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t nvec,
+             const std::vector<double>& a) {
+  const size_t order = a.size() - 1;
+  double2* A[kMaxMany];
+  double2* B[kMaxMany];
+  for (size_t i = 0; i < nvec; ++i) {
+    A[i] = p->d_scratch[2 * i];
+    B[i] = p->d_scratch[2 * i + 1];
+  }
+  // TermWeight (src/core/Propagator.cpp:14-36): w_1 = -i (real) or -1 (imag)
+  double wre = mode == 0 ? 0.0 : -1.0, wim = mode == 0 ? -1.0 : 0.0;
+  auto advance = [&] {
+    if (mode == 0) {
+      const double pr = wre;
+      wre = wim;
+      wim = -pr;
+    } else {
+      wre = -wre;
+      wim = -wim;
+    }
+  };
+  OrderLaunch v{};
+  v.nvec = static_cast<int>(nvec);
+  for (size_t i = 0; i < nvec; ++i) {
+    v.in[i] = states[i]->d;
+    v.out[i] = B[i];
+    v.psi[i] = states[i]->d;
+  }
+  SPX_TRY(launch_order(h, 1, v, 0, 0, 0, 0, 0));
+  const double c1r = 2.0 * a[1] * wre, c1i = 2.0 * a[1] * wim;
+  advance();
+  if (order == 1) {
+    for (size_t i = 0; i < nvec; ++i)
+      SPX_TRY(launch_axpby(p->ctx, states[i]->d, a[0], B[i], c1r, c1i, states[i]->n));
+    return SPX_OK;
+  }
+  for (size_t i = 0; i < nvec; ++i) {
+    v.in[i] = B[i];
+    v.out[i] = A[i];
+  }
+  SPX_TRY(launch_order(h, 2, v, a[0], c1r, c1i, 2.0 * a[2] * wre, 2.0 * a[2] * wim));
+  advance();
+  for (size_t n = 3; n <= order; ++n) {
+    for (size_t i = 0; i < nvec; ++i) {
+      const bool odd = (n % 2) == 1;
+      v.in[i] = odd ? A[i] : B[i];    // T_{n-1}
+      v.out[i] = odd ? B[i] : A[i];   // T_{n-2} -> T_n
+    }
+    SPX_TRY(launch_order(h, 3, v, 0, 0, 0, 2.0 * a[n] * wre, 2.0 * a[n] * wim));
+    advance();
+  }
+  return SPX_OK;
+}
+
+int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t nstates,
+                const std::vector<double>* prebuilt, double prebuilt_bound, double tau,
+                size_t nsteps, int autonormalize) {
+  if (!p || !h || !states) return fail(SPX_EINVAL, "null argument");
+  if (nstates == 0) return SPX_OK;
+  for (size_t i = 0; i < nstates; ++i) {
+    SPX_TRY(check_state(states[i]));
+    if (states[i]->nspins != h->nspins)
+      return fail(SPX_EINVAL, "state and Hamiltonian disagree on spin count");
+    if (states[i]->nspins != p->nspins)
+      return fail(SPX_EINVAL, "state does not match the propagator's size");
+    for (size_t j = 0; j < i; ++j)
+      if (states[j] == states[i]) return fail(SPX_EINVAL, "a state appears twice");
+  }
+  if (mode != 0 && mode != 1) return fail(SPX_EINVAL, "unknown time mode");
+  if (nsteps == 0) return SPX_OK;
+
+  std::vector<double> coeffs;
+  double bound = 1.0;
+  if (prebuilt) {
+    coeffs = *prebuilt;
+    bound = prebuilt_bound;
+    if (coeffs.empty()) return fail(SPX_EINVAL, "empty expansion");
+  } else {
+    double sb = 0.0;
+    spx_ham_spectral_bound(h, &sb);
+    SPX_TRY(cheb_build(mode, sb, tau, p->epsilon, 6000, coeffs, bound));
+    p->last_order = coeffs.size() - 1;
+  }
+  if (coeffs.size() == 1) return SPX_OK;  // identity: psi untouched
+  if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
+  SPX_TRY(set_device(p->ctx));
+  SPX_TRY(ensure_scratch(p, nstates < kMaxMany ? nstates : kMaxMany));
+  SPX_TRY(ham_prepare(h, -2.0 / bound));
+  for (size_t step = 0; step < nsteps; ++step) {
+    for (size_t first = 0; first < nstates; first += kMaxMany) {
+      const size_t cnt = nstates - first < static_cast<size_t>(kMaxMany) ? nstates - first : kMaxMany;
+      SPX_TRY(one_step(p, h, mode, states + first, cnt, coeffs));
+    }
+    if (autonormalize)
+      for (size_t i = 0; i < nstates; ++i) SPX_TRY(spx_state_normalize(states[i], nullptr));
+  }
+  return SPX_OK;
+}
+
+}  // namespace
+
+int spx_prop_evolve(spx_prop* p, spx_ham* h, int mode, spx_state* psi, double tau, size_t nsteps,
+                    int autonormalize) {
+  spx_state* one[1] = {psi};
+  return evolve_impl(p, h, mode, one, 1, nullptr, 0.0, tau, nsteps, autonormalize);
+}
+
+int spx_prop_evolve_many(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t nstates,
+                         double tau, size_t nsteps, int autonormalize) {
+  return evolve_impl(p, h, mode, states, nstates, nullptr, 0.0, tau, nsteps, autonormalize);
+}
+
+int spx_prop_steps(spx_prop* p, spx_ham* h, int mode, spx_state* psi, const double* coeffs,
+                   size_t order, double bound, size_t nsteps, int autonormalize) {
+  if (!coeffs) return fail(SPX_EINVAL, "null coefficients");
+  std::vector<double> c(coeffs, coeffs + order + 1);
+  spx_state* one[1] = {psi};
+  return evolve_impl(p, h, mode, one, 1, &c, bound, 0.0, nsteps, autonormalize);
+}
+
+// -------------------------------------------------------------------- Gates
+
+int spx_rotate(spx_state* s, double ax, double ay, double az, double angle, const uint32_t* targets,
+               size_t ntargets) {
+  SPX_TRY(check_state(s));
+  // Axis3::normalized and Rotation2x2::make (src/core/Gate.cpp:20-51)
+  const double length = std::sqrt(ax * ax + ay * ay + az * az);
+  if (!(length > 0.0)) return fail(SPX_EINVAL, "rotation axis must have non-zero length");
+  if (ntargets == 0 || !targets) return fail(SPX_EINVAL, "a rotation must name at least one spin");
+  const double nx = ax / length, ny = ay / length, nz = az / length;
+  const double c = std::cos(0.5 * angle), sn = std::sin(0.5 * angle);
+  const double u[8] = {c, sn * nz, sn * ny, -sn * nx, -sn * ny, -sn * nx, c, -sn * nz};
+  for (size_t t = 0; t < ntargets; ++t) {
+    if (targets[t] >= s->nspins) return fail(SPX_ERANGE, "rotation names a spin outside the system");
+    SPX_TRY(launch_rotate(s->ctx, s->d, s->n, targets[t], u));
+  }
+  return SPX_OK;
+}
+
+int spx_two_spin_gate(spx_state* s, const double* u16, size_t a, size_t b) {
+  SPX_TRY(check_state(s));
+  if (a == b) return fail(SPX_EINVAL, "a spin cannot couple to itself");
+  const size_t lo = a < b ? a : b, hi = a < b ? b : a;
+  if (hi >= s->nspins) return fail(SPX_ERANGE, "gate names a spin outside the system");
+  return launch_two_spin(s->ctx, s->d, s->n, static_cast<uint32_t>(lo), static_cast<uint32_t>(hi), u16);
+}
+
+// -------------------------------------------------------------- Observables
+
+int spx_apply_sigma(int axis, double c, const spx_state* in, size_t spin, spx_state* out,
+                    int accumulate) {
+  SPX_TRY(same_space(in, out));
+  if (in == out) return fail(SPX_EINVAL, "kernels require distinct input and output");
+  if (axis < 0 || axis > 2) return fail(SPX_EINVAL, "axis must be one of x, y, z");
+  if (spin >= in->nspins) return fail(SPX_ERANGE, "spin index out of range");
+  return launch_sigma(in->ctx, axis, c, in->d, spin, out->d, in->n, accumulate != 0);
+}
+
+int spx_apply_total_spin(int axis, const spx_state* in, spx_state* out) {
+  SPX_TRY(same_space(in, out));
+  if (in == out) return fail(SPX_EINVAL, "kernels require distinct input and output");
+  if (axis < 0 || axis > 2) return fail(SPX_EINVAL, "axis must be one of x, y, z");
+  return launch_total_spin(in->ctx, axis, in->d, out->d, in->nspins);
+}
+
+int spx_measure_corr(int axis, const spx_state* state, const spx_state* ref, double* re, double* im) {
+  SPX_TRY(same_space(state, ref));
+  if (axis < 0 || axis > 2) return fail(SPX_EINVAL, "axis must be one of x, y, z");
+  return reduce_corr(state->ctx, axis, state->d, ref->d, state->nspins, re, im);
+}
+
+int spx_measure_mag(int axis, size_t spin, const spx_state* state, double* value) {
+  SPX_TRY(check_state(state));
+  if (axis < 0 || axis > 2) return fail(SPX_EINVAL, "axis must be one of x, y, z");
+  if (spin >= state->nspins) return fail(SPX_ERANGE, "observable names a spin outside the system");
+  return reduce_mag(state->ctx, axis, static_cast<uint32_t>(spin), state->d, state->nspins, value);
+}
+
+int spx_measure_energy(spx_ham* h, const spx_state* state, double* value) {
+  SPX_TRY(check_state(state));
+  if (state->nspins != h->nspins) return fail(SPX_EINVAL, "state and Hamiltonian disagree on spin count");
+  return reduce_energy(h, state->d, value);
+}
+
+// ------------------------------------------------------------------ Program
+
+int spx_program_create(spx_program** out) {
+  *out = new spx_program();
+  return SPX_OK;
+}
+
+int spx_program_destroy(spx_program* p) {
+  delete p;
+  return SPX_OK;
+}
+
+int spx_program_evolve(spx_program* p, double tau, size_t nsteps, int mode, int autonormalize,
+                       const double* fields3L, size_t nspins) {
+  if (mode != 0 && mode != 1) return fail(SPX_EINVAL, "unknown time mode");
+  ProgOp op;
+  op.kind = OP_EVOLVE;
+  op.tau = tau;
+  op.nsteps = nsteps;
+  op.mode = mode;
+  op.autonorm = autonormalize != 0;
+  if (fields3L) op.fields.assign(fields3L, fields3L + 3 * nspins);
+  p->ops.push_back(op);
+  return SPX_OK;
+}
+
+int spx_program_pulse(spx_program* p, double ax, double ay, double az, double angle,
+                      const uint32_t* targets, size_t ntargets) {
+  const double length = std::sqrt(ax * ax + ay * ay + az * az);
+  if (!(length > 0.0)) return fail(SPX_EINVAL, "rotation axis must have non-zero length");
+  ProgOp op;
+  op.kind = OP_PULSE;
+  op.axis[0] = ax;
+  op.axis[1] = ay;
+  op.axis[2] = az;
+  op.angle = angle;
+  if (targets && ntargets) op.targets.assign(targets, targets + ntargets);
+  p->ops.push_back(op);
+  return SPX_OK;
+}
+
+int spx_program_measure(spx_program* p) {
+  ProgOp op;
+  op.kind = OP_MEASURE;
+  p->ops.push_back(op);
+  return SPX_OK;
+}
+
+int spx_program_repeat(spx_program* p, size_t count) {
+  if (p->open.size() >= 32) return fail(SPX_EINVAL, "repeat nesting deeper than 32");
+  ProgOp op;
+  op.kind = OP_REPEAT;
+  op.count = count;
+  p->open.push_back(p->ops.size());
+  p->ops.push_back(op);
+  return SPX_OK;
+}
+
+int spx_program_end(spx_program* p) {
+  if (p->open.empty()) return fail(SPX_EINVAL, "end without an open repeat");
+  const size_t b = p->open.back();
+  p->open.pop_back();
+  ProgOp op;
+  op.kind = OP_END;
+  op.match = b;
+  p->ops[b].match = p->ops.size();
+  p->ops.push_back(op);
+  return SPX_OK;
+}
+
+int spx_program_measurements(const spx_program* p, size_t* count) {
+  if (!p->open.empty()) return fail(SPX_EINVAL, "program has an unclosed repeat");
+  *count = program_measurements(*p, 0, p->ops.size());
+  return SPX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+// Internal definitions shared by the libspx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "spx.h"
+
+namespace spx {
+
+// Coefficients at or below this magnitude are treated as absent, as in the
+// reference's accumulate (src/core/Hamiltonian.cpp:12-16).
+constexpr double kNegligible = 1.0e-14;
+constexpr size_t kMaxSpins = 30;          // include/spinsim/core/Bits.hpp:13
+constexpr int kMaxEvents = 16;
+constexpr int kMaxMany = 4;               // states per batched evolve launch
+constexpr int kReduceBlocks = 1024;       // fixed reduction fan-in (determinism)
+constexpr int kReduceThreads = 256;
+
+// ---------------------------------------------------------------- errors
+int fail(int code, const std::string& msg);
+int ok();
+void set_convergence(double bound, double tau, double eps, size_t order);
+
+#define SPX_HIP(expr)                                                      \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return ::spx::fail(SPX_EDEVICE, std::string(#expr ": ") +            \
+                                          hipGetErrorString(e_));          \
+  } while (0)
+
+#define SPX_TRY(expr)              \
+  do {                             \
+    int rc_ = (expr);              \
+    if (rc_ != SPX_OK) return rc_; \
+  } while (0)
+
+// --------------------------------------------------------- device tables
+// One flip term of the Pauli-basis Hamiltonian, pre-multiplied by the apply
+// scale: for a coupled pair (mask = bit lo | bit hi) `anti` is the
+// flip-flop weight (|ud> <-> |du>, scale*(jx+jy)/4) and `aligned` the
+// flip-both weight (|dd> <-> |uu>, scale*(jx-jy)/4); for a transverse field
+// (mask = one bit) `anti` = scale*hx/2 and `aligned` = scale*hy/2 and the
+// term is (cx -/+ i cy) on the partner depending on the spin's bit.
+struct FlipTerm {
+  uint32_t mask;
+  uint32_t kind;  // 0 = pair, 1 = field
+  double anti;
+  double aligned;
+  double pad;
+};
+
+// ----------------------------------------------------------------- objects
+}  // namespace spx
+
+struct spx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* d_partials = nullptr;   // [2 * kReduceBlocks]
+  double* d_result = nullptr;     // [16]
+  double* h_result = nullptr;     // pinned [16]
+  hipEvent_t events[spx::kMaxEvents] = {};
+  int arch_cus = 0;
+};
+
+struct spx_state {
+  spx_ctx* ctx = nullptr;
+  size_t nspins = 0;
+  size_t n = 0;           // 2^L amplitudes
+  double2* d = nullptr;   // device, interleaved complex
+};
+
+struct spx_ham {
+  spx_ctx* ctx = nullptr;
+  size_t nspins = 0;
+  std::vector<double> hx, hy, hz;
+  std::vector<uint32_t> lo, hi;
+  std::vector<double> jx, jy, jz;
+  // device sigma_z diagonal (Pauli units) and the (hz, jz) it was built from
+  double* d_diag = nullptr;
+  bool diag_valid = false;
+  std::vector<double> diag_hz, diag_jz;
+  std::vector<uint32_t> diag_lo, diag_hi;
+  // device flip table for the most recent scale
+  spx::FlipTerm* d_terms = nullptr;
+  size_t terms_cap = 0;
+  int nterms = 0;
+  int terms_nfield = 0, terms_nlocal = 0, terms_nremote = 0, terms_tb = 0;
+  double terms_scale = 0.0;
+  bool terms_valid = false;
+  std::vector<spx::FlipTerm> h_terms;
+  double* d_tab = nullptr;   // diagonal build table
+  size_t tab_cap = 0;
+};
+
+struct spx_prop {
+  spx_ctx* ctx = nullptr;
+  size_t nspins = 0;
+  double epsilon = 1e-7;
+  size_t last_order = 0;
+  double2* d_scratch[2 * spx::kMaxMany] = {};
+};
+
+namespace spx {
+
+// ---------------------------------------------------------- host helpers
+int bessel(int kind, double x, size_t count, double* out, size_t* nonzero);
+int cheb_build(int mode, double bound, double tau, double eps, size_t max_order,
+               std::vector<double>& coeffs, double& eff_bound);
+
+// ------------------------------------------------------ device launchers
+struct OrderLaunch {
+  const double2* in[kMaxMany];   // operand of H
+  double2* out[kMaxMany];        // result / T_n (in-place over T_{n-2})
+  double2* psi[kMaxMany];        // accumulator
+  int nvec;
+};
+
+int ham_prepare(spx_ham* h, double scale);   // diagonal + flip table at scale
+int launch_apply(spx_ham* h, double scale, const double2* in, double2* out, bool accumulate);
+// kind: 1 = first order (out = 0.5 acc), 2 = second order (fused psi
+// a0 T0 + c1 T1 + c2 T2), 3 = general order (T = acc - out; psi += c T)
+int launch_order(spx_ham* h, int kind, const OrderLaunch& v, double a0, double c1r, double c1i,
+                 double cr, double ci);
+int launch_axpby(spx_ctx* c, double2* y, double a, const double2* x, double br, double bi, size_t n);
+int launch_fill(spx_ctx* c, double2* y, size_t n);
+int launch_scale(spx_ctx* c, double2* y, double f, size_t n);
+int launch_negate(spx_ctx* c, double2* y, size_t n);
+int launch_add_scaled(spx_ctx* c, double2* y, double cr, double ci, const double2* x, size_t n);
+int launch_sigma(spx_ctx* c, int axis, double coef, const double2* in, size_t spin, double2* out,
+                 size_t n, bool accumulate);
+int launch_total_spin(spx_ctx* c, int axis, const double2* in, double2* out, size_t L);
+int launch_rotate(spx_ctx* c, double2* s, size_t n, uint32_t spin, const double* u8);
+int launch_two_spin(spx_ctx* c, double2* s, size_t n, uint32_t lo, uint32_t hi, const double* u32);
+
+// deterministic reductions; results land in host doubles
+int reduce_norm2(spx_ctx* c, const double2* x, size_t n, double* out);
+int reduce_inner(spx_ctx* c, const double2* self, const double2* other, size_t n, double* re,
+                 double* im);
+int reduce_corr(spx_ctx* c, int axis, const double2* state, const double2* ref, size_t L,
+                double* re, double* im);
+int reduce_mag(spx_ctx* c, int axis, uint32_t spin, const double2* state, size_t L, double* out);
+int reduce_energy(spx_ham* h, const double2* state, double* out);
+
+// tunables (environment overridable for sweeps)
+int tile_bits_for(size_t L);
+
+}  // namespace spx

@@ -1,0 +1,74 @@
+"""CPU checks of the C ABI: libspx loads, exports every symbol include/spx.h
+declares, and its host-side numerics (Bessel, Chebyshev, programs, argument
+checks) agree with the pinned oracle. No device call is made here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from spinsim_amd import spx
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "spx.h")).read()
+    return set(re.findall(r"^\s*(?:int|const char\*)\s+(spx_\w+)\s*\(", text, re.M))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = header_symbols()
+    assert len(syms) > 50
+    assert syms == set(spx.SIGNATURES), syms ^ set(spx.SIGNATURES)
+    lib = spx.lib()
+    for name in syms:
+        assert getattr(lib, name) is not None
+
+
+def test_bessel_matches_oracle_bitwise():
+    for x in (1e-6, 0.3, 1.0, 3.05, 25.0, 88.0, 350.0):
+        for kind in ("J", "I"):
+            rc, want, nzw = O.bessel(kind, x, 400)
+            if rc != 0:
+                continue
+            got, nz = spx.bessel(kind, x, 400)
+            assert np.array_equal(got, want) and nz == nzw, (kind, x)
+    with pytest.raises(spx.InvalidArgument):
+        spx.bessel("J", -1.0, 4)
+    with pytest.raises(spx.InvalidArgument):
+        spx.bessel("J", 1.0, 0)
+    with pytest.raises(spx.OverflowErr):
+        spx.bessel("I", 1000.0, 4)
+
+
+def test_chebyshev_build_matches_oracle_bitwise():
+    for mode in (0, 1):
+        for bound, tau, eps in ((7.7, 0.02, 1e-12), (19.6, 0.05, 1e-7), (3.0, -0.7, 1e-12),
+                                (150.0, 0.05, 1e-12), (1.0, 1e-12, 1e-7)):
+            rc, want, weff = O.cheb_build(mode, bound, tau, eps)
+            got, eff = spx.cheb_build(mode, bound, tau, eps)
+            assert rc == 0 and np.array_equal(got, want) and eff == weff
+
+
+def test_chebyshev_convergence_error_carries_context():
+    with pytest.raises(spx.ConvergenceError) as e:
+        spx.cheb_build(0, 1.0, 1e6, 1e-7, 100)
+    assert "tau" in str(e.value)
+    assert e.value.spectral_bound == 1.0 and e.value.epsilon == 1e-7
+    with pytest.raises(spx.InvalidArgument):
+        spx.cheb_build(0, 1.0, 0.1, 0.0)
+
+
+def test_program_counts_measurements_through_nested_repeats():
+    p = spx.Program()
+    p.measure().repeat(8).repeat(2).evolve(0.05).evolve(0.0003, fields=np.zeros((3, 3))).end()
+    p.measure().end()
+    assert p.measurements() == 9
+    q = spx.Program().repeat(3)
+    with pytest.raises(spx.InvalidArgument):
+        q.measurements()
+    with pytest.raises(spx.InvalidArgument):
+        spx.Program().end()

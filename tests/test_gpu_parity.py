@@ -1,0 +1,350 @@
+"""GPU parity of libspx (through the C ABI) against the pinned CPU oracle and
+the dense oracle. FP64 throughout; the device path uses FMA contraction and a
+one-pass gather order, so agreement is to rounding (stated per test), except
+where the arithmetic is identical by construction (diagonal, Bessel,
+Chebyshev coefficients, identity steps), which is checked bitwise."""
+import math
+
+import numpy as np
+import pytest
+
+import dense as D
+from oracle import oracle as O
+from spinsim_amd import spx
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def dipolar_system(L, index, disorder=0.5, transverse=False, anisotropic=False, angular=True):
+    c, hz, _ = O.dipolar_lattice(L, 20240726, index, min_sep=0.5, angular=angular, disorder=disorder)
+    fields = np.zeros((L, 3))
+    fields[:, 2] = hz
+    if transverse:
+        fields[:, 0] = 0.7
+        fields[:, 1] = -0.3
+    if anisotropic:
+        c.jx = c.jx * 1.25
+    return fields, c
+
+
+def device_ham(ctx, L, fields, c):
+    h = spx.Hamiltonian(ctx, L)
+    h.set_fields(fields)
+    h.set_couplings(c.lo, c.hi, c.jx, c.jy, c.jz)
+    return h
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ------------------------------------------------------------ Hamiltonian
+
+@pytest.mark.parametrize("L,transverse,aniso", [(1, True, False), (3, True, True), (6, False, True),
+                                                (9, True, False), (10, False, False),
+                                                (11, True, True), (13, False, False),
+                                                (16, True, False)])
+def test_hamiltonian_apply_matches_oracle(ctx, L, transverse, aniso):
+    banner(f"ham apply L={L}")
+    fields, c = dipolar_system(L, L, transverse=transverse, anisotropic=aniso)
+    ho = O.Ham(L, fields, c)
+    h = device_ham(ctx, L, fields, c)
+    assert h.spectral_bound() == ho.bound()
+    psi = O.random_state(L, 3 + L)
+    acc0 = O.random_state(L, 4 + L)
+    sin, sout = spx.StateVector(ctx, L, psi), spx.StateVector(ctx, L)
+    bound = ho.bound()
+    h.apply_physical(sin, sout)
+    assert rel_err(sout.download(), ho.apply_physical(psi)) < 1e-13
+    want = np.zeros_like(psi)
+    O.lib().so_ham_apply_twice_normalized(ho.h, bound, O._d(psi), O._d(want))
+    h.apply_twice_normalized(bound, sin, sout)
+    assert rel_err(sout.download(), want) < 1e-13
+    want = acc0.copy()
+    O.lib().so_ham_accumulate_twice_normalized(ho.h, bound, O._d(psi), O._d(want))
+    sout.upload(acc0)
+    h.accumulate_twice_normalized(bound, sin, sout)
+    assert rel_err(sout.download(), want) < 1e-13
+    # the sigma_z diagonal is the reference's, bit for bit
+    assert np.array_equal(h.diagonal(), ho.diag())
+
+
+def test_hamiltonian_errors(ctx):
+    h = spx.Hamiltonian(ctx, 3)
+    with pytest.raises(spx.OutOfRange):
+        h.set_field(3, 1.0)
+    with pytest.raises(spx.OutOfRange):
+        h.set_coupling(0, 3, 1, 1, 1)
+    with pytest.raises(spx.InvalidArgument):
+        h.set_coupling(1, 1, 1, 1, 1)
+    a, b = spx.StateVector(ctx, 3), spx.StateVector(ctx, 3)
+    with pytest.raises(spx.DomainError):
+        h.apply_twice_normalized(0.0, a, b)
+    with pytest.raises(spx.InvalidArgument):
+        h.apply_physical(a, spx.StateVector(ctx, 4))
+    h.set_coupling(2, 0, 0.1, 0.2, 0.3)
+    h.set_coupling(0, 2, 0.4, 0.5, 0.6)   # repeated pair overwrites in place
+    assert h.num_couplings() == 1
+
+
+# ------------------------------------------------------------ StateVector
+
+def test_state_vector_ops(ctx):
+    L = 12
+    a0, b0 = O.random_state(L, 1), O.random_state(L, 2)
+    a, b = spx.StateVector(ctx, L, a0), spx.StateVector(ctx, L, b0)
+    assert np.array_equal(a.download(), a0)
+    assert abs(a.norm() - O.lib().so_state_norm(O._d(a0), 1 << L)) < 1e-14
+    re, im = a.inner(b)
+    wr, wi = O.C.c_double(), O.C.c_double()
+    O.lib().so_state_inner(O._d(a0), O._d(b0), 1 << L, O.C.byref(wr), O.C.byref(wi))
+    assert abs(re - wr.value) < 1e-14 and abs(im - wi.value) < 1e-14
+    a.add_scaled(0.3, -0.7, b)
+    want = a0.copy()
+    O.lib().so_state_add_scaled(O._d(want), 1 << L, 0.3, -0.7, O._d(b0))
+    assert np.array_equal(a.download(), want)
+    a.negate()
+    a.scale(2.0)
+    assert np.array_equal(a.download(), -2.0 * want)
+    a.set_basis(5)
+    d = a.download()
+    assert d[10] == 1.0 and np.count_nonzero(d) == 1
+    with pytest.raises(spx.OutOfRange):
+        a.set_basis(1 << L)
+    a.set_zero()
+    with pytest.raises(spx.DomainError):
+        a.normalize()
+
+
+# ------------------------------------------------------ kernels/observables
+
+@pytest.mark.parametrize("L", [1, 4, 9, 14])
+def test_sigma_rotation_and_observables_match_oracle(ctx, L):
+    banner(f"sigma/rotation/observables L={L}")
+    psi, ref = O.random_state(L, 10 + L), O.random_state(L, 20 + L)
+    sp, sr, so = spx.StateVector(ctx, L, psi), spx.StateVector(ctx, L, ref), spx.StateVector(ctx, L)
+    for axis in range(3):
+        for spin in sorted({0, L // 2, L - 1}):
+            so.upload(ref)
+            spx.apply_sigma(axis, 0.75, sp, spin, so, True)
+            want = ref.copy()
+            O.lib().so_axpy_sigma(axis, 0.75, O._d(psi), L, spin, O._d(want))
+            assert np.array_equal(so.download(), want)
+            spx.apply_sigma(axis, 1.0, sp, spin, so, False)
+            O.lib().so_apply_sigma(axis, O._d(psi), L, spin, O._d(want))
+            assert np.array_equal(so.download(), want)
+            scratch = np.zeros_like(psi)
+            m = spx.measure_mag(axis, spin, sp)
+            assert abs(m - O.lib().so_mag(axis, spin, O._d(psi), L, O._d(scratch))) < 1e-13
+        spx.apply_total_spin(axis, sp, so)
+        want = np.zeros_like(psi)
+        O.lib().so_apply_total_spin(axis, O._d(psi), L, O._d(want))
+        assert np.array_equal(so.download(), want)
+        re, im = spx.measure_corr(axis, sp, sr)
+        wr, wi = O.C.c_double(), O.C.c_double()
+        O.lib().so_corr(axis, O._d(psi), O._d(ref), L, O._d(np.zeros_like(psi)), O.C.byref(wr),
+                        O.C.byref(wi))
+        assert abs(re - wr.value) < 1e-12 and abs(im - wi.value) < 1e-12
+    for axis, angle in (((1, 0, 0), math.pi / 2), ((0, 1, 0), math.pi), ((0.3, -0.5, 0.8), 1.7)):
+        targets = list(range(L)) if L < 9 else [0, L - 1, L // 2]
+        sp.upload(psi)
+        spx.rotate(sp, axis, angle, targets)
+        want = psi.copy()
+        t = np.array(targets, np.uint32)
+        O.lib().so_rotate(O._d(want), L, *axis, angle, O._u32(t), t.size)
+        assert rel_err(sp.download(), want) < 1e-14
+    fields, c = dipolar_system(L, 7)
+    ho, h = O.Ham(L, fields, c), device_ham(ctx, L, fields, c)
+    sp.upload(psi)
+    e = spx.measure_energy(h, sp)
+    assert abs(e - O.lib().so_energy(ho.h, O._d(psi), O._d(np.zeros_like(psi)))) < 1e-12 * max(1, abs(e))
+
+
+def test_two_spin_gate_matches_oracle(ctx):
+    L = 6
+    g = np.random.default_rng(5)
+    m = g.standard_normal((4, 4)) + 1j * g.standard_normal((4, 4))
+    U, _ = np.linalg.qr(m)
+    psi = O.random_state(L, 8)
+    for a, b in ((0, 1), (4, 1), (2, 5)):
+        s = spx.StateVector(ctx, L, psi)
+        spx.two_spin_gate(s, U, a, b)
+        want = psi.copy()
+        u = np.ascontiguousarray(U.reshape(16)).view(np.float64)
+        O.lib().so_two_spin_gate(O._d(want), L, O._d(u), a, b)
+        assert rel_err(s.download(), want) < 1e-14
+
+
+# --------------------------------------------------------------- Propagator
+
+def test_propagator_matches_exact_exponential(ctx):
+    """tests/test_Propagator.cpp:36-68 on the device: L<=6, eps 1e-12."""
+    banner("propagator vs dense L<=6")
+    for L in range(1, 7):
+        f, lo, hi, jx, jy, jz = D.sample_hamiltonian(L)
+        h = spx.Hamiltonian(ctx, L)
+        h.set_fields(f)
+        h.set_couplings(lo, hi, jx, jy, jz)
+        H = D.hamiltonian(L, f, lo, hi, jx, jy, jz)
+        psi0 = O.ref_test_state(L, 0x9001 + L)
+        prop = spx.Propagator(ctx, L, 1e-12)
+        for tau in (0.05, 0.3, 1.0):
+            s = spx.StateVector(ctx, L, psi0)
+            prop.evolve(h, spx.REAL, s, tau, 1)
+            assert D.max_abs_diff(s.download(), D.expm_apply(H, 1j * tau, psi0)) < 1e-10
+        s = spx.StateVector(ctx, L, psi0)
+        prop.evolve(h, spx.IMAGINARY, s, 0.4, 1)
+        assert D.max_abs_diff(s.download(), D.expm_apply(H, 0.4, psi0)) < 1e-10
+
+
+@pytest.mark.parametrize("L,eps,tau,steps,mode", [(8, 1e-7, 0.05, 10, 0), (10, 1e-7, 0.05, 10, 0),
+                                                  (12, 1e-12, 0.05, 4, 0), (13, 1e-7, -0.3, 2, 0),
+                                                  (14, 1e-12, 0.05, 2, 0), (11, 1e-10, 0.1, 3, 1)])
+def test_propagator_matches_oracle(ctx, L, eps, tau, steps, mode):
+    banner(f"propagator vs oracle L={L} eps={eps} tau={tau} steps={steps} mode={mode}")
+    fields, c = dipolar_system(L, 100 + L, transverse=(L == 13))
+    ho, h = O.Ham(L, fields, c), device_ham(ctx, L, fields, c)
+    psi0 = O.random_state(L, 30 + L)
+    want = psi0.copy()
+    rc, order = ho.evolve(want, tau, steps, mode, False, eps)
+    assert rc == 0
+    s = spx.StateVector(ctx, L, psi0)
+    prop = spx.Propagator(ctx, L, eps)
+    prop.evolve(h, mode, s, tau, steps)
+    assert prop.last_order() == order
+    got = s.download()
+    scale = max(1.0, float(np.max(np.abs(want))))
+    assert float(np.max(np.abs(got - want))) < 1e-12 * scale
+
+
+def test_propagator_edge_cases(ctx):
+    L = 5
+    fields, c = dipolar_system(L, 1)
+    h = device_ham(ctx, L, fields, c)
+    psi0 = O.random_state(L, 99)
+    prop = spx.Propagator(ctx, L, 1e-7)
+    s = spx.StateVector(ctx, L, psi0)
+    prop.evolve(h, spx.REAL, s, 1e-12, 1)          # identity shortcut: bitwise untouched
+    assert prop.last_order() == 0 and np.array_equal(s.download(), psi0)
+    prop.evolve(h, spx.REAL, s, 0.3, 0)            # zero steps: untouched
+    assert np.array_equal(s.download(), psi0)
+    prop.evolve(h, spx.REAL, s, 0.7, 1)
+    prop.evolve(h, spx.REAL, s, -0.7, 1)           # time reversal
+    assert D.max_abs_diff(s.download(), psi0) < 1e-6
+    with pytest.raises(spx.ConvergenceError):
+        spx.Propagator(ctx, L, 1e-7).evolve(h, spx.REAL, s, 1e5, 1)
+    with pytest.raises(spx.InvalidArgument):
+        prop.evolve(h, spx.REAL, spx.StateVector(ctx, L + 1), 0.1, 1)
+    with pytest.raises(spx.InvalidArgument):
+        spx.Propagator(ctx, L, 0.0)
+    # single-spin Rabi: <sigma_z> = -cos(Hx t) (tests/test_Propagator.cpp:126-152)
+    h1 = spx.Hamiltonian(ctx, 1)
+    h1.set_field(0, 2.0, 0.0, 0.0)
+    s1 = spx.StateVector(ctx, 1)
+    s1.set_basis(0)
+    p1 = spx.Propagator(ctx, 1, 1e-13)
+    for step in range(1, 61):
+        p1.evolve(h1, spx.REAL, s1, 0.05, 1)
+        assert abs(spx.measure_mag(2, 0, s1) + math.cos(2.0 * 0.05 * step)) < 1e-9
+
+
+def test_evolve_many_equals_separate_evolves_bitwise(ctx):
+    L = 14
+    fields, c = dipolar_system(L, 3)
+    h = device_ham(ctx, L, fields, c)
+    vecs = [O.random_state(L, 40 + i) for i in range(5)]
+    prop = spx.Propagator(ctx, L, 1e-7)
+    many = [spx.StateVector(ctx, L, v) for v in vecs]
+    prop.evolve_many(h, spx.REAL, many, 0.05, 3)
+    for v, m in zip(vecs, many):
+        one = spx.StateVector(ctx, L, v)
+        prop.evolve(h, spx.REAL, one, 0.05, 3)
+        assert np.array_equal(one.download(), m.download())
+
+
+def test_l18_single_step_matches_oracle_and_is_deterministic(ctx):
+    """Full-size L=18 (153 pairs): one Chebyshev step against the oracle, and
+    the same run twice is bitwise identical."""
+    L = 18
+    banner("L=18 step vs oracle")
+    fields, c = dipolar_system(L, 0)
+    ho, h = O.Ham(L, fields, c), device_ham(ctx, L, fields, c)
+    psi0 = O.random_state(L, 18)
+    want = psi0.copy()
+    rc, order = ho.evolve(want, 0.05, 1, 0, False, 1e-12)
+    assert rc == 0
+    prop = spx.Propagator(ctx, L, 1e-12)
+    outs = []
+    for _ in range(2):
+        s = spx.StateVector(ctx, L, psi0)
+        prop.evolve(h, spx.REAL, s, 0.05, 1)
+        outs.append(s.download())
+    assert prop.last_order() == order
+    assert np.array_equal(outs[0], outs[1])
+    assert float(np.max(np.abs(outs[0] - want))) < 1e-12
+
+
+def test_l18_conservation_and_time_reversal(ctx):
+    """Size-independent properties at L=18, eps 1e-12, autonormalise off:
+    norm and energy drift over 200 steps, and +tau then -tau returns."""
+    L = 18
+    banner("L=18 conservation")
+    fields, c = dipolar_system(L, 1)
+    h = device_ham(ctx, L, fields, c)
+    psi0 = O.random_state(L, 181)
+    s = spx.StateVector(ctx, L, psi0)
+    e0, n0 = spx.measure_energy(h, s), s.norm()
+    prop = spx.Propagator(ctx, L, 1e-12)
+    prop.evolve(h, spx.REAL, s, 0.05, 200)
+    assert abs(s.norm() - n0) < 1e-11
+    assert abs(spx.measure_energy(h, s) - e0) < 1e-10 * max(1.0, abs(e0))
+    prop.evolve(h, spx.REAL, s, -0.05, 200)
+    assert D.max_abs_diff(s.download(), psi0) < 1e-10
+
+
+# ------------------------------------------------------------------ Ensemble
+
+def realisation_inputs(L, seed, index, disorder=0.5, min_sep=0.5, angular=True):
+    """runRealization's draws (src/io/Experiment.cpp:10-55) from the oracle's
+    pinned stream: geometry, Hz disorder, random initial state."""
+    c, hz, r = O.dipolar_lattice(L, seed, index, min_sep=min_sep, angular=angular, disorder=disorder)
+    psi0 = np.zeros(2 << L)
+    assert O.lib().so_prepare_initial_state(O._d(psi0), L, 0, O.C.byref(r), 0) == 0
+    fields = np.zeros((L, 3))
+    fields[:, 2] = hz
+    return fields, c, psi0
+
+
+def cpmg(P, L, cycles=8):
+    allspins = list(range(L))
+    P.pulse((1, 0, 0), math.pi / 2, allspins)
+    P.repeat(cycles)
+    P.evolve(0.05, 10)
+    P.pulse((0, 1, 0), math.pi, allspins)
+    P.evolve(0.05, 10)
+    P.measure()
+    P.end()
+    return P
+
+
+def test_config0_cpmg_echoes_match_oracle_per_realisation(ctx):
+    """Plain CPMG echo, L=10, 20 realisations (20240726, 0..19): per-realisation
+    echo curves within 1e-10 of the CPU path, and the ensemble statistics."""
+    banner("config0 CPMG ensemble")
+    L, n = 10, 20
+    obs = ((O.OBS_CORR_X, 0), (O.OBS_NORM, 0), (O.OBS_ENERGY, 0))
+    cfg = O.Config(L=L, program=cpmg(O.Program(), L), observables=obs, field_disorder=0.5,
+                   min_separation=0.5, angular=True)
+    times, per, mean, sd, se = O.run_ensemble(cfg, n, threads=8)
+    b = spx.Batch(ctx, cpmg(spx.Program(), L), obs, epsilon=1e-7, reference_axis=0)
+    for i in range(n):
+        fields, c, psi0 = realisation_inputs(L, 20240726, i)
+        b.add(L, fields, c.lo, c.hi, c.jx, c.jy, c.jz, psi0)
+    gt, gv = b.run()
+    assert np.array_equal(gt, times)
+    assert float(np.max(np.abs(gv - per))) < 1e-10
+    assert float(np.max(np.abs(gv.mean(axis=0) - mean))) < 1e-10
