@@ -139,9 +139,10 @@ int spx_state_create(spx_ctx* c, size_t nspins, spx_state** out) {
   return SPX_OK;
 }
 
+// Destroy functions do not touch the owning context (it may already be
+// gone); hipFree waits for outstanding work on the buffer itself.
 int spx_state_destroy(spx_state* s) {
   if (!s) return SPX_OK;
-  hipStreamSynchronize(s->ctx->stream);
   hipFree(s->d);
   delete s;
   return SPX_OK;
@@ -263,10 +264,10 @@ int spx_ham_create(spx_ctx* c, size_t nspins, spx_ham** out) {
 
 int spx_ham_destroy(spx_ham* h) {
   if (!h) return SPX_OK;
-  hipStreamSynchronize(h->ctx->stream);
   if (h->d_diag) hipFree(h->d_diag);
   if (h->d_terms) hipFree(h->d_terms);
   if (h->d_tab) hipFree(h->d_tab);
+  if (h->d_scratch) hipFree(h->d_scratch);
   delete h;
   return SPX_OK;
 }
@@ -368,7 +369,6 @@ int spx_prop_create(spx_ctx* c, size_t nspins, double epsilon, spx_prop** out) {
 
 int spx_prop_destroy(spx_prop* p) {
   if (!p) return SPX_OK;
-  hipStreamSynchronize(p->ctx->stream);
   for (auto& d : p->d_scratch)
     if (d) hipFree(d);
   delete p;

@@ -90,11 +90,15 @@ struct spx_ham {
   spx::FlipTerm* d_terms = nullptr;
   size_t terms_cap = 0;
   int nterms = 0;
-  int terms_nfield = 0, terms_nlocal = 0, terms_nremote = 0, terms_tb = 0;
+  int terms_nlocal = 0, terms_nboth = 0, terms_g = 0, terms_tb = 0;
+  std::vector<int> terms_grp;   // v2 group offsets
+  std::vector<int> terms_off;   // v3 segment ends
+  int terms_variant = 0;
   double terms_scale = 0.0;
   bool terms_valid = false;
   std::vector<spx::FlipTerm> h_terms;
   double* d_tab = nullptr;   // diagonal build table
+  double2* d_scratch = nullptr;  // H psi for the energy observable
   size_t tab_cap = 0;
 };
 
@@ -149,5 +153,7 @@ int reduce_energy(spx_ham* h, const double2* state, double* out);
 
 // tunables (environment overridable for sweeps)
 int tile_bits_for(size_t L);
+int block_threads();
+int xcd_remap();
 
 }  // namespace spx

@@ -26,9 +26,17 @@ struct KArgs {
   double2* out[kMaxMany];
   double2* psi[kMaxMany];
   const double* diag;
-  const FlipTerm* terms;  // [nfield field terms][nlocal local pairs][nremote remote pairs]
-  int nfield, nlocal, nremote;
+  // Flip-term table, grouped by where the partner amplitude lives relative to
+  // a tile of 2^tb consecutive amplitudes:
+  //   [0, nlocal)                       both bits (or the field bit) < tb: own tile in LDS
+  //   [grp[r], grp[r+1]) for r < g      one bit is tb + r: neighbour tile x ^ (1 << r),
+  //                                     mask = the local bit (0 for a field on tb + r)
+  //   [grp[g], grp[g] + nboth)          both bits >= tb: mask in tile-index space
+  const FlipTerm* terms;
+  int nlocal, g, nboth;
+  int grp[kMaxSpins + 1];
   int tb;
+  int xcd_remap;
   double sdiag;
   double a0, c1r, c1i, cr, ci;
 };
@@ -50,77 +58,278 @@ __device__ __forceinline__ void field_term(double2& acc, bool up, double cx, dou
   acc.y = fma(sy, p.x, acc.y);
 }
 
-template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_horder(KArgs a) {
-  extern __shared__ double2 tile[];
+// One H application (or fused Chebyshev order) over a tile of 2^tb
+// amplitudes; each thread owns AMPS amplitudes j = tid + r * blockDim.x.
+template <int KIND, int AMPS>
+__global__ __launch_bounds__(1024) void k_horder(KArgs a) {
+  extern __shared__ double2 smem[];
+  const uint32_t T = 1u << a.tb;
+  const uint32_t B = blockDim.x;
+  double2* own = smem;
+  double2* nb = smem + T;
   const int v = blockIdx.y;
   const double2* __restrict__ in = a.in[v];
-  const uint32_t tsize = 1u << a.tb;
-  const uint32_t lmask = tsize - 1u;
-  const size_t base = static_cast<size_t>(blockIdx.x) << a.tb;
-  for (uint32_t j = threadIdx.x; j < tsize; j += kBlock) tile[j] = in[base + j];
+
+  // Tile index. Workgroups are dealt to the XCDs round-robin; with the remap
+  // the three most significant tile bits select the XCD, so every pair
+  // among the other bits finds its partner tile on the same XCD's L2.
+  uint32_t x = blockIdx.x;
+  if (a.xcd_remap && a.g >= 3) x = ((x & 7u) << (a.g - 3)) | (x >> 3);
+  const size_t base = static_cast<size_t>(x) << a.tb;
+
+  double2 acc[AMPS], xv[AMPS], pre[AMPS];
+  uint32_t jj[AMPS];
+#pragma unroll
+  for (int r = 0; r < AMPS; ++r) {
+    jj[r] = threadIdx.x + r * B;
+    xv[r] = in[base + jj[r]];
+  }
+  if (a.g > 0) {
+    const size_t nbase = static_cast<size_t>(x ^ 1u) << a.tb;
+#pragma unroll
+    for (int r = 0; r < AMPS; ++r) pre[r] = in[nbase + jj[r]];
+  }
+#pragma unroll
+  for (int r = 0; r < AMPS; ++r) {
+    own[jj[r]] = xv[r];
+    const double f = a.sdiag * a.diag[base + jj[r]];
+    acc[r] = make_double2(f * xv[r].x, f * xv[r].y);
+  }
   __syncthreads();
 
-  const FlipTerm* __restrict__ fterm = a.terms;
-  const FlipTerm* __restrict__ lterm = a.terms + a.nfield;
-  const FlipTerm* __restrict__ rterm = a.terms + a.nfield + a.nlocal;
-
-  for (uint32_t j = threadIdx.x; j < tsize; j += kBlock) {
-    const size_t k = base + j;
-    const double2 x = tile[j];
-    const double f = a.sdiag * a.diag[k];
-    double2 acc = make_double2(f * x.x, f * x.y);
-
-    for (int t = 0; t < a.nfield; ++t) {
-      const FlipTerm ft = fterm[t];
-      const double2 p = (ft.mask & ~lmask) ? in[k ^ ft.mask] : tile[j ^ ft.mask];
-      field_term(acc, (k & ft.mask) != 0, ft.anti, ft.aligned, p);
+  const FlipTerm* __restrict__ tt = a.terms;
+  for (int t = 0; t < a.nlocal; ++t) {
+    const FlipTerm ft = tt[t];
+    if (ft.kind) {
+#pragma unroll
+      for (int r = 0; r < AMPS; ++r)
+        field_term(acc[r], (jj[r] & ft.mask) != 0, ft.anti, ft.aligned, own[jj[r] ^ ft.mask]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < AMPS; ++r) {
+        const double c = (__popc(jj[r] & ft.mask) == 1) ? ft.anti : ft.aligned;
+        if (c != 0.0) cmad(acc[r], c, own[jj[r] ^ ft.mask]);
+      }
     }
-#pragma unroll 4
-    for (int t = 0; t < a.nlocal; ++t) {
-      const FlipTerm ft = lterm[t];
-      const bool anti = __popc(static_cast<uint32_t>(k) & ft.mask) == 1;
-      const double c = anti ? ft.anti : ft.aligned;
-      if (c != 0.0) cmad(acc, c, tile[j ^ ft.mask]);
-    }
-#pragma unroll 4
-    for (int t = 0; t < a.nremote; ++t) {
-      const FlipTerm ft = rterm[t];
-      const bool anti = __popc(static_cast<uint32_t>(k) & ft.mask) == 1;
-      const double c = anti ? ft.anti : ft.aligned;
-      if (c != 0.0) cmad(acc, c, in[k ^ ft.mask]);
-    }
+  }
 
+  for (int rb = 0; rb < a.g; ++rb) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < AMPS; ++r) nb[jj[r]] = pre[r];
+    __syncthreads();
+    if (rb + 1 < a.g) {
+      const size_t nbase = static_cast<size_t>(x ^ (1u << (rb + 1))) << a.tb;
+#pragma unroll
+      for (int r = 0; r < AMPS; ++r) pre[r] = in[nbase + jj[r]];
+    }
+    const uint32_t xb = (x >> rb) & 1u;
+    for (int t = a.grp[rb]; t < a.grp[rb + 1]; ++t) {
+      const FlipTerm ft = tt[t];
+      if (ft.kind) {
+#pragma unroll
+        for (int r = 0; r < AMPS; ++r) field_term(acc[r], xb != 0, ft.anti, ft.aligned, nb[jj[r]]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < AMPS; ++r) {
+          const bool anti = ((jj[r] & ft.mask) != 0) != (xb != 0);
+          const double c = anti ? ft.anti : ft.aligned;
+          if (c != 0.0) cmad(acc[r], c, nb[jj[r] ^ ft.mask]);
+        }
+      }
+    }
+  }
+
+  const int b0 = a.grp[a.g];
+  for (int t = 0; t < a.nboth; ++t) {
+    const FlipTerm ft = tt[b0 + t];
+    const double c = (__popc(x & ft.mask) == 1) ? ft.anti : ft.aligned;  // uniform per tile
+    if (c != 0.0) {
+      const size_t pbase = static_cast<size_t>(x ^ ft.mask) << a.tb;
+#pragma unroll
+      for (int r = 0; r < AMPS; ++r) cmad(acc[r], c, in[pbase + jj[r]]);
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < AMPS; ++r) {
+    const size_t k = base + jj[r];
+    const double2 ac = acc[r];
     if constexpr (KIND == K_APPLY) {
-      a.out[v][k] = acc;
+      a.out[v][k] = ac;
     } else if constexpr (KIND == K_ACCUM) {
       double2 o = a.out[v][k];
-      o.x += acc.x;
-      o.y += acc.y;
+      o.x += ac.x;
+      o.y += ac.y;
       a.out[v][k] = o;
     } else if constexpr (KIND == K_ORDER1) {
       // T_1 = Hhat T_0: the general recurrence's 2 Hhat T_0 halved.
-      a.out[v][k] = make_double2(0.5 * acc.x, 0.5 * acc.y);
+      a.out[v][k] = make_double2(0.5 * ac.x, 0.5 * ac.y);
     } else if constexpr (KIND == K_ORDER2) {
       // psi still holds T_0; psi = a0 T_0 + c1 T_1 + c2 T_2.
       const double2 t0 = a.psi[v][k];
-      const double2 t2 = make_double2(acc.x - t0.x, acc.y - t0.y);
+      const double2 t1 = xv[r];
+      const double2 t2 = make_double2(ac.x - t0.x, ac.y - t0.y);
       a.out[v][k] = t2;
       double2 p = make_double2(a.a0 * t0.x, a.a0 * t0.y);
-      p.x += a.c1r * x.x - a.c1i * x.y;
-      p.y += a.c1r * x.y + a.c1i * x.x;
+      p.x += a.c1r * t1.x - a.c1i * t1.y;
+      p.y += a.c1r * t1.y + a.c1i * t1.x;
       p.x += a.cr * t2.x - a.ci * t2.y;
       p.y += a.cr * t2.y + a.ci * t2.x;
       a.psi[v][k] = p;
     } else {
       const double2 tm2 = a.out[v][k];
-      const double2 tn = make_double2(acc.x - tm2.x, acc.y - tm2.y);
+      const double2 tn = make_double2(ac.x - tm2.x, ac.y - tm2.y);
       a.out[v][k] = tn;
       double2 p = a.psi[v][k];
       p.x += a.cr * tn.x - a.ci * tn.y;
       p.y += a.cr * tn.y + a.ci * tn.x;
       a.psi[v][k] = p;
     }
+  }
+}
+
+// ------------------------------------------------------------ order kernel v3
+// One amplitude per thread and a tile of T = blockDim.x <= 1024 amplitudes,
+// so index bits 0-5 are lane bits and bits 6..tb-1 are the same for a whole
+// wavefront. Every term whose activity (flip-flop: the two spins
+// anti-aligned) does not involve a lane bit is decided once per wavefront
+// (scalar branch) instead of per lane. The own tile and all g neighbour
+// tiles (tile ^ one remote bit) are loaded together into LDS behind one
+// barrier; pairs with both bits remote read their partner tile from global
+// memory in batches of eight loads in flight.
+constexpr int kMaxG3 = 12;
+
+struct K3Args {
+  const double2* in[kMaxMany];
+  double2* out[kMaxMany];
+  double2* psi[kMaxMany];
+  const double* diag;
+  const FlipTerm* terms;
+  // segment ends: [0] local fields, [1] wave-uniform local pairs, [2] per-lane
+  // local pairs, then per remote bit r: [3+3r] field on tb+r, [4+3r]
+  // wave-uniform (local bit >= 6), [5+3r] per-lane; [3+3g] pairs with both
+  // bits remote (tile-index masks)
+  int off[3 + 3 * kMaxG3 + 1];
+  int tb, g;
+  int xcd_remap;
+  double sdiag;
+  double a0, c1r, c1i, cr, ci;
+};
+
+template <int KIND, int G>
+__global__ __launch_bounds__(1024) void k_order3(K3Args a) {
+  extern __shared__ double2 smem[];
+  const uint32_t T = blockDim.x;
+  const uint32_t j = threadIdx.x;
+  const uint32_t wj = __builtin_amdgcn_readfirstlane(j) & ~63u;
+  const int v = blockIdx.y;
+  const double2* __restrict__ in = a.in[v];
+  uint32_t x = blockIdx.x;
+  if (a.xcd_remap && G >= 3) x = ((x & 7u) << (G - 3)) | (x >> 3);
+  const size_t base = static_cast<size_t>(x) << a.tb;
+  const size_t k = base + j;
+
+  // every global operand of this thread, issued up front
+  const double2 xv = in[k];
+  double2 nbv[G > 0 ? G : 1];
+#pragma unroll
+  for (int r = 0; r < G; ++r) nbv[r] = in[(static_cast<size_t>(x ^ (1u << r)) << a.tb) + j];
+  const double dg = a.diag[k];
+  double2 prev = make_double2(0.0, 0.0), psiv = make_double2(0.0, 0.0);
+  if constexpr (KIND == K_ACCUM || KIND == K_ORDERN) prev = a.out[v][k];
+  if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN) psiv = a.psi[v][k];
+
+  smem[j] = xv;
+#pragma unroll
+  for (int r = 0; r < G; ++r) smem[(1 + r) * T + j] = nbv[r];
+  __syncthreads();
+
+  const double2* own = smem;
+  const FlipTerm* __restrict__ tt = a.terms;
+  const double f = a.sdiag * dg;
+  double2 acc = make_double2(f * xv.x, f * xv.y);
+
+  for (int t = 0; t < a.off[0]; ++t) {
+    const FlipTerm ft = tt[t];
+    field_term(acc, (j & ft.mask) != 0, ft.anti, ft.aligned, own[j ^ ft.mask]);
+  }
+  for (int t = a.off[0]; t < a.off[1]; ++t) {
+    const FlipTerm ft = tt[t];
+    const double c = (__popc(wj & ft.mask) == 1) ? ft.anti : ft.aligned;
+    if (c != 0.0) cmad(acc, c, own[j ^ ft.mask]);
+  }
+#pragma unroll 4
+  for (int t = a.off[1]; t < a.off[2]; ++t) {
+    const FlipTerm ft = tt[t];
+    const double c = (__popc(j & ft.mask) == 1) ? ft.anti : ft.aligned;
+    if (c != 0.0) cmad(acc, c, own[j ^ ft.mask]);
+  }
+  for (int r = 0; r < G; ++r) {
+    const double2* nb = smem + (1 + r) * T;
+    const bool xb = ((x >> r) & 1u) != 0;
+    const int s0 = a.off[2 + 3 * r], s1 = a.off[3 + 3 * r], s2 = a.off[4 + 3 * r],
+              s3 = a.off[5 + 3 * r];
+    for (int t = s0; t < s1; ++t) {
+      const FlipTerm ft = tt[t];
+      field_term(acc, xb, ft.anti, ft.aligned, nb[j]);
+    }
+    for (int t = s1; t < s2; ++t) {
+      const FlipTerm ft = tt[t];
+      const double c = (((wj & ft.mask) != 0) != xb) ? ft.anti : ft.aligned;
+      if (c != 0.0) cmad(acc, c, nb[j ^ ft.mask]);
+    }
+#pragma unroll 4
+    for (int t = s2; t < s3; ++t) {
+      const FlipTerm ft = tt[t];
+      const double c = (((j & ft.mask) != 0) != xb) ? ft.anti : ft.aligned;
+      if (c != 0.0) cmad(acc, c, nb[j ^ ft.mask]);
+    }
+  }
+  const int e0 = a.off[2 + 3 * G], e1 = a.off[3 + 3 * G];
+  // batches of eight partner loads in flight before their FMAs; named
+  // registers, not an array, so nothing goes to scratch
+#define SPX_BOTH_LOAD(u)                                                          \
+  double c##u = 0.0;                                                              \
+  double2 p##u = make_double2(0.0, 0.0);                                          \
+  if (t0 + u < e1) {                                                              \
+    const FlipTerm ft = tt[t0 + u];                                               \
+    c##u = (__popc(x & ft.mask) == 1) ? ft.anti : ft.aligned;                     \
+    if (c##u != 0.0) p##u = in[(static_cast<size_t>(x ^ ft.mask) << a.tb) + j];   \
+  }
+#define SPX_BOTH_FMA(u) \
+  if (c##u != 0.0) cmad(acc, c##u, p##u);
+  for (int t0 = e0; t0 < e1; t0 += 8) {
+    SPX_BOTH_LOAD(0) SPX_BOTH_LOAD(1) SPX_BOTH_LOAD(2) SPX_BOTH_LOAD(3)
+    SPX_BOTH_LOAD(4) SPX_BOTH_LOAD(5) SPX_BOTH_LOAD(6) SPX_BOTH_LOAD(7)
+    SPX_BOTH_FMA(0) SPX_BOTH_FMA(1) SPX_BOTH_FMA(2) SPX_BOTH_FMA(3)
+    SPX_BOTH_FMA(4) SPX_BOTH_FMA(5) SPX_BOTH_FMA(6) SPX_BOTH_FMA(7)
+  }
+#undef SPX_BOTH_LOAD
+#undef SPX_BOTH_FMA
+
+  if constexpr (KIND == K_APPLY) {
+    a.out[v][k] = acc;
+  } else if constexpr (KIND == K_ACCUM) {
+    a.out[v][k] = make_double2(prev.x + acc.x, prev.y + acc.y);
+  } else if constexpr (KIND == K_ORDER1) {
+    a.out[v][k] = make_double2(0.5 * acc.x, 0.5 * acc.y);
+  } else if constexpr (KIND == K_ORDER2) {
+    const double2 t0 = psiv, t1 = xv;
+    const double2 t2 = make_double2(acc.x - t0.x, acc.y - t0.y);
+    a.out[v][k] = t2;
+    double2 q = make_double2(a.a0 * t0.x, a.a0 * t0.y);
+    q.x += a.c1r * t1.x - a.c1i * t1.y;
+    q.y += a.c1r * t1.y + a.c1i * t1.x;
+    q.x += a.cr * t2.x - a.ci * t2.y;
+    q.y += a.cr * t2.y + a.ci * t2.x;
+    a.psi[v][k] = q;
+  } else {
+    const double2 tn = make_double2(acc.x - prev.x, acc.y - prev.y);
+    a.out[v][k] = tn;
+    psiv.x += a.cr * tn.x - a.ci * tn.y;
+    psiv.y += a.cr * tn.y + a.ci * tn.x;
+    a.psi[v][k] = psiv;
   }
 }
 
@@ -153,6 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_fill(double2* y, size_t n) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale(double2* y, double f, size_t n) {
+#pragma clang fp contract(off)
   const size_t k = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (k < n) {
     double2 v = y[k];
@@ -168,9 +378,11 @@ __global__ __launch_bounds__(kBlock) void k_negate(double2* y, size_t n) {
   }
 }
 
-// addScaled (src/core/StateVector.cpp:21-32).
+// addScaled (src/core/StateVector.cpp:21-32). The BLAS-1 kernels keep the
+// reference's separate multiply and add (no FMA), so they are bitwise its own.
 __global__ __launch_bounds__(kBlock) void k_add_scaled(double2* __restrict__ y, double cr, double ci,
                                                        const double2* __restrict__ x, size_t n) {
+#pragma clang fp contract(off)
   const size_t k = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (k < n) {
     const double2 o = x[k];
@@ -185,6 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_add_scaled(double2* __restrict__ y, 
 __global__ __launch_bounds__(kBlock) void k_axpby(double2* __restrict__ y, double a,
                                                   const double2* __restrict__ x, double br,
                                                   double bi, size_t n) {
+#pragma clang fp contract(off)
   const size_t k = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (k < n) {
     const double2 o = x[k];
@@ -206,6 +419,7 @@ __device__ __forceinline__ size_t insert_zero(size_t q, uint32_t bit) {
 __global__ __launch_bounds__(kBlock) void k_sigma(int axis, double c, const double2* __restrict__ in,
                                                   uint32_t spin, double2* __restrict__ out,
                                                   size_t half, int accumulate) {
+#pragma clang fp contract(off)
   const size_t q = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (q >= half) return;
   const size_t l = insert_zero(q, spin);
@@ -284,6 +498,7 @@ struct Rot {
 // SpinRotation::apply for one target (src/core/Gate.cpp:60-85).
 __global__ __launch_bounds__(kBlock) void k_rotate(double2* __restrict__ s, size_t half, uint32_t spin,
                                                    Rot u) {
+#pragma clang fp contract(off)
   const size_t q = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (q >= half) return;
   const size_t l = insert_zero(q, spin);
@@ -304,6 +519,7 @@ struct Gate4 {
 // TwoSpinGate::apply (src/core/Gate.cpp:90-121).
 __global__ __launch_bounds__(kBlock) void k_two_spin(double2* __restrict__ s, size_t quarter,
                                                      uint32_t lo, uint32_t hi, Gate4 u) {
+#pragma clang fp contract(off)
   const size_t q = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (q >= quarter) return;
   const size_t dd = insert_zero(insert_zero(q, lo), hi);
@@ -378,29 +594,6 @@ struct OpMag {  // Re <state|sigma(spin)|state>
   }
 };
 
-struct OpEnergy {  // Re <H state|state>
-  const double2* x;
-  const double* diag;
-  const FlipTerm* terms;
-  int nfield, npair;
-  __device__ double2 operator()(size_t k) const {
-    const double2 v = x[k];
-    const double f = diag[k];
-    double2 acc = make_double2(f * v.x, f * v.y);
-    for (int t = 0; t < nfield; ++t) {
-      const FlipTerm ft = terms[t];
-      field_term(acc, (k & ft.mask) != 0, ft.anti, ft.aligned, x[k ^ ft.mask]);
-    }
-    for (int t = 0; t < npair; ++t) {
-      const FlipTerm ft = terms[nfield + t];
-      const bool anti = __popc(static_cast<uint32_t>(k) & ft.mask) == 1;
-      const double c = anti ? ft.anti : ft.aligned;
-      if (c != 0.0) cmad(acc, c, x[k ^ ft.mask]);
-    }
-    return make_double2(acc.x * v.x + acc.y * v.y, 0.0);
-  }
-};
-
 __device__ __forceinline__ void block_tree(double* lr, double* li, double sr, double si) {
   const int t = threadIdx.x;
   lr[t] = sr;
@@ -469,16 +662,31 @@ int reduce(spx_ctx* c, Op op, size_t n, double* re, double* im) {
 
 // --------------------------------------------------------------- launchers
 
+static int env_int(const char* name, int fallback) {
+  const char* e = std::getenv(name);
+  return e ? std::atoi(e) : fallback;
+}
+
+// Tile bits (2^tb amplitudes staged in LDS per workgroup), threads per
+// workgroup and the XCD-aware tile order; SPX_TILE_BITS / SPX_BLOCK /
+// SPX_XCD_REMAP override them for sweeps.
 int tile_bits_for(size_t L) {
-  static int forced = [] {
-    const char* e = std::getenv("SPX_TILE_BITS");
-    return e ? std::atoi(e) : 0;
-  }();
+  static const int forced = env_int("SPX_TILE_BITS", 0);
   int tb = forced > 0 ? forced : 10;
   if (tb > 12) tb = 12;
   if (tb < 6) tb = 6;
   if (static_cast<size_t>(tb) > L) tb = static_cast<int>(L);
   return tb;
+}
+
+int block_threads() {
+  static const int b = env_int("SPX_BLOCK", 256);
+  return b >= 64 && b <= 1024 ? b : 256;
+}
+
+int xcd_remap() {
+  static const int r = env_int("SPX_XCD_REMAP", 0);
+  return r;
 }
 
 // Diagonal (when hz or the couplings changed) and the flip table at `scale`,
@@ -516,6 +724,7 @@ int ham_prepare(spx_ham* h, double scale) {
                              hipMemcpyHostToDevice, c->stream));
     k_build_diag<<<blocks_for(n), kBlock, 0, c->stream>>>(h->d_diag, n, h->d_tab, nf, np);
     SPX_HIP(hipGetLastError());
+    SPX_HIP(hipStreamSynchronize(c->stream));  // `tab` is pageable and local
     h->diag_valid = true;
     h->diag_hz = h->hz;
     h->diag_jz = h->jz;
@@ -526,13 +735,65 @@ int ham_prepare(spx_ham* h, double scale) {
   if (h->terms_valid && h->terms_scale == scale) return SPX_OK;
 
   const int tb = tile_bits_for(L);
-  const uint32_t lmask = (1u << tb) - 1u;
-  std::vector<FlipTerm> fields, local, remote;
+  const int g = static_cast<int>(L) - tb;
+  if (tb <= 10 && g <= kMaxG3 &&
+      static_cast<size_t>(1 + (g > 0 ? g : 0)) * (sizeof(double2) << tb) <= 160 * 1024) {
+    // v3 layout (see K3Args::off)
+    std::vector<FlipTerm> seg[3];
+    std::vector<std::vector<FlipTerm>> rseg(3 * (g > 0 ? g : 0));
+    std::vector<FlipTerm> both;
+    for (size_t s = 0; s < L; ++s) {
+      const double cx = scale * 0.5 * h->hx[s];
+      const double cy = scale * 0.5 * h->hy[s];
+      if (!(std::abs(cx) > kNegligible || std::abs(cy) > kNegligible)) continue;
+      if (static_cast<int>(s) < tb) seg[0].push_back(FlipTerm{1u << s, 1u, cx, cy, 0.0});
+      else rseg[3 * (s - tb)].push_back(FlipTerm{0u, 1u, cx, cy, 0.0});
+    }
+    for (size_t p = 0; p < h->lo.size(); ++p) {
+      const double quarter = scale * 0.25;
+      double ff = quarter * (h->jx[p] + h->jy[p]);
+      double fb = quarter * (h->jx[p] - h->jy[p]);
+      if (!(std::abs(ff) > kNegligible)) ff = 0.0;
+      if (!(std::abs(fb) > kNegligible)) fb = 0.0;
+      if (ff == 0.0 && fb == 0.0) continue;
+      const int lo = static_cast<int>(h->lo[p]), hi = static_cast<int>(h->hi[p]);
+      if (hi < tb) {
+        seg[lo >= 6 ? 1 : 2].push_back(FlipTerm{(1u << lo) | (1u << hi), 0u, ff, fb, 0.0});
+      } else if (lo < tb) {
+        rseg[3 * (hi - tb) + (lo >= 6 ? 1 : 2)].push_back(FlipTerm{1u << lo, 0u, ff, fb, 0.0});
+      } else {
+        both.push_back(FlipTerm{(1u << (lo - tb)) | (1u << (hi - tb)), 0u, ff, fb, 0.0});
+      }
+    }
+    h->h_terms.clear();
+    h->terms_off.clear();
+    for (auto& sg : seg) {
+      h->h_terms.insert(h->h_terms.end(), sg.begin(), sg.end());
+      h->terms_off.push_back(static_cast<int>(h->h_terms.size()));
+    }
+    for (auto& sg : rseg) {
+      h->h_terms.insert(h->h_terms.end(), sg.begin(), sg.end());
+      h->terms_off.push_back(static_cast<int>(h->h_terms.size()));
+    }
+    h->h_terms.insert(h->h_terms.end(), both.begin(), both.end());
+    h->terms_off.push_back(static_cast<int>(h->h_terms.size()));
+    h->terms_variant = 3;
+    h->nterms = static_cast<int>(h->h_terms.size());
+    h->terms_g = g > 0 ? g : 0;
+    h->terms_tb = tb;
+    h->terms_scale = scale;
+    h->terms_valid = true;
+  } else {
+  std::vector<FlipTerm> local;
+  std::vector<std::vector<FlipTerm>> group(g > 0 ? g : 0);
+  std::vector<FlipTerm> both;
+  // local fields first, then pairs, each in the reference's order
   for (size_t s = 0; s < L; ++s) {
     const double cx = scale * 0.5 * h->hx[s];
     const double cy = scale * 0.5 * h->hy[s];
-    if (std::abs(cx) > kNegligible || std::abs(cy) > kNegligible)
-      fields.push_back(FlipTerm{1u << s, 1u, cx, cy, 0.0});
+    if (!(std::abs(cx) > kNegligible || std::abs(cy) > kNegligible)) continue;
+    if (static_cast<int>(s) < tb) local.push_back(FlipTerm{1u << s, 1u, cx, cy, 0.0});
+    else group[s - tb].push_back(FlipTerm{0u, 1u, cx, cy, 0.0});
   }
   for (size_t p = 0; p < h->lo.size(); ++p) {
     const double quarter = scale * 0.25;
@@ -541,57 +802,149 @@ int ham_prepare(spx_ham* h, double scale) {
     if (!(std::abs(ff) > kNegligible)) ff = 0.0;
     if (!(std::abs(fb) > kNegligible)) fb = 0.0;
     if (ff == 0.0 && fb == 0.0) continue;
-    const uint32_t m = (1u << h->lo[p]) | (1u << h->hi[p]);
-    ((m & ~lmask) ? remote : local).push_back(FlipTerm{m, 0u, ff, fb, 0.0});
+    const int lo = static_cast<int>(h->lo[p]), hi = static_cast<int>(h->hi[p]);
+    if (hi < tb) local.push_back(FlipTerm{(1u << lo) | (1u << hi), 0u, ff, fb, 0.0});
+    else if (lo < tb) group[hi - tb].push_back(FlipTerm{1u << lo, 0u, ff, fb, 0.0});
+    else both.push_back(FlipTerm{(1u << (lo - tb)) | (1u << (hi - tb)), 0u, ff, fb, 0.0});
   }
   h->h_terms.clear();
-  h->h_terms.insert(h->h_terms.end(), fields.begin(), fields.end());
   h->h_terms.insert(h->h_terms.end(), local.begin(), local.end());
-  h->h_terms.insert(h->h_terms.end(), remote.begin(), remote.end());
+  h->terms_grp.assign(1, static_cast<int>(h->h_terms.size()));
+  for (int r = 0; r < g; ++r) {
+    h->h_terms.insert(h->h_terms.end(), group[r].begin(), group[r].end());
+    h->terms_grp.push_back(static_cast<int>(h->h_terms.size()));
+  }
+  h->h_terms.insert(h->h_terms.end(), both.begin(), both.end());
   h->nterms = static_cast<int>(h->h_terms.size());
-  h->terms_nfield = static_cast<int>(fields.size());
   h->terms_nlocal = static_cast<int>(local.size());
-  h->terms_nremote = static_cast<int>(remote.size());
+  h->terms_nboth = static_cast<int>(both.size());
+  h->terms_g = g > 0 ? g : 0;
   h->terms_tb = tb;
   h->terms_scale = scale;
   h->terms_valid = true;
+  h->terms_variant = 2;
+  }
   const size_t bytes = (h->h_terms.size() + 1) * sizeof(FlipTerm);
   if (bytes > h->terms_cap) {
     if (h->d_terms) SPX_HIP(hipFree(h->d_terms));
     SPX_HIP(hipMalloc(&h->d_terms, bytes));
     h->terms_cap = bytes;
   }
-  if (!h->h_terms.empty())
+  if (!h->h_terms.empty()) {
     SPX_HIP(hipMemcpyAsync(h->d_terms, h->h_terms.data(), h->h_terms.size() * sizeof(FlipTerm),
                            hipMemcpyHostToDevice, c->stream));
+    SPX_HIP(hipStreamSynchronize(c->stream));  // pageable source, rewritten on the next change
+  }
   static_assert(sizeof(FlipTerm) == 32, "flip term layout");
   return SPX_OK;
 }
 
-static int launch_k(spx_ham* h, int kind, const KArgs& a, unsigned nvec) {
-  spx_ctx* c = h->ctx;
-  const size_t n = static_cast<size_t>(1) << h->nspins;
-  const dim3 grid(static_cast<unsigned>(n >> a.tb), nvec);
-  const size_t lds = sizeof(double2) << a.tb;
-  switch (kind) {
-    case K_APPLY: k_horder<K_APPLY><<<grid, kBlock, lds, c->stream>>>(a); break;
-    case K_ACCUM: k_horder<K_ACCUM><<<grid, kBlock, lds, c->stream>>>(a); break;
-    case K_ORDER1: k_horder<K_ORDER1><<<grid, kBlock, lds, c->stream>>>(a); break;
-    case K_ORDER2: k_horder<K_ORDER2><<<grid, kBlock, lds, c->stream>>>(a); break;
-    default: k_horder<K_ORDERN><<<grid, kBlock, lds, c->stream>>>(a); break;
+template <int KIND>
+static int launch_kind(spx_ctx* c, const KArgs& a, dim3 grid, unsigned block, int amps, size_t lds) {
+  switch (amps) {
+    case 1: k_horder<KIND, 1><<<grid, block, lds, c->stream>>>(a); break;
+    case 2: k_horder<KIND, 2><<<grid, block, lds, c->stream>>>(a); break;
+    case 4: k_horder<KIND, 4><<<grid, block, lds, c->stream>>>(a); break;
+    case 8: k_horder<KIND, 8><<<grid, block, lds, c->stream>>>(a); break;
+    default: k_horder<KIND, 16><<<grid, block, lds, c->stream>>>(a); break;
   }
   SPX_HIP(hipGetLastError());
   return SPX_OK;
+}
+
+template <int KIND, int G>
+static void launch3g(spx_ctx* c, const K3Args& a, dim3 grid, unsigned block, size_t lds) {
+  k_order3<KIND, G><<<grid, block, lds, c->stream>>>(a);
+}
+
+template <int KIND>
+static int launch3(spx_ctx* c, const K3Args& a, dim3 grid, unsigned block, size_t lds) {
+  switch (a.g) {
+    case 0: launch3g<KIND, 0>(c, a, grid, block, lds); break;
+    case 1: launch3g<KIND, 1>(c, a, grid, block, lds); break;
+    case 2: launch3g<KIND, 2>(c, a, grid, block, lds); break;
+    case 3: launch3g<KIND, 3>(c, a, grid, block, lds); break;
+    case 4: launch3g<KIND, 4>(c, a, grid, block, lds); break;
+    case 5: launch3g<KIND, 5>(c, a, grid, block, lds); break;
+    case 6: launch3g<KIND, 6>(c, a, grid, block, lds); break;
+    case 7: launch3g<KIND, 7>(c, a, grid, block, lds); break;
+    case 8: launch3g<KIND, 8>(c, a, grid, block, lds); break;
+    case 9: launch3g<KIND, 9>(c, a, grid, block, lds); break;
+    case 10: launch3g<KIND, 10>(c, a, grid, block, lds); break;
+    case 11: launch3g<KIND, 11>(c, a, grid, block, lds); break;
+    default: launch3g<KIND, 12>(c, a, grid, block, lds); break;
+  }
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+static int launch_k3(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
+  K3Args a{};
+  for (int i = 0; i < kMaxMany; ++i) {
+    a.in[i] = b.in[i];
+    a.out[i] = b.out[i];
+    a.psi[i] = b.psi[i];
+  }
+  a.diag = b.diag;
+  a.terms = h->d_terms;
+  for (size_t i = 0; i < h->terms_off.size(); ++i) a.off[i] = h->terms_off[i];
+  a.tb = h->terms_tb;
+  a.g = h->terms_g;
+  a.xcd_remap = b.xcd_remap;
+  a.sdiag = b.sdiag;
+  a.a0 = b.a0;
+  a.c1r = b.c1r;
+  a.c1i = b.c1i;
+  a.cr = b.cr;
+  a.ci = b.ci;
+  spx_ctx* c = h->ctx;
+  const size_t n = static_cast<size_t>(1) << h->nspins;
+  const unsigned T = 1u << a.tb;
+  const dim3 grid(static_cast<unsigned>(n >> a.tb), nvec);
+  const size_t lds = sizeof(double2) * T * (1 + a.g);
+  switch (kind) {
+    case K_APPLY: return launch3<K_APPLY>(c, a, grid, T, lds);
+    case K_ACCUM: return launch3<K_ACCUM>(c, a, grid, T, lds);
+    case K_ORDER1: return launch3<K_ORDER1>(c, a, grid, T, lds);
+    case K_ORDER2: return launch3<K_ORDER2>(c, a, grid, T, lds);
+    default: return launch3<K_ORDERN>(c, a, grid, T, lds);
+  }
+}
+
+static int launch_k(spx_ham* h, int kind, const KArgs& a, unsigned nvec) {
+  if (h->terms_variant == 3) return launch_k3(h, kind, a, nvec);
+  spx_ctx* c = h->ctx;
+  const size_t n = static_cast<size_t>(1) << h->nspins;
+  const uint32_t T = 1u << a.tb;
+  unsigned block = static_cast<unsigned>(block_threads());
+  if (block > T) block = T;
+  int amps = static_cast<int>(T / block);
+  if (amps > 16) {
+    amps = 16;
+    block = T / 16;
+  }
+  const dim3 grid(static_cast<unsigned>(n >> a.tb), nvec);
+  const size_t lds = sizeof(double2) * (a.g > 0 ? 2 * T : T);
+  switch (kind) {
+    case K_APPLY: return launch_kind<K_APPLY>(c, a, grid, block, amps, lds);
+    case K_ACCUM: return launch_kind<K_ACCUM>(c, a, grid, block, amps, lds);
+    case K_ORDER1: return launch_kind<K_ORDER1>(c, a, grid, block, amps, lds);
+    case K_ORDER2: return launch_kind<K_ORDER2>(c, a, grid, block, amps, lds);
+    default: return launch_kind<K_ORDERN>(c, a, grid, block, amps, lds);
+  }
 }
 
 static KArgs base_args(spx_ham* h, double sdiag) {
   KArgs a{};
   a.diag = h->d_diag;
   a.terms = h->d_terms;
-  a.nfield = h->terms_nfield;
   a.nlocal = h->terms_nlocal;
-  a.nremote = h->terms_nremote;
+  a.g = h->terms_g;
+  a.nboth = h->terms_nboth;
+  if (h->terms_variant == 2)
+    for (int r = 0; r <= h->terms_g; ++r) a.grp[r] = h->terms_grp[r];
   a.tb = h->terms_tb;
+  a.xcd_remap = xcd_remap();
   a.sdiag = sdiag;
   return a;
 }
@@ -705,11 +1058,14 @@ int reduce_mag(spx_ctx* c, int axis, uint32_t spin, const double2* state, size_t
   return reduce(c, OpMag{axis, spin, state}, static_cast<size_t>(1) << L, out, nullptr);
 }
 
+// Energy::measure (src/core/Observable.cpp:112-120): H psi into the
+// Hamiltonian's scratch vector, then Re <H psi|psi> by the fixed-order
+// reduction.
 int reduce_energy(spx_ham* h, const double2* state, double* out) {
-  SPX_TRY(ham_prepare(h, 1.0));
-  OpEnergy op{state, h->d_diag, h->d_terms, h->terms_nfield,
-              h->terms_nlocal + h->terms_nremote};
-  return reduce(h->ctx, op, static_cast<size_t>(1) << h->nspins, out, nullptr);
+  const size_t n = static_cast<size_t>(1) << h->nspins;
+  if (!h->d_scratch) SPX_HIP(hipMalloc(&h->d_scratch, n * sizeof(double2)));
+  SPX_TRY(launch_apply(h, 1.0, state, h->d_scratch, false));
+  return reduce(h->ctx, OpInner{state, h->d_scratch}, n, out, nullptr);
 }
 
 }  // namespace spx

@@ -17,6 +17,6 @@ def ctx():
     """One device context for the whole GPU session (device 0)."""
     from spinsim_amd import spx
     print(f"[gpu] opening spx context on device 0 ({spx.LIB_PATH})", flush=True)
-    c = spx.Context(0)
-    yield c
-    c.close()
+    # Not closed explicitly: every device object keeps a reference to its
+    # context, so the context is released after the last of them.
+    return spx.Context(0)
