@@ -28,7 +28,7 @@ extern "C" {
 #endif
 
 /* Status codes. The C++ layer rethrows them as the reference's exception
- * types (include/spinsim/core/*.hpp): */
+ * types (include/spinsim/core of the reference): */
 typedef enum {
   SPX_OK = 0,
   SPX_EINVAL = 1,     /* std::invalid_argument */

@@ -1,0 +1,434 @@
+// Tests of the spinsim C++ drop-in (spinsim_amd/host) written like the
+// reference's own suite (tests/test_*.cpp), with a minimal harness.
+//
+//   test_host_api cpu            host-only cases (no device call)
+//   test_host_api gpu            device cases
+//   test_host_api dump <file>    config-0 echo run, per-realisation rows to <file>
+//                                (compared against the oracle by pytest)
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <numbers>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "spinsim/core/Bessel.hpp"
+#include "spinsim/core/Propagator.hpp"
+#include "spinsim/core/SpinOperators.hpp"
+#include "spinsim/io/Experiment.hpp"
+
+using namespace spinsim;
+
+namespace {
+
+int g_failed = 0;
+std::string g_case;
+
+#define CHECK(cond)                                                                \
+  do {                                                                             \
+    if (!(cond)) {                                                                 \
+      std::printf("  FAILED %s:%d in [%s]: %s\n", __FILE__, __LINE__, g_case.c_str(), #cond); \
+      ++g_failed;                                                                  \
+    }                                                                              \
+  } while (0)
+
+template <class E, class F>
+void checkThrows(F f, const char* what) {
+  try {
+    f();
+    std::printf("  FAILED [%s]: expected exception: %s\n", g_case.c_str(), what);
+    ++g_failed;
+  } catch (const E&) {
+  } catch (const std::exception& e) {
+    std::printf("  FAILED [%s]: wrong exception (%s): %s\n", g_case.c_str(), e.what(), what);
+    ++g_failed;
+  }
+}
+
+void run(const char* name, const std::function<void()>& body) {
+  g_case = name;
+  const int before = g_failed;
+  try {
+    body();
+  } catch (const std::exception& e) {
+    std::printf("  FAILED [%s]: uncaught %s\n", name, e.what());
+    ++g_failed;
+  }
+  std::printf("%s %s\n", g_failed == before ? "PASS" : "FAIL", name);
+  std::fflush(stdout);
+}
+
+double maxAbsDiff(const StateVector& a, const StateVector& b) {
+  double worst = 0.0;
+  for (std::size_t k = 0; k < a.size(); ++k)
+    worst = std::max(worst, std::hypot(a.re(k) - b.re(k), a.im(k) - b.im(k)));
+  return worst;
+}
+
+/// tests/test_Propagator.cpp:19-32
+Hamiltonian sampleHamiltonian(std::size_t n) {
+  Hamiltonian h(n);
+  for (SpinIndex s = 0; s < n; ++s) {
+    const double k = static_cast<double>(s) + 1.0;
+    h.setField(s, Field{0.4 * k, -0.3 / k, 0.9});
+  }
+  for (SpinIndex a = 0; a < n; ++a)
+    for (SpinIndex b = a + 1; b < n; ++b) {
+      const double r = static_cast<double>(b - a);
+      h.setCoupling(SpinPair{a, b}, 0.5 / r, -0.2 / r, 0.8 / (r * r));
+    }
+  return h;
+}
+
+StateVector randomState(std::size_t n, std::uint64_t seed) {
+  StateVector psi(n);
+  Rng rng(seed);
+  prepareInitialState(psi, InitialStateKind::Random, rng);
+  return psi;
+}
+
+/// The config-0 CPMG echo: pi/2_x, then 8 x [tau 0.5, ideal pi_y, tau 0.5, measure].
+RunConfig cpmgConfig(std::size_t L, std::size_t n) {
+  RunConfig c;
+  c.nspins = L;
+  c.baseFields.assign(L, Field{});
+  c.realizations = n;
+  c.seed = 20240726;
+  c.generateGeometry = true;
+  c.minSeparation = 0.5;
+  c.secularAngle = true;
+  c.fieldDisorder = 0.5;
+  c.referenceAxis = Axis::X;
+  c.observables = {"corr:x", "norm", "energy"};
+  c.epsilon = 1e-7;
+  std::vector<Step> body;
+  body.push_back(Evolve{0.05, 10, TimeMode::Real, false, {}});
+  body.push_back(Pulse{makeRotation("pi:y", L)});
+  body.push_back(Evolve{0.05, 10, TimeMode::Real, false, {}});
+  body.push_back(Measure{});
+  std::vector<Step> steps;
+  steps.push_back(Pulse{makeRotation("pi/2:x", L)});
+  steps.push_back(Repeat{8, body});
+  c.sequence = Sequence(std::move(steps));
+  return c;
+}
+
+// ------------------------------------------------------------------ host-only
+
+void cpuTests() {
+  run("rng streams depend only on the index", [] {
+    for (std::size_t idx : {0u, 7u, 179u}) {
+      Rng a = Rng::forRealization(12345, idx), b = Rng::forRealization(12345, idx);
+      for (int k = 0; k < 16; ++k) CHECK(a.uniform() == b.uniform());
+    }
+    Rng a = Rng::forRealization(999, 4), b = Rng::forRealization(999, 5);
+    int matches = 0;
+    for (int k = 0; k < 32; ++k) matches += a.uniform() == b.uniform();
+    CHECK(matches == 0);
+  });
+  run("accumulator textbook set and merge", [] {
+    const std::vector<double> s{2, 4, 4, 4, 5, 5, 7, 9};
+    Accumulator acc(1);
+    for (double v : s) acc.add(std::span<const double>(&v, 1));
+    CHECK(std::abs(acc.mean()[0] - 5.0) < 1e-15);
+    CHECK(std::abs(acc.standardDeviation()[0] - std::sqrt(32.0 / 7.0)) < 1e-14);
+    CHECK(std::abs(acc.standardError()[0] - std::sqrt(32.0 / 7.0) / std::sqrt(8.0)) < 1e-14);
+    Accumulator first(1), second(1);
+    for (std::size_t k = 0; k < s.size(); ++k) (k < 3 ? first : second).add(std::span<const double>(&s[k], 1));
+    first.merge(second);
+    CHECK(first.count() == 8 && std::abs(first.mean()[0] - 5.0) < 1e-14);
+    checkThrows<std::invalid_argument>([&] { acc.add(std::vector<double>{1.0, 2.0}); }, "width");
+  });
+  run("local runner is bitwise independent of thread count", [] {
+    const std::vector<std::string> cols{"a", "b"};
+    const RealizationJob job = [](std::size_t index, VectorRecorder& out) {
+      Rng rng = Rng::forRealization(7777, index);
+      for (int p = 0; p < 5; ++p) {
+        const std::vector<double> row{rng.gaussian(), rng.uniform()};
+        out.record(0.1 * p, row);
+      }
+    };
+    LocalRunner one(1), many(18);
+    const EnsembleResult a = one.run(64, cols, job), b = many.run(64, cols, job);
+    CHECK(a.times == b.times);
+    CHECK(a.statistics.mean() == b.statistics.mean());
+    CHECK(a.statistics.standardDeviation() == b.statistics.standardDeviation());
+  });
+  run("runner failures surface as exceptions", [] {
+    const std::vector<std::string> cols{"x"};
+    LocalRunner r(4);
+    checkThrows<std::runtime_error>([&] {
+      (void)r.run(32, cols, [](std::size_t i, VectorRecorder& out) {
+        if (i == 13) throw std::runtime_error("realisation blew up");
+        const std::vector<double> row{1.0};
+        out.record(0.0, row);
+      });
+    }, "failing job");
+    checkThrows<std::runtime_error>([&] {
+      (void)r.run(8, cols, [](std::size_t i, VectorRecorder& out) {
+        const std::vector<double> row{1.0};
+        out.record(0.0, row);
+        if (i == 2) out.record(1.0, row);
+      });
+    }, "shape mismatch");
+    checkThrows<std::invalid_argument>([&] { (void)r.run(0, cols, [](std::size_t, VectorRecorder&) {}); },
+                                       "zero realisations");
+  });
+  run("dipolar generator and its secular-angle option", [] {
+    Rng rng = Rng::forRealization(31337, 0);
+    const auto c = generateDipolarLattice2D(8, rng, 1.0);
+    CHECK(c.size() == 28);
+    for (const Coupling& k : c) {
+      CHECK(k.pair.low() < k.pair.high());
+      CHECK(k.jx == -0.5 * k.jz && k.jy == -0.5 * k.jz && k.jz > 0.0);
+    }
+    Rng a = Rng::forRealization(42, 5), b = Rng::forRealization(42, 5);
+    const auto plain = generateDipolarLattice2D(6, a, 2.0, 0.5, false);
+    const auto angled = generateDipolarLattice2D(6, b, 2.0, 0.5, true);
+    bool negative = false;
+    for (std::size_t k = 0; k < plain.size(); ++k) {
+      CHECK(plain[k].pair == angled[k].pair);
+      CHECK(std::abs(angled[k].jz) <= 2.0 * plain[k].jz + 1e-12);
+      negative |= angled[k].jz < 0.0;
+    }
+    CHECK(negative);
+  });
+  run("pulse specifications", [] {
+    const SpinRotation r = makeRotation("pi/2:y:3", 4);
+    CHECK(r.targets().size() == 1 && r.targets()[0] == 2);
+    CHECK(std::abs(r.angle() - 0.5 * std::numbers::pi) < 1e-15);
+    CHECK(makeRotation("pi:x", 5).targets().size() == 5);
+    for (const char* bad : {"pi:w", "pi/3:x", "pi:x:0", "pi:x:9", "pi", "pi:x:1:2"})
+      checkThrows<std::invalid_argument>([&] { (void)makeRotation(bad, 4); }, bad);
+    checkThrows<std::invalid_argument>([] { (void)SpinRotation(Axis3{0, 0, 0}, 1.0, {0}); }, "zero axis");
+  });
+  run("observables are built from configuration strings", [] {
+    CHECK(makeObservable("corr:x")->columns() == std::vector<std::string>({"corr_x_re", "corr_x_im"}));
+    CHECK(makeObservable("mag:2:z")->name() == "mag:2:z");
+    CHECK(makeObservable("norm")->width() == 2 && makeObservable("energy")->width() == 1);
+    for (const char* bad : {"corr", "corr:w", "mag:0:x", "magnet", "norm:x"})
+      checkThrows<std::invalid_argument>([&] { (void)makeObservable(bad); }, bad);
+  });
+  run("bessel and chebyshev through the library", [] {
+    const BesselSequence j = besselJ(3.0, 300);
+    double sum = j.values[0];
+    for (std::size_t n = 2; n < j.values.size(); n += 2) sum += 2.0 * j.values[n];
+    CHECK(std::abs(sum - 1.0) < 1e-13);
+    checkThrows<std::invalid_argument>([] { (void)besselJ(-1.0, 4); }, "negative x");
+    checkThrows<std::overflow_error>([] { (void)besselI(1000.0, 4); }, "I overflow");
+    const auto shortStep = ChebyshevExpansion::build(TimeMode::Real, 3.0, 0.1, 1e-10);
+    const auto longStep = ChebyshevExpansion::build(TimeMode::Real, 3.0, 2.0, 1e-10);
+    CHECK(shortStep.order() > 0 && longStep.order() > shortStep.order());
+    CHECK(ChebyshevExpansion::build(TimeMode::Real, 3.0, 1e-12, 1e-7).isIdentity());
+    try {
+      (void)ChebyshevExpansion::build(TimeMode::Real, 1.0, 1e6, 1e-7, 100);
+      CHECK(false);
+    } catch (const ChebyshevConvergenceError& e) {
+      CHECK(std::string(e.what()).find("tau") != std::string::npos);
+      CHECK(e.spectralBound() == 1.0 && e.epsilon() == 1e-7);
+    }
+  });
+  run("config validation", [] {
+    RunConfig c = cpmgConfig(6, 4);
+    c.validate();
+    RunConfig bad = c;
+    bad.nspins = 99;
+    checkThrows<ConfigError>([&] { bad.validate("<test>"); }, "spins");
+    bad = c;
+    bad.observables = {"corr:q"};
+    checkThrows<ConfigError>([&] { bad.validate("<test>"); }, "observable");
+    bad = c;
+    bad.sequence = Sequence({});
+    checkThrows<ConfigError>([&] { bad.validate("<test>"); }, "no measure");
+  });
+}
+
+// --------------------------------------------------------------------- device
+
+void gpuTests() {
+  run("state vector host view and device ops", [] {
+    StateVector psi(4);
+    CHECK(psi.size() == 16);
+    for (std::size_t k = 0; k < 16; ++k) CHECK(psi.re(k) == 0.0 && psi.im(k) == 0.0);
+    psi.setBasisState(6);
+    CHECK(psi.re(6) == 1.0);
+    psi.re(3) = 2.0;  // host edit is uploaded by the next device op
+    CHECK(std::abs(psi.norm() - std::sqrt(5.0)) < 1e-15);
+    psi.scale(2.0);
+    CHECK(psi.re(3) == 4.0 && psi.re(6) == 2.0);
+    checkThrows<std::out_of_range>([&] { psi.setBasisState(16); }, "basis index");
+    StateVector zero(3);
+    checkThrows<std::domain_error>([&] { zero.normalize(); }, "zero state");
+    const StateVector a = randomState(5, 1), b = randomState(5, 2);
+    const auto [re, im] = a.innerProduct(a);
+    CHECK(std::abs(re - 1.0) < 1e-14 && std::abs(im) < 1e-15);
+    StateVector c = a;
+    c.addScaled(0.5, -1.5, b);
+    const auto [r2, i2] = c.innerProduct(b);
+    const auto [r3, i3] = a.innerProduct(b);
+    CHECK(std::abs(r2 - (r3 + 0.5)) < 1e-13 && std::abs(i2 - (i3 - 1.5)) < 1e-13);
+  });
+  run("hamiltonian checks and spectral bound", [] {
+    checkThrows<std::out_of_range>([] { Hamiltonian h(0); }, "zero spins");
+    Hamiltonian h = sampleHamiltonian(4);
+    checkThrows<std::out_of_range>([&] { h.setField(4, Field{}); }, "spin");
+    h.setCoupling(SpinPair{2, 0}, 1, 2, 3);
+    std::size_t seen = 0;
+    for (const Coupling& c : h.couplings()) seen += c.pair == SpinPair{0, 2};
+    CHECK(seen == 1);
+    StateVector in = randomState(4, 3), out(4);
+    checkThrows<std::domain_error>([&] { h.applyTwiceNormalized(0.0, in, out); }, "bound");
+    // ||H psi|| <= bound, and -2H/bound is applyTwiceNormalized
+    h.applyPhysical(in, out);
+    CHECK(out.norm() <= h.spectralBound() + 1e-12);
+    StateVector tw(4);
+    h.applyTwiceNormalized(h.spectralBound(), in, tw);
+    tw.addScaled(2.0 / h.spectralBound(), 0.0, out);
+    CHECK(tw.norm() < 1e-13);
+  });
+  run("pi rotation about x is -i sigma_x", [] {
+    const StateVector psi0 = randomState(3, 0x9A9A);
+    StateVector got = psi0;
+    SpinRotation(Axis3{1, 0, 0}, std::numbers::pi, {1}).apply(got);
+    StateVector sx(3);
+    applySigmaX(psi0, 1, sx);
+    StateVector want(3);
+    for (std::size_t k = 0; k < 8; ++k) {
+      want.re(k) = sx.im(k);
+      want.im(k) = -sx.re(k);
+    }
+    CHECK(maxAbsDiff(got, want) < 1e-13);
+    StateVector n = randomState(4, 0xB0A7);
+    SpinRotation::onAllSpins(Axis3{0.3, -0.5, 0.8}, 1.7, 4).apply(n);
+    CHECK(std::abs(n.norm() - 1.0) < 1e-14);
+    checkThrows<std::out_of_range>([&] { SpinRotation(Axis3{}, 1.0, {7}).apply(n); }, "target");
+  });
+  run("two-spin swap gate permutes amplitudes", [] {
+    TwoSpinGate::Matrix swap{};
+    swap[0] = 1.0;
+    swap[1 * 4 + 2] = 1.0;
+    swap[2 * 4 + 1] = 1.0;
+    swap[15] = 1.0;
+    const TwoSpinGate g(swap, SpinPair{0, 2});
+    CHECK(g.unitarityDefect() < 1e-15);
+    StateVector psi = randomState(3, 5);
+    const StateVector before = psi;
+    g.apply(psi);
+    CHECK(psi.re(1) == before.re(4) && psi.im(4) == before.im(1) && psi.re(2) == before.re(2));
+  });
+  run("propagation: reversal, rabi, identity, errors", [] {
+    const Hamiltonian h = sampleHamiltonian(4);
+    const StateVector psi0 = randomState(4, 0x2718);
+    StateVector psi = psi0;
+    Propagator prop(4, 1e-12);
+    prop.evolve(h, TimeMode::Real, psi, 0.7, 1);
+    CHECK(maxAbsDiff(psi, psi0) > 1e-3);
+    prop.evolve(h, TimeMode::Real, psi, -0.7, 1);
+    CHECK(maxAbsDiff(psi, psi0) < 1e-11);
+    Hamiltonian h1(1);
+    h1.setField(0, Field{2.0, 0.0, 0.0});
+    StateVector s(1);
+    s.setBasisState(0);
+    Propagator p1(1, 1e-13);
+    double t = 0.0;
+    for (int step = 0; step < 60; ++step) {
+      p1.evolve(h1, TimeMode::Real, s, 0.05, 1);
+      t += 0.05;
+      StateVector z(1);
+      applySigmaZ(s, 0, z);
+      CHECK(std::abs(s.innerProduct(z).first + std::cos(2.0 * t)) < 1e-9);
+    }
+    StateVector idle = psi0;
+    Propagator p3(4, 1e-7);
+    p3.evolve(h, TimeMode::Real, idle, 1e-12, 1);
+    CHECK(p3.lastOrder() == 0 && maxAbsDiff(idle, psi0) == 0.0);
+    StateVector wrong(5);
+    checkThrows<std::invalid_argument>([&] { p3.evolve(h, TimeMode::Real, wrong, 0.1, 1); }, "size");
+    checkThrows<ChebyshevConvergenceError>([&] {
+      StateVector q = psi0;
+      Propagator(4, 1e-7).evolve(h, TimeMode::Real, q, 1e5, 1);
+    }, "over-long step");
+  });
+  run("energy is conserved and norms are reported", [] {
+    const Hamiltonian h = sampleHamiltonian(5);
+    StateVector psi = randomState(5, 0x3141), ref = randomState(5, 7);
+    Energy e;
+    StateNorms nrm;
+    MeasureContext mc;
+    mc.state = &psi;
+    mc.reference = &ref;
+    mc.hamiltonian = &h;
+    double e0[1], e1[1], n[2];
+    e.measure(mc, e0);
+    Propagator prop(5, 1e-12);
+    for (int b = 0; b < 10; ++b) prop.evolve(h, TimeMode::Real, psi, 0.02, 100);
+    e.measure(mc, e1);
+    nrm.measure(mc, n);
+    CHECK(std::abs(e1[0] - e0[0]) < 1e-10 && std::abs(n[0] - 1.0) < 1e-10 && std::abs(n[1] - 1.0) < 1e-14);
+    MeasureContext bad;
+    bad.state = &psi;
+    checkThrows<std::invalid_argument>([&] { TotalSpinCorrelator(Axis::X).measure(bad, n); }, "no reference");
+  });
+  run("a realisation is a pure function of config and index", [] {
+    RunConfig cfg = cpmgConfig(6, 4);
+    cfg.seed = 4242;
+    VectorRecorder a, b, c;
+    runRealization(cfg, 9, a);
+    runRealization(cfg, 9, b);
+    runRealization(cfg, 10, c);
+    CHECK(a.values() == b.values() && a.times() == b.times());
+    CHECK(c.values() != a.values());
+  });
+  run("ensembles: thread-count independence, batched runner, csv", [] {
+    const RunConfig cfg = cpmgConfig(8, 12);
+    LocalRunner one(1), four(4);
+    const EnsembleResult r1 = runExperiment(cfg, one);
+    const EnsembleResult r4 = runExperiment(cfg, four);
+    CHECK(r1.statistics.mean() == r4.statistics.mean());
+    CHECK(r1.statistics.standardDeviation() == r4.statistics.standardDeviation());
+    const EnsembleResult rb = runExperimentBatched(cfg);
+    CHECK(rb.times == r1.times);
+    double worst = 0.0;
+    for (std::size_t k = 0; k < rb.statistics.mean().size(); ++k)
+      worst = std::max(worst, std::abs(rb.statistics.mean()[k] - r1.statistics.mean()[k]));
+    CHECK(worst < 1e-10);
+    std::ostringstream csv;
+    writeCsv(r1, csv);
+    CHECK(csv.str().rfind("time,corr_x_re_mean,corr_x_re_sd,corr_x_re_stderr,", 0) == 0);
+    CHECK(r1.statistics.count() == 12);
+  });
+}
+
+int dump(const char* path) {
+  const RunConfig cfg = cpmgConfig(10, 20);
+  std::ofstream out(path);
+  out.precision(17);
+  for (std::size_t i = 0; i < cfg.realizations; ++i) {
+    VectorRecorder rec;
+    runRealization(cfg, i, rec);
+    for (std::size_t p = 0; p < rec.rows(); ++p) {
+      out << i << ' ' << rec.times()[p];
+      for (std::size_t c = 0; c < 5; ++c) out << ' ' << rec.values()[p * 5 + c];
+      out << '\n';
+    }
+  }
+  const EnsembleResult batched = runExperimentBatched(cfg);
+  for (std::size_t k = 0; k < batched.statistics.mean().size(); ++k)
+    out << "mean " << batched.statistics.mean()[k] << '\n';
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const std::string mode = argc > 1 ? argv[1] : "cpu";
+  if (mode == "dump" && argc > 2) return dump(argv[2]);
+  if (mode == "cpu" || mode == "all") cpuTests();
+  if (mode == "gpu" || mode == "all") gpuTests();
+  std::printf("%s: %d failure(s)\n", g_failed ? "FAILED" : "OK", g_failed);
+  return g_failed ? 1 : 0;
+}
