@@ -268,6 +268,7 @@ int spx_ham_destroy(spx_ham* h) {
   if (h->d_terms) hipFree(h->d_terms);
   if (h->d_tab) hipFree(h->d_tab);
   if (h->d_scratch) hipFree(h->d_scratch);
+  if (h->d_coef) hipFree(h->d_coef);
   delete h;
   return SPX_OK;
 }

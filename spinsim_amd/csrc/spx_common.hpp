@@ -94,6 +94,9 @@ struct spx_ham {
   std::vector<int> terms_grp;   // v2 group offsets
   std::vector<int> terms_off;   // v3 segment ends
   int terms_variant = 0;
+  int terms_flipboth = 0;
+  bool dense_ok = false;         // specialised all-pairs kernel usable
+  double* d_coef = nullptr;      // its weights
   double terms_scale = 0.0;
   bool terms_valid = false;
   std::vector<spx::FlipTerm> h_terms;

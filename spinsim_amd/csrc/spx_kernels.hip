@@ -12,6 +12,7 @@
 // fused into the same pass (src/core/Propagator.cpp:58-72), so an order moves
 // T_{n-1}, T_{n-2}, T_n, the diagonal and psi exactly once through HBM.
 #include <cstdlib>
+#include <utility>
 
 #include "spx_common.hpp"
 
@@ -212,12 +213,16 @@ struct K3Args {
   // bits remote (tile-index masks)
   int off[3 + 3 * kMaxG3 + 1];
   int tb, g;
+  int nterms;     // table length (padded), staged into LDS after the tiles
   int xcd_remap;
+  int flipboth;   // any nonzero flip-both (jx != jy) weight
   double sdiag;
   double a0, c1r, c1i, cr, ci;
 };
 
-template <int KIND, int G>
+constexpr int kChunk = 8;
+
+template <int KIND, int G, bool FLIPBOTH>
 __global__ __launch_bounds__(1024) void k_order3(K3Args a) {
   extern __shared__ double2 smem[];
   const uint32_t T = blockDim.x;
@@ -243,27 +248,74 @@ __global__ __launch_bounds__(1024) void k_order3(K3Args a) {
   smem[j] = xv;
 #pragma unroll
   for (int r = 0; r < G; ++r) smem[(1 + r) * T + j] = nbv[r];
+  // The flip-term table goes to LDS too: read from there, its entries
+  // do not serialise against the LDS gathers the way scalar loads
+  // (which share the LGKM counter and may return out of order) would.
+  FlipTerm* lterms = reinterpret_cast<FlipTerm*>(smem + (1 + G) * T);
+  for (int i = static_cast<int>(j); i < 2 * a.nterms; i += static_cast<int>(T))
+    reinterpret_cast<double2*>(lterms)[i] = reinterpret_cast<const double2*>(a.terms)[i];
   __syncthreads();
 
   const double2* own = smem;
-  const FlipTerm* __restrict__ tt = a.terms;
+  const FlipTerm* tt = lterms;
   const double f = a.sdiag * dg;
   double2 acc = make_double2(f * xv.x, f * xv.y);
 
-  for (int t = 0; t < a.off[0]; ++t) {
-    const FlipTerm ft = tt[t];
-    field_term(acc, (j & ft.mask) != 0, ft.anti, ft.aligned, own[j ^ ft.mask]);
+  // Every segment is padded by the host to a multiple of kChunk with inert
+  // terms (mask 0, zero weights), so each chunk is straight-line code: its
+  // table entries, then its LDS gathers, then its FMAs, with the memory
+  // operations of the whole chunk in flight together. Without flip-both
+  // weights (the dipolar case) an inactive lane simply sits out the gather
+  // and the FMA under the exec mask.
+  for (int t0 = 0; t0 < a.off[0]; t0 += kChunk) {
+    double2 p[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) p[u] = own[j ^ tt[t0 + u].mask];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const FlipTerm ft = tt[t0 + u];
+      field_term(acc, (j & ft.mask) != 0, ft.anti, ft.aligned, p[u]);
+    }
   }
-  for (int t = a.off[0]; t < a.off[1]; ++t) {
-    const FlipTerm ft = tt[t];
-    const double c = (__popc(wj & ft.mask) == 1) ? ft.anti : ft.aligned;
-    if (c != 0.0) cmad(acc, c, own[j ^ ft.mask]);
+  // wave-uniform local pairs: scalar activity test
+  for (int t0 = a.off[0]; t0 < a.off[1]; t0 += kChunk) {
+    double c[kChunk];
+    double2 p[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const FlipTerm ft = tt[t0 + u];
+      c[u] = (__popc(wj & ft.mask) == 1) ? ft.anti : ft.aligned;
+      if (c[u] != 0.0) p[u] = own[j ^ ft.mask];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (c[u] != 0.0) cmad(acc, c[u], p[u]);
   }
-#pragma unroll 4
-  for (int t = a.off[1]; t < a.off[2]; ++t) {
-    const FlipTerm ft = tt[t];
-    const double c = (__popc(j & ft.mask) == 1) ? ft.anti : ft.aligned;
-    if (c != 0.0) cmad(acc, c, own[j ^ ft.mask]);
+  // per-lane local pairs
+  for (int t0 = a.off[1]; t0 < a.off[2]; t0 += kChunk) {
+    double2 p[kChunk];
+    bool on[kChunk];
+    if (FLIPBOTH) {
+      double c[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const FlipTerm ft = tt[t0 + u];
+        c[u] = (__popc(j & ft.mask) == 1) ? ft.anti : ft.aligned;
+        p[u] = own[j ^ ft.mask];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) cmad(acc, c[u], p[u]);
+    } else {
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const uint32_t m = tt[t0 + u].mask;
+        on[u] = __popc(j & m) == 1;
+        if (on[u]) p[u] = own[j ^ m];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u)
+        if (on[u]) cmad(acc, tt[t0 + u].anti, p[u]);
+    }
   }
   for (int r = 0; r < G; ++r) {
     const double2* nb = smem + (1 + r) * T;
@@ -274,39 +326,64 @@ __global__ __launch_bounds__(1024) void k_order3(K3Args a) {
       const FlipTerm ft = tt[t];
       field_term(acc, xb, ft.anti, ft.aligned, nb[j]);
     }
-    for (int t = s1; t < s2; ++t) {
-      const FlipTerm ft = tt[t];
-      const double c = (((wj & ft.mask) != 0) != xb) ? ft.anti : ft.aligned;
-      if (c != 0.0) cmad(acc, c, nb[j ^ ft.mask]);
+    for (int t0 = s1; t0 < s2; t0 += kChunk) {
+      double c[kChunk];
+      double2 p[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const FlipTerm ft = tt[t0 + u];
+        c[u] = (((wj & ft.mask) != 0) != xb) ? ft.anti : ft.aligned;
+        if (ft.mask == 0) c[u] = 0.0;
+        if (c[u] != 0.0) p[u] = nb[j ^ ft.mask];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u)
+        if (c[u] != 0.0) cmad(acc, c[u], p[u]);
     }
-#pragma unroll 4
-    for (int t = s2; t < s3; ++t) {
-      const FlipTerm ft = tt[t];
-      const double c = (((j & ft.mask) != 0) != xb) ? ft.anti : ft.aligned;
-      if (c != 0.0) cmad(acc, c, nb[j ^ ft.mask]);
+    for (int t0 = s2; t0 < s3; t0 += kChunk) {
+      double2 p[kChunk];
+      bool on[kChunk];
+      if (FLIPBOTH) {
+        double c[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const FlipTerm ft = tt[t0 + u];
+          c[u] = (((j & ft.mask) != 0) != xb) ? ft.anti : ft.aligned;
+          if (ft.mask == 0) c[u] = 0.0;
+          p[u] = nb[j ^ ft.mask];
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) cmad(acc, c[u], p[u]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const uint32_t m = tt[t0 + u].mask;
+          on[u] = m != 0 && (((j & m) != 0) != xb);
+          if (on[u]) p[u] = nb[j ^ m];
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+          if (on[u]) cmad(acc, tt[t0 + u].anti, p[u]);
+      }
     }
   }
+  // pairs with both bits remote: the partner tile from global memory,
+  // activity decided per tile
   const int e0 = a.off[2 + 3 * G], e1 = a.off[3 + 3 * G];
-  // batches of eight partner loads in flight before their FMAs; named
-  // registers, not an array, so nothing goes to scratch
-#define SPX_BOTH_LOAD(u)                                                          \
-  double c##u = 0.0;                                                              \
-  double2 p##u = make_double2(0.0, 0.0);                                          \
-  if (t0 + u < e1) {                                                              \
-    const FlipTerm ft = tt[t0 + u];                                               \
-    c##u = (__popc(x & ft.mask) == 1) ? ft.anti : ft.aligned;                     \
-    if (c##u != 0.0) p##u = in[(static_cast<size_t>(x ^ ft.mask) << a.tb) + j];   \
+  for (int t0 = e0; t0 < e1; t0 += kChunk) {
+    double c[kChunk];
+    double2 p[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const FlipTerm ft = tt[t0 + u];
+      c[u] = (__popc(x & ft.mask) == 1) ? ft.anti : ft.aligned;
+      if (ft.mask == 0) c[u] = 0.0;
+      if (c[u] != 0.0) p[u] = in[(static_cast<size_t>(x ^ ft.mask) << a.tb) + j];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u)
+      if (c[u] != 0.0) cmad(acc, c[u], p[u]);
   }
-#define SPX_BOTH_FMA(u) \
-  if (c##u != 0.0) cmad(acc, c##u, p##u);
-  for (int t0 = e0; t0 < e1; t0 += 8) {
-    SPX_BOTH_LOAD(0) SPX_BOTH_LOAD(1) SPX_BOTH_LOAD(2) SPX_BOTH_LOAD(3)
-    SPX_BOTH_LOAD(4) SPX_BOTH_LOAD(5) SPX_BOTH_LOAD(6) SPX_BOTH_LOAD(7)
-    SPX_BOTH_FMA(0) SPX_BOTH_FMA(1) SPX_BOTH_FMA(2) SPX_BOTH_FMA(3)
-    SPX_BOTH_FMA(4) SPX_BOTH_FMA(5) SPX_BOTH_FMA(6) SPX_BOTH_FMA(7)
-  }
-#undef SPX_BOTH_LOAD
-#undef SPX_BOTH_FMA
 
   if constexpr (KIND == K_APPLY) {
     a.out[v][k] = acc;
@@ -324,6 +401,152 @@ __global__ __launch_bounds__(1024) void k_order3(K3Args a) {
     q.x += a.cr * t2.x - a.ci * t2.y;
     q.y += a.cr * t2.y + a.ci * t2.x;
     a.psi[v][k] = q;
+  } else {
+    const double2 tn = make_double2(acc.x - prev.x, acc.y - prev.y);
+    a.out[v][k] = tn;
+    psiv.x += a.cr * tn.x - a.ci * tn.y;
+    psiv.y += a.cr * tn.y + a.ci * tn.x;
+    a.psi[v][k] = psiv;
+  }
+}
+
+// ------------------------------------------------------ dense-pair kernel
+// The hot configuration: every pair (a < b) may carry a flip-flop weight
+// (all-to-all dipolar couplings), no flip-both weight and no transverse
+// field. The pair structure is then fixed by L alone, so it is compiled in:
+// each pair's partner offset, its tile class (own tile / neighbour tile /
+// partner tile in global memory) and whether its activity test is per lane
+// or per wavefront are constants; only the L(L-1)/2 weights are read, eight
+// per scalar load. One amplitude per thread, tiles of 2^TB amplitudes.
+// Pair q of (0,1),(0,2),...,(0,L-1),(1,2),...: its two spins.
+constexpr int pair_a(int L, int q) {
+  int x = 0;
+  while (q >= L - 1 - x) {
+    q -= L - 1 - x;
+    ++x;
+  }
+  return x;
+}
+constexpr int pair_b(int L, int q) {
+  int x = 0;
+  while (q >= L - 1 - x) {
+    q -= L - 1 - x;
+    ++x;
+  }
+  return x + 1 + q;
+}
+
+struct DenseLane {
+  uint32_t j, wj, x;
+  const double2* smem;
+  const double2* in;
+};
+
+// Gather of pair Q for this lane: partner from the own tile, a neighbour
+// tile (both in LDS) or the partner tile in global memory.
+template <int L, int TB, int Q>
+__device__ __forceinline__ void dense_gather(const DenseLane& d, double c, bool& on, double2& p) {
+  constexpr int NP = L * (L - 1) / 2;
+  if constexpr (Q < NP) {
+    constexpr int A = pair_a(L, Q), B = pair_b(L, Q);
+    constexpr int T = 1 << TB;
+    const uint32_t bA = A < 6 ? (d.j >> A) & 1u : A < TB ? (d.wj >> A) & 1u : (d.x >> (A - TB)) & 1u;
+    const uint32_t bB = B < 6 ? (d.j >> B) & 1u : B < TB ? (d.wj >> B) & 1u : (d.x >> (B - TB)) & 1u;
+    on = (bA != bB) && c != 0.0;
+    if (on) {
+      if constexpr (B < TB) {
+        p = d.smem[d.j ^ ((1u << A) | (1u << B))];
+      } else if constexpr (A < TB) {
+        p = d.smem[(1 + B - TB) * T + (d.j ^ (1u << A))];
+      } else {
+        p = d.in[(static_cast<size_t>(d.x ^ ((1u << (A - TB)) | (1u << (B - TB)))) << TB) + d.j];
+      }
+    }
+  } else {
+    on = false;
+  }
+}
+
+template <int L, int TB, int CH, int... U>
+__device__ __forceinline__ void dense_chunk(std::integer_sequence<int, U...>, const DenseLane& d,
+                                            const double* __restrict__ coef, double2& acc) {
+  constexpr int NP = L * (L - 1) / 2;
+  __builtin_amdgcn_sched_barrier(0);
+  double c[sizeof...(U)];
+  double2 p[sizeof...(U)];
+  bool on[sizeof...(U)];
+  ((c[U] = (CH * 8 + U < NP) ? coef[CH * 8 + U] : 0.0), ...);
+  __builtin_amdgcn_sched_barrier(0);
+  (dense_gather<L, TB, CH * 8 + U>(d, c[U], on[U], p[U]), ...);
+  ((on[U] ? cmad(acc, c[U], p[U]) : void()), ...);
+}
+
+template <int L, int TB, int... CH>
+__device__ __forceinline__ void dense_all(std::integer_sequence<int, CH...>, const DenseLane& d,
+                                          const double* __restrict__ coef, double2& acc) {
+  (dense_chunk<L, TB, CH>(std::make_integer_sequence<int, 8>{}, d, coef, acc), ...);
+}
+
+struct KDArgs {
+  const double2* in[kMaxMany];
+  double2* out[kMaxMany];
+  double2* psi[kMaxMany];
+  const double* diag;
+  const double* coef;   // [L(L-1)/2] flip-flop weights, pair order (0,1),(0,2),...
+  int xcd_remap;
+  double sdiag;
+  double a0, c1r, c1i, cr, ci;
+};
+
+template <int KIND, int L, int TB>
+__global__ __launch_bounds__(1 << TB) void k_dense(KDArgs a) {
+  constexpr int T = 1 << TB;
+  constexpr int G = L - TB;
+  constexpr int NCH = (L * (L - 1) / 2 + 7) / 8;
+  extern __shared__ double2 smem[];
+  const uint32_t j = threadIdx.x;
+  const uint32_t wj = __builtin_amdgcn_readfirstlane(j) & ~63u;
+  const int v = blockIdx.y;
+  const double2* __restrict__ in = a.in[v];
+  uint32_t x = blockIdx.x;
+  if (a.xcd_remap && G >= 3) x = ((x & 7u) << (G - 3)) | (x >> 3);
+  const size_t base = static_cast<size_t>(x) << TB;
+  const size_t k = base + j;
+
+  const double2 xv = in[k];
+  double2 nbv[G > 0 ? G : 1];
+#pragma unroll
+  for (int r = 0; r < G; ++r) nbv[r] = in[(static_cast<size_t>(x ^ (1u << r)) << TB) + j];
+  const double dg = a.diag[k];
+  double2 prev = make_double2(0.0, 0.0), psiv = make_double2(0.0, 0.0);
+  if constexpr (KIND == K_ACCUM || KIND == K_ORDERN) prev = a.out[v][k];
+  if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN) psiv = a.psi[v][k];
+  smem[j] = xv;
+#pragma unroll
+  for (int r = 0; r < G; ++r) smem[(1 + r) * T + j] = nbv[r];
+  __syncthreads();
+
+  const double f = a.sdiag * dg;
+  double2 acc = make_double2(f * xv.x, f * xv.y);
+  const DenseLane d{j, wj, x, smem, in};
+  dense_all<L, TB>(std::make_integer_sequence<int, NCH>{}, d, a.coef, acc);
+
+  if constexpr (KIND == K_APPLY) {
+    a.out[v][k] = acc;
+  } else if constexpr (KIND == K_ACCUM) {
+    a.out[v][k] = make_double2(prev.x + acc.x, prev.y + acc.y);
+  } else if constexpr (KIND == K_ORDER1) {
+    a.out[v][k] = make_double2(0.5 * acc.x, 0.5 * acc.y);
+  } else if constexpr (KIND == K_ORDER2) {
+    const double2 t0 = psiv, t1 = xv;
+    const double2 t2 = make_double2(acc.x - t0.x, acc.y - t0.y);
+    a.out[v][k] = t2;
+    double2 qv = make_double2(a.a0 * t0.x, a.a0 * t0.y);
+    qv.x += a.c1r * t1.x - a.c1i * t1.y;
+    qv.y += a.c1r * t1.y + a.c1i * t1.x;
+    qv.x += a.cr * t2.x - a.ci * t2.y;
+    qv.y += a.cr * t2.y + a.ci * t2.x;
+    a.psi[v][k] = qv;
   } else {
     const double2 tn = make_double2(acc.x - prev.x, acc.y - prev.y);
     a.out[v][k] = tn;
@@ -691,6 +914,36 @@ int xcd_remap() {
 
 // Diagonal (when hz or the couplings changed) and the flip table at `scale`,
 // with the reference's skip rule applied to the scaled coefficients.
+// Dense-pair eligibility and its weight table (pair order (0,1),(0,2),...):
+// L = 14..18 with the default tile, no transverse field and no flip-both
+// weight at this scale. SPX_DENSE=0 switches the specialised kernel off.
+static int prepare_dense(spx_ham* h, double scale) {
+  static const int enabled = env_int("SPX_DENSE", 1);
+  const size_t L = h->nspins;
+  h->dense_ok = false;
+  if (!enabled || L < 14 || L > 18 || tile_bits_for(L) != 10) return SPX_OK;
+  for (size_t s = 0; s < L; ++s)
+    if (std::abs(scale * 0.5 * h->hx[s]) > kNegligible || std::abs(scale * 0.5 * h->hy[s]) > kNegligible)
+      return SPX_OK;
+  const size_t np = L * (L - 1) / 2;
+  std::vector<double> coef(np, 0.0);
+  for (size_t p = 0; p < h->lo.size(); ++p) {
+    const double quarter = scale * 0.25;
+    const double ff = quarter * (h->jx[p] + h->jy[p]);
+    const double fb = quarter * (h->jx[p] - h->jy[p]);
+    if (std::abs(fb) > kNegligible) return SPX_OK;
+    const size_t lo = h->lo[p], hi = h->hi[p];
+    const size_t q = lo * L - lo * (lo + 1) / 2 + (hi - lo - 1);
+    coef[q] = std::abs(ff) > kNegligible ? ff : 0.0;
+  }
+  if (!h->d_coef) SPX_HIP(hipMalloc(&h->d_coef, 32 * 31 / 2 * sizeof(double)));
+  SPX_HIP(hipMemcpyAsync(h->d_coef, coef.data(), np * sizeof(double), hipMemcpyHostToDevice,
+                         h->ctx->stream));
+  SPX_HIP(hipStreamSynchronize(h->ctx->stream));
+  h->dense_ok = true;
+  return SPX_OK;
+}
+
 int ham_prepare(spx_ham* h, double scale) {
   spx_ctx* c = h->ctx;
   const size_t L = h->nspins, n = static_cast<size_t>(1) << L;
@@ -733,11 +986,14 @@ int ham_prepare(spx_ham* h, double scale) {
     h->terms_valid = false;
   }
   if (h->terms_valid && h->terms_scale == scale) return SPX_OK;
+  SPX_TRY(prepare_dense(h, scale));
 
   const int tb = tile_bits_for(L);
   const int g = static_cast<int>(L) - tb;
+  // the padded table is at most L fields + L(L-1)/2 pairs plus kChunk per segment
+  const size_t table_max = sizeof(FlipTerm) * (L + L * (L - 1) / 2 + kChunk * (4 + 2 * (g > 0 ? g : 0)));
   if (tb <= 10 && g <= kMaxG3 &&
-      static_cast<size_t>(1 + (g > 0 ? g : 0)) * (sizeof(double2) << tb) <= 160 * 1024) {
+      static_cast<size_t>(1 + (g > 0 ? g : 0)) * (sizeof(double2) << tb) + table_max <= 160 * 1024) {
     // v3 layout (see K3Args::off)
     std::vector<FlipTerm> seg[3];
     std::vector<std::vector<FlipTerm>> rseg(3 * (g > 0 ? g : 0));
@@ -765,18 +1021,25 @@ int ham_prepare(spx_ham* h, double scale) {
         both.push_back(FlipTerm{(1u << (lo - tb)) | (1u << (hi - tb)), 0u, ff, fb, 0.0});
       }
     }
+    // pad every chunked segment to a multiple of kChunk with inert terms
+    // (mask 0, zero weights); the field segments of remote bits are walked
+    // term by term and stay unpadded
+    h->terms_flipboth = 0;
+    auto append = [&](const std::vector<FlipTerm>& sg, bool pad) {
+      for (const FlipTerm& t : sg)
+        if (t.kind == 0 && t.aligned != 0.0) h->terms_flipboth = 1;
+      const size_t start = h->h_terms.size();
+      h->h_terms.insert(h->h_terms.end(), sg.begin(), sg.end());
+      if (pad)  // the chunk loops step from the segment's own start
+        while ((h->h_terms.size() - start) % kChunk != 0)
+          h->h_terms.push_back(FlipTerm{0u, 0u, 0.0, 0.0, 0.0});
+      h->terms_off.push_back(static_cast<int>(h->h_terms.size()));
+    };
     h->h_terms.clear();
     h->terms_off.clear();
-    for (auto& sg : seg) {
-      h->h_terms.insert(h->h_terms.end(), sg.begin(), sg.end());
-      h->terms_off.push_back(static_cast<int>(h->h_terms.size()));
-    }
-    for (auto& sg : rseg) {
-      h->h_terms.insert(h->h_terms.end(), sg.begin(), sg.end());
-      h->terms_off.push_back(static_cast<int>(h->h_terms.size()));
-    }
-    h->h_terms.insert(h->h_terms.end(), both.begin(), both.end());
-    h->terms_off.push_back(static_cast<int>(h->h_terms.size()));
+    for (auto& sg : seg) append(sg, true);
+    for (size_t i = 0; i < rseg.size(); ++i) append(rseg[i], i % 3 != 0);
+    append(both, true);
     h->terms_variant = 3;
     h->nterms = static_cast<int>(h->h_terms.size());
     h->terms_g = g > 0 ? g : 0;
@@ -854,7 +1117,8 @@ static int launch_kind(spx_ctx* c, const KArgs& a, dim3 grid, unsigned block, in
 
 template <int KIND, int G>
 static void launch3g(spx_ctx* c, const K3Args& a, dim3 grid, unsigned block, size_t lds) {
-  k_order3<KIND, G><<<grid, block, lds, c->stream>>>(a);
+  if (a.flipboth) k_order3<KIND, G, true><<<grid, block, lds, c->stream>>>(a);
+  else k_order3<KIND, G, false><<<grid, block, lds, c->stream>>>(a);
 }
 
 template <int KIND>
@@ -890,6 +1154,8 @@ static int launch_k3(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
   for (size_t i = 0; i < h->terms_off.size(); ++i) a.off[i] = h->terms_off[i];
   a.tb = h->terms_tb;
   a.g = h->terms_g;
+  a.nterms = h->nterms;
+  a.flipboth = h->terms_flipboth;
   a.xcd_remap = b.xcd_remap;
   a.sdiag = b.sdiag;
   a.a0 = b.a0;
@@ -901,7 +1167,7 @@ static int launch_k3(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
   const size_t n = static_cast<size_t>(1) << h->nspins;
   const unsigned T = 1u << a.tb;
   const dim3 grid(static_cast<unsigned>(n >> a.tb), nvec);
-  const size_t lds = sizeof(double2) * T * (1 + a.g);
+  const size_t lds = sizeof(double2) * T * (1 + a.g) + sizeof(FlipTerm) * a.nterms;
   switch (kind) {
     case K_APPLY: return launch3<K_APPLY>(c, a, grid, T, lds);
     case K_ACCUM: return launch3<K_ACCUM>(c, a, grid, T, lds);
@@ -911,7 +1177,55 @@ static int launch_k3(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
   }
 }
 
+template <int KIND, int L>
+static void launch_dense_l(spx_ctx* c, const KDArgs& a, unsigned nvec) {
+  constexpr int TB = 10;
+  const dim3 grid(1u << (L - TB), nvec);
+  const size_t lds = sizeof(double2) * (static_cast<size_t>(1 + L - TB) << TB);
+  k_dense<KIND, L, TB><<<grid, 1u << TB, lds, c->stream>>>(a);
+}
+
+template <int KIND>
+static int launch_dense_kind(spx_ham* h, const KDArgs& a, unsigned nvec) {
+  spx_ctx* c = h->ctx;
+  switch (h->nspins) {
+    case 14: launch_dense_l<KIND, 14>(c, a, nvec); break;
+    case 15: launch_dense_l<KIND, 15>(c, a, nvec); break;
+    case 16: launch_dense_l<KIND, 16>(c, a, nvec); break;
+    case 17: launch_dense_l<KIND, 17>(c, a, nvec); break;
+    default: launch_dense_l<KIND, 18>(c, a, nvec); break;
+  }
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+static int launch_dense(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
+  KDArgs a{};
+  for (int i = 0; i < kMaxMany; ++i) {
+    a.in[i] = b.in[i];
+    a.out[i] = b.out[i];
+    a.psi[i] = b.psi[i];
+  }
+  a.diag = b.diag;
+  a.coef = h->d_coef;
+  a.xcd_remap = b.xcd_remap;
+  a.sdiag = b.sdiag;
+  a.a0 = b.a0;
+  a.c1r = b.c1r;
+  a.c1i = b.c1i;
+  a.cr = b.cr;
+  a.ci = b.ci;
+  switch (kind) {
+    case K_ORDER1: return launch_dense_kind<K_ORDER1>(h, a, nvec);
+    case K_ORDER2: return launch_dense_kind<K_ORDER2>(h, a, nvec);
+    case K_ORDERN: return launch_dense_kind<K_ORDERN>(h, a, nvec);
+    case K_ACCUM: return launch_dense_kind<K_ACCUM>(h, a, nvec);
+    default: return launch_dense_kind<K_APPLY>(h, a, nvec);
+  }
+}
+
 static int launch_k(spx_ham* h, int kind, const KArgs& a, unsigned nvec) {
+  if (h->dense_ok) return launch_dense(h, kind, a, nvec);
   if (h->terms_variant == 3) return launch_k3(h, kind, a, nvec);
   spx_ctx* c = h->ctx;
   const size_t n = static_cast<size_t>(1) << h->nspins;
