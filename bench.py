@@ -193,7 +193,7 @@ def main():
         "hpsi_per_s": round(args.steps * order / (ms * 1e-3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_horder<K_ORDERN>", "bytes_per_launch": kernel_bytes,
+                     "kernel": "k_dense<K_ORDERN, 18, 10> (general Chebyshev order)", "bytes_per_launch": kernel_bytes,
                      "us_per_launch": round(per_order_ms * 1e3, 3)},
     }
     if not args.no_cpu:

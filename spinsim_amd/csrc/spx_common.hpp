@@ -158,5 +158,6 @@ int reduce_energy(spx_ham* h, const double2* state, double* out);
 int tile_bits_for(size_t L);
 int block_threads();
 int xcd_remap();
+int dense_tile_bits();
 
 }  // namespace spx

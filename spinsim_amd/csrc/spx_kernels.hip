@@ -481,10 +481,84 @@ __device__ __forceinline__ void dense_chunk(std::integer_sequence<int, U...>, co
   ((on[U] ? cmad(acc, c[U], p[U]) : void()), ...);
 }
 
-template <int L, int TB, int... CH>
-__device__ __forceinline__ void dense_all(std::integer_sequence<int, CH...>, const DenseLane& d,
+// Pairs are split into those whose partner is in LDS (first NL in pair
+// order) and those with both bits remote (the last C(L-TB, 2)), whose partner
+// tiles come from global memory. Each remote chunk is issued four LDS chunks
+// before it is consumed, so its L2/MALL latency hides behind LDS work.
+struct RemoteSlot {
+  double c[8];
+  double2 p[8];
+  bool on[8];
+};
+
+template <int L, int TB>
+struct DenseShape {
+  static constexpr int NP = L * (L - 1) / 2;
+  static constexpr int NR = (L - TB) * (L - TB - 1) / 2;
+  static constexpr int NL = NP - NR;
+  static constexpr int NCL = (NL + 7) / 8;
+  static constexpr int NCR = (NR + 7) / 8;
+};
+
+template <int L, int TB, int CH, int... U>
+__device__ __forceinline__ void lds_chunk(std::integer_sequence<int, U...>, const DenseLane& d,
                                           const double* __restrict__ coef, double2& acc) {
-  (dense_chunk<L, TB, CH>(std::make_integer_sequence<int, 8>{}, d, coef, acc), ...);
+  constexpr int NL = DenseShape<L, TB>::NL;
+  __builtin_amdgcn_sched_barrier(0);
+  double c[8];
+  double2 p[8];
+  bool on[8];
+  ((c[U] = (CH * 8 + U < NL) ? coef[CH * 8 + U] : 0.0), ...);
+  __builtin_amdgcn_sched_barrier(0);
+  ((CH * 8 + U < NL ? dense_gather<L, TB, CH * 8 + U>(d, c[U], on[U], p[U]) : void(on[U] = false)), ...);
+  ((on[U] ? cmad(acc, c[U], p[U]) : void()), ...);
+}
+
+template <int L, int TB, int R, int... U>
+__device__ __forceinline__ void remote_issue(std::integer_sequence<int, U...>, const DenseLane& d,
+                                             const double* __restrict__ coef, RemoteSlot& s) {
+  constexpr int NL = DenseShape<L, TB>::NL, NP = DenseShape<L, TB>::NP;
+  __builtin_amdgcn_sched_barrier(0);
+  ((s.c[U] = (NL + R * 8 + U < NP) ? coef[NL + R * 8 + U] : 0.0), ...);
+  __builtin_amdgcn_sched_barrier(0);
+  (dense_gather<L, TB, NL + R * 8 + U>(d, s.c[U], s.on[U], s.p[U]), ...);
+}
+
+template <int... U>
+__device__ __forceinline__ void remote_consume(std::integer_sequence<int, U...>, const RemoteSlot& s,
+                                               double2& acc) {
+  __builtin_amdgcn_sched_barrier(0);
+  ((s.on[U] ? cmad(acc, s.c[U], s.p[U]) : void()), ...);
+}
+
+template <int L, int TB, int CH>
+__device__ __forceinline__ void dense_step(const DenseLane& d, const double* __restrict__ coef,
+                                           double2& acc, RemoteSlot& slot) {
+  using S = DenseShape<L, TB>;
+  constexpr auto U8 = std::make_integer_sequence<int, 8>{};
+  if constexpr (CH % 4 == 0 && CH / 4 < S::NCR) remote_issue<L, TB, CH / 4>(U8, d, coef, slot);
+  lds_chunk<L, TB, CH>(U8, d, coef, acc);
+  if constexpr (CH % 4 == 3 && CH / 4 < S::NCR) remote_consume(U8, slot, acc);
+}
+
+// remote chunks whose issue or consume fell past the last LDS chunk
+template <int L, int TB, int R>
+__device__ __forceinline__ void dense_tail(const DenseLane& d, const double* __restrict__ coef,
+                                           double2& acc, RemoteSlot& slot) {
+  using S = DenseShape<L, TB>;
+  constexpr auto U8 = std::make_integer_sequence<int, 8>{};
+  if constexpr (R * 4 + 3 >= S::NCL) {
+    if constexpr (R * 4 >= S::NCL) remote_issue<L, TB, R>(U8, d, coef, slot);
+    remote_consume(U8, slot, acc);
+  }
+}
+
+template <int L, int TB, int... CH, int... R>
+__device__ __forceinline__ void dense_all(std::integer_sequence<int, CH...>, std::integer_sequence<int, R...>,
+                                          const DenseLane& d, const double* __restrict__ coef, double2& acc) {
+  RemoteSlot slot;
+  (dense_step<L, TB, CH>(d, coef, acc, slot), ...);
+  (dense_tail<L, TB, R>(d, coef, acc, slot), ...);
 }
 
 struct KDArgs {
@@ -502,7 +576,6 @@ template <int KIND, int L, int TB>
 __global__ __launch_bounds__(1 << TB) void k_dense(KDArgs a) {
   constexpr int T = 1 << TB;
   constexpr int G = L - TB;
-  constexpr int NCH = (L * (L - 1) / 2 + 7) / 8;
   extern __shared__ double2 smem[];
   const uint32_t j = threadIdx.x;
   const uint32_t wj = __builtin_amdgcn_readfirstlane(j) & ~63u;
@@ -529,7 +602,8 @@ __global__ __launch_bounds__(1 << TB) void k_dense(KDArgs a) {
   const double f = a.sdiag * dg;
   double2 acc = make_double2(f * xv.x, f * xv.y);
   const DenseLane d{j, wj, x, smem, in};
-  dense_all<L, TB>(std::make_integer_sequence<int, NCH>{}, d, a.coef, acc);
+  dense_all<L, TB>(std::make_integer_sequence<int, DenseShape<L, TB>::NCL>{},
+                   std::make_integer_sequence<int, DenseShape<L, TB>::NCR>{}, d, a.coef, acc);
 
   if constexpr (KIND == K_APPLY) {
     a.out[v][k] = acc;
@@ -917,6 +991,11 @@ int xcd_remap() {
 // Dense-pair eligibility and its weight table (pair order (0,1),(0,2),...):
 // L = 14..18 with the default tile, no transverse field and no flip-both
 // weight at this scale. SPX_DENSE=0 switches the specialised kernel off.
+int dense_tile_bits() {
+  static const int tb = env_int("SPX_DENSE_TB", 10);
+  return tb == 9 ? 9 : 10;
+}
+
 static int prepare_dense(spx_ham* h, double scale) {
   static const int enabled = env_int("SPX_DENSE", 1);
   const size_t L = h->nspins;
@@ -1177,12 +1256,19 @@ static int launch_k3(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
   }
 }
 
-template <int KIND, int L>
-static void launch_dense_l(spx_ctx* c, const KDArgs& a, unsigned nvec) {
-  constexpr int TB = 10;
+template <int KIND, int L, int TB>
+static void launch_dense_lt(spx_ctx* c, const KDArgs& a, unsigned nvec) {
   const dim3 grid(1u << (L - TB), nvec);
   const size_t lds = sizeof(double2) * (static_cast<size_t>(1 + L - TB) << TB);
   k_dense<KIND, L, TB><<<grid, 1u << TB, lds, c->stream>>>(a);
+}
+
+// Tile of 2^10 amplitudes (one workgroup of 1024 threads per CU) or 2^9
+// (two of 512); SPX_DENSE_TB selects.
+template <int KIND, int L>
+static void launch_dense_l(spx_ctx* c, const KDArgs& a, unsigned nvec) {
+  if (dense_tile_bits() == 9) launch_dense_lt<KIND, L, 9>(c, a, nvec);
+  else launch_dense_lt<KIND, L, 10>(c, a, nvec);
 }
 
 template <int KIND>
