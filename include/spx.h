@@ -158,6 +158,12 @@ int spx_program_destroy(spx_program* p);
 /* Evolve{tau, nsteps, mode, autonormalize, fields}; fields3L NULL = keep base. */
 int spx_program_evolve(spx_program* p, double tau, size_t nsteps, int mode, int autonormalize,
                        const double* fields3L, size_t nspins);
+/* Evolve step whose field table is (hx, hy, hz) on every spin, whatever a
+ * realisation's L: an evolve step's `hx`/`hy`/`hz` keys
+ * (docs/configuration.md of the reference, `evolve` table), with the
+ * unnamed components passed as their base values. */
+int spx_program_evolve_uniform(spx_program* p, double tau, size_t nsteps, int mode,
+                               int autonormalize, double hx, double hy, double hz);
 /* Pulse{SpinRotation(axis, angle, targets)}; ntargets 0 = every spin. */
 int spx_program_pulse(spx_program* p, double ax, double ay, double az, double angle,
                       const uint32_t* targets, size_t ntargets);

@@ -612,6 +612,23 @@ int spx_program_evolve(spx_program* p, double tau, size_t nsteps, int mode, int 
   return SPX_OK;
 }
 
+int spx_program_evolve_uniform(spx_program* p, double tau, size_t nsteps, int mode, int autonormalize,
+                               double hx, double hy, double hz) {
+  if (mode != 0 && mode != 1) return fail(SPX_EINVAL, "unknown time mode");
+  ProgOp op;
+  op.kind = OP_EVOLVE;
+  op.tau = tau;
+  op.nsteps = nsteps;
+  op.mode = mode;
+  op.autonorm = autonormalize != 0;
+  op.uniform = true;
+  op.ufield[0] = hx;
+  op.ufield[1] = hy;
+  op.ufield[2] = hz;
+  p->ops.push_back(op);
+  return SPX_OK;
+}
+
 int spx_program_pulse(spx_program* p, double ax, double ay, double az, double angle,
                       const uint32_t* targets, size_t ntargets) {
   const double length = std::sqrt(ax * ax + ay * ay + az * az);

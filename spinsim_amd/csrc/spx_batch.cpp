@@ -1,8 +1,12 @@
 // Batched ensemble runner: the realisation loop of LocalRunner::run
 // (src/ensemble/EnsembleRunner.cpp:27-88) with each realisation's program
 // walked as Sequence::run does (src/sequence/Sequence.cpp:35-106).
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "spx_batch.hpp"
@@ -76,14 +80,15 @@ struct Walker {
     for (size_t i = beg; i < end; ++i) {
       const ProgOp& op = ops[i];
       if (op.kind == OP_EVOLVE) {
-        std::vector<double> saved;
-        if (!op.fields.empty()) {
-          if (op.fields.size() != 3 * L)
+        std::vector<double> saved, tmp;
+        if (!op.fields.empty() || op.uniform) {
+          const std::vector<double>& f = step_fields(op, r->fields, L, tmp);
+          if (f.size() != 3 * L)
             return fail(SPX_EINVAL, "step field table does not match the number of spins");
           saved.resize(3 * L);
           for (size_t s = 0; s < L; ++s) {
             SPX_TRY(spx_ham_get_field(h, s, &saved[3 * s]));
-            SPX_TRY(spx_ham_set_field(h, s, op.fields[3 * s], op.fields[3 * s + 1], op.fields[3 * s + 2]));
+            SPX_TRY(spx_ham_set_field(h, s, f[3 * s], f[3 * s + 1], f[3 * s + 2]));
           }
         }
         spx_state* both[2] = {state, ref};
@@ -146,6 +151,234 @@ int batch_run_streaming(spx_batch* b, size_t first, size_t count, double* times,
     spx_state_destroy(state);
     spx_ham_destroy(h);
     SPX_TRY(rc);
+  }
+  return SPX_OK;
+}
+
+namespace {
+
+// Spectral bound of one field table + coupling set (spectralBound,
+// src/core/Hamiltonian.cpp:82-93), same summation order.
+double bound_of(const std::vector<double>& f, const Realisation& r) {
+  double b = 0.0;
+  for (size_t s = 0; s < r.nspins; ++s)
+    b += 0.5 * (std::abs(f[3 * s]) + std::abs(f[3 * s + 1]) + std::abs(f[3 * s + 2]));
+  for (size_t p = 0; p < r.lo.size(); ++p)
+    b += 0.25 * (std::abs(r.jx[p]) + std::abs(r.jy[p]) + std::abs(r.jz[p]));
+  return b;
+}
+
+// Hamiltonian weights of one field table at `scale`, appended to `w`:
+// pairs (anti, aligned) in pair order, fields (cx, cy), sigma_z terms.
+void append_weights(std::vector<double>& w, const std::vector<double>& f, const Realisation& r,
+                    double scale, DevEvolve& e) {
+  const size_t L = r.nspins, np = L * (L - 1) / 2;
+  e.pair_off = static_cast<int>(w.size());
+  w.resize(w.size() + 2 * np, 0.0);
+  for (size_t p = 0; p < r.lo.size(); ++p) {
+    const double quarter = scale * 0.25;
+    const double ff = quarter * (r.jx[p] + r.jy[p]);
+    const double fb = quarter * (r.jx[p] - r.jy[p]);
+    const size_t lo = r.lo[p], hi = r.hi[p];
+    const size_t q = lo * L - lo * (lo + 1) / 2 + (hi - lo - 1);
+    w[e.pair_off + 2 * q] = std::abs(ff) > kNegligible ? ff : 0.0;
+    w[e.pair_off + 2 * q + 1] = std::abs(fb) > kNegligible ? fb : 0.0;
+  }
+  e.field_off = static_cast<int>(w.size());
+  e.has_field = 0;
+  for (size_t s = 0; s < L; ++s) {
+    const double cx = scale * 0.5 * f[3 * s];
+    const double cy = scale * 0.5 * f[3 * s + 1];
+    const bool keep = std::abs(cx) > kNegligible || std::abs(cy) > kNegligible;
+    w.push_back(keep ? cx : 0.0);
+    w.push_back(keep ? cy : 0.0);
+    e.has_field |= keep ? 1 : 0;
+  }
+  e.zdiag_off = static_cast<int>(w.size());
+  for (size_t s = 0; s < L; ++s) {
+    const double c = 0.5 * f[3 * s + 2];
+    w.push_back(std::abs(c) <= kNegligible ? 0.0 : c);
+  }
+  std::vector<double> zz(np, 0.0);
+  for (size_t p = 0; p < r.lo.size(); ++p) {
+    const size_t lo = r.lo[p], hi = r.hi[p];
+    const double c = 0.25 * r.jz[p];
+    zz[lo * L - lo * (lo + 1) / 2 + (hi - lo - 1)] = std::abs(c) <= kNegligible ? 0.0 : c;
+  }
+  w.insert(w.end(), zz.begin(), zz.end());
+}
+
+template <class T>
+int upload(spx_ctx* c, T** d, const std::vector<T>& h) {
+  const size_t bytes = (h.size() ? h.size() : 1) * sizeof(T);
+  hipError_t e = hipMalloc(d, bytes);
+  if (e != hipSuccess) return fail(SPX_ENOMEM, "hipMalloc for the batch tables failed");
+  if (!h.empty()) SPX_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return SPX_OK;
+}
+
+bool resident_ok(const spx_batch* b) {
+  for (const Realisation& r : b->items)
+    if (r.nspins > 12) return false;
+  int depth = 0, maxd = 0;
+  for (const ProgOp& op : b->program.ops) {
+    if (op.kind == OP_REPEAT) maxd = std::max(maxd, ++depth);
+    if (op.kind == OP_END) --depth;
+  }
+  return maxd <= 16;
+}
+
+}  // namespace
+
+// All realisations in one persistent launch (L <= 12).
+int batch_run_resident(spx_batch* b, double* times, double* values) {
+  spx_ctx* c = b->ctx;
+  const std::vector<ProgOp>& prog = b->program.ops;
+  const size_t n = b->items.size();
+  const size_t points = program_measurements(b->program, 0, prog.size());
+
+  std::vector<DevOp> ops;
+  std::vector<DevPulse> pulses;
+  std::vector<const ProgOp*> evolves;
+  for (const ProgOp& op : prog) {
+    DevOp d{op.kind, 0, static_cast<int>(op.match), static_cast<int>(op.nsteps), op.mode,
+            op.autonorm ? 1 : 0};
+    if (op.kind == OP_EVOLVE) {
+      d.arg = static_cast<int>(evolves.size());
+      evolves.push_back(&op);
+    } else if (op.kind == OP_PULSE) {
+      // Axis3::normalized + Rotation2x2::make (src/core/Gate.cpp:20-51)
+      const double len = std::sqrt(op.axis[0] * op.axis[0] + op.axis[1] * op.axis[1] + op.axis[2] * op.axis[2]);
+      const double nx = op.axis[0] / len, ny = op.axis[1] / len, nz = op.axis[2] / len;
+      const double cs = std::cos(0.5 * op.angle), sn = std::sin(0.5 * op.angle);
+      DevPulse p{{cs, sn * nz, sn * ny, -sn * nx, -sn * ny, -sn * nx, cs, -sn * nz}, 0u, 0};
+      if (op.targets.empty()) p.targets = 0xffffffffu;
+      for (uint32_t tg : op.targets) p.targets |= 1u << tg;
+      d.arg = static_cast<int>(pulses.size());
+      pulses.push_back(p);
+    } else if (op.kind == OP_REPEAT) {
+      d.arg = static_cast<int>(op.count);
+    }
+    ops.push_back(d);
+  }
+
+  // per-realisation expansions and weights, built on the host
+  std::vector<DevEvolve> evo;
+  std::vector<DevReal> reals(n);
+  std::vector<double> w, psi;
+  std::vector<double> cost(n, 0.0);
+  long long psi_off = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const Realisation& r = b->items[i];
+    DevReal& R = reals[i];
+    R.nspins = static_cast<int>(r.nspins);
+    R.evolve_off = static_cast<int>(evo.size());
+    for (const ProgOp* op : evolves) {
+      std::vector<double> tmp;
+      const std::vector<double>& f = step_fields(*op, r.fields, r.nspins, tmp);
+      DevEvolve e{};
+      const double sb = bound_of(f, r);
+      std::vector<double> coeffs;
+      double eb = 1.0;
+      SPX_TRY(cheb_build(op->mode, sb, op->tau, b->epsilon, 6000, coeffs, eb));
+      e.bound = eb;
+      e.order = static_cast<int>(coeffs.size()) - 1;
+      e.coef_off = static_cast<int>(w.size());
+      w.insert(w.end(), coeffs.begin(), coeffs.end());
+      append_weights(w, f, r, e.order > 0 ? -2.0 / eb : 1.0, e);
+      cost[i] += static_cast<double>(e.order) * static_cast<double>(op->nsteps);
+      evo.push_back(e);
+    }
+    DevEvolve me{};
+    append_weights(w, r.fields, r, 1.0, me);
+    R.measure_off = static_cast<int>(evo.size());
+    evo.push_back(me);
+    R.psi_off = psi_off;
+    psi.insert(psi.end(), r.psi0.begin(), r.psi0.end());
+    psi_off += static_cast<long long>(r.psi0.size());
+    R.out_off = static_cast<long long>(i * points * b->width);
+    cost[i] *= static_cast<double>(1u << r.nspins);
+  }
+  if (w.size() > static_cast<size_t>(0x7fffffff)) return fail(SPX_ENOMEM, "batch tables too large");
+  // realisations grouped by L (one persistent launch per group, queue
+  // position = group offset), longest first within a group
+  std::vector<int> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = static_cast<int>(i);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (reals[x].nspins != reals[y].nspins) return reals[x].nspins > reals[y].nspins;
+    return cost[x] > cost[y];
+  });
+
+  DevOp* d_ops = nullptr;
+  DevPulse* d_pulses = nullptr;
+  DevEvolve* d_evo = nullptr;
+  DevReal* d_reals = nullptr;
+  int *d_order = nullptr, *d_codes = nullptr, *d_spins = nullptr, *d_queue = nullptr;
+  double *d_w = nullptr, *d_psi = nullptr, *d_out = nullptr;
+  std::vector<int> groups;  // start offsets into `order` per L-group
+  for (size_t i = 0; i < n; ++i)
+    if (i == 0 || reals[order[i]].nspins != reals[order[i - 1]].nspins) groups.push_back(static_cast<int>(i));
+  groups.push_back(static_cast<int>(n));
+  std::vector<int> queue0(2 * groups.size(), 0);   // one counter per group, flag at [1]
+  int rc = SPX_OK;
+  const size_t nout = n * points * b->width;
+  if (rc == SPX_OK) rc = upload(c, &d_ops, ops);
+  if (rc == SPX_OK) rc = upload(c, &d_pulses, pulses);
+  if (rc == SPX_OK) rc = upload(c, &d_evo, evo);
+  if (rc == SPX_OK) rc = upload(c, &d_reals, reals);
+  if (rc == SPX_OK) rc = upload(c, &d_order, order);
+  if (rc == SPX_OK) rc = upload(c, &d_codes, b->obs_codes);
+  if (rc == SPX_OK) rc = upload(c, &d_spins, b->obs_spins);
+  if (rc == SPX_OK) rc = upload(c, &d_queue, queue0);
+  if (rc == SPX_OK) rc = upload(c, &d_w, w);
+  if (rc == SPX_OK) rc = upload(c, &d_psi, psi);
+  if (rc == SPX_OK) rc = upload(c, &d_out, std::vector<double>(nout ? nout : 1, 0.0));
+  if (rc == SPX_OK) rc = hipStreamSynchronize(c->stream) == hipSuccess ? SPX_OK : fail(SPX_EDEVICE, "upload");
+  for (size_t gi = 0; rc == SPX_OK && gi + 1 < groups.size(); ++gi) {
+    const int beg = groups[gi], cnt = groups[gi + 1] - groups[gi];
+    const int L = reals[order[beg]].nspins;
+    const int nblocks = std::min(cnt, 2 * c->arch_cus);
+    rc = launch_resident(c, L, d_ops, static_cast<int>(ops.size()), d_pulses, d_evo, d_reals,
+                         d_order + beg, cnt, d_w, d_psi, d_out, static_cast<int>(b->width), d_codes,
+                         d_spins, static_cast<int>(b->obs_codes.size()), b->reference_axis,
+                         d_queue + 2 * gi, nblocks);
+  }
+  std::vector<int> flags(queue0.size(), 0);
+  if (rc == SPX_OK) {
+    hipError_t e1 = hipMemcpyAsync(values, d_out, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e2 = hipMemcpyAsync(flags.data(), d_queue, flags.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                   c->stream);
+    hipError_t e3 = hipStreamSynchronize(c->stream);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+      rc = fail(SPX_EDEVICE, std::string("resident batch: ") + hipGetErrorString(e3));
+  }
+  for (void* p : {static_cast<void*>(d_ops), static_cast<void*>(d_pulses), static_cast<void*>(d_evo),
+                  static_cast<void*>(d_reals), static_cast<void*>(d_order), static_cast<void*>(d_codes),
+                  static_cast<void*>(d_spins), static_cast<void*>(d_queue), static_cast<void*>(d_w),
+                  static_cast<void*>(d_psi), static_cast<void*>(d_out)})
+    if (p) hipFree(p);
+  SPX_TRY(rc);
+  for (size_t gi = 0; gi + 1 < groups.size(); ++gi)
+    if (flags[2 * gi + 1]) return fail(SPX_EDOMAIN, "cannot normalize a zero state");
+  // times: Sequence::run's clock (src/sequence/Sequence.cpp:47)
+  if (times) {
+    double now = 0.0;
+    size_t row = 0;
+    std::vector<std::pair<size_t, size_t>> stack;
+    for (size_t pc = 0; pc < prog.size(); ++pc) {
+      const ProgOp& op = prog[pc];
+      if (op.kind == OP_EVOLVE) {
+        now += op.tau * static_cast<double>(op.nsteps);
+      } else if (op.kind == OP_MEASURE) {
+        times[row++] = now;
+      } else if (op.kind == OP_REPEAT) {
+        if (op.count == 0) pc = op.match;
+        else stack.emplace_back(pc, op.count);
+      } else if (op.kind == OP_END) {
+        if (--stack.back().second > 0) pc = stack.back().first;
+        else stack.pop_back();
+      }
+    }
   }
   return SPX_OK;
 }
@@ -224,6 +457,11 @@ int spx_batch_clear(spx_batch* b) {
 
 int spx_batch_run(spx_batch* b, double* times, double* values) {
   if (b->items.empty()) return fail(SPX_EINVAL, "an ensemble needs at least one realisation");
+  static const int forced = [] {
+    const char* e = std::getenv("SPX_BATCH_STREAMING");
+    return e ? std::atoi(e) : 0;
+  }();
+  if (!forced && resident_ok(b)) return batch_run_resident(b, times, values);
   return batch_run_streaming(b, 0, b->items.size(), times, values);
 }
 

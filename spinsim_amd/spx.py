@@ -121,6 +121,8 @@ SIGNATURES = {
     "spx_program_destroy": (C.c_int, [_vp]),
     "spx_program_evolve": (C.c_int, [_vp, C.c_double, C.c_size_t, C.c_int, C.c_int, _dp,
                                      C.c_size_t]),
+    "spx_program_evolve_uniform": (C.c_int, [_vp, C.c_double, C.c_size_t, C.c_int, C.c_int,
+                                             C.c_double, C.c_double, C.c_double]),
     "spx_program_pulse": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _u32p,
                                     C.c_size_t]),
     "spx_program_measure": (C.c_int, [_vp]),
@@ -458,6 +460,10 @@ class Program:
         else:
             f = np.ascontiguousarray(fields, dtype=float).reshape(-1)
             check(lib().spx_program_evolve(self.h, tau, nsteps, mode, int(autonorm), _d(f), f.size // 3))
+        return self
+
+    def evolve_uniform(self, tau, nsteps=1, hx=0.0, hy=0.0, hz=0.0, mode=REAL, autonorm=False):
+        check(lib().spx_program_evolve_uniform(self.h, tau, nsteps, mode, int(autonorm), hx, hy, hz))
         return self
 
     def pulse(self, axis, angle, targets=()):

@@ -351,3 +351,50 @@ def test_config0_cpmg_echoes_match_oracle_per_realisation(ctx):
     assert np.array_equal(gt, times)
     assert float(np.max(np.abs(gv - per))) < 1e-10
     assert float(np.max(np.abs(gv.mean(axis=0) - mean))) < 1e-10
+
+
+def xy8(P, L, blocks, omega=50.0, oracle=False):
+    """XY-8 (X Y X Y Y X Y X) with finite pulses: each pulse is an evolve of
+    tau_p = pi/omega under hx = omega (X) or hy = omega (Y) on every spin,
+    preceded by free evolution 10 x 0.05; a measure after each block."""
+    tp = math.pi / omega
+    P.measure()
+    P.repeat(blocks)
+    for ax in "XYXYYXYX":
+        P.evolve(0.05, 10)
+        hx, hy = (omega, 0.0) if ax == "X" else (0.0, omega)
+        if oracle:
+            f = np.zeros((L, 3))
+            f[:, 0], f[:, 1] = hx, hy
+            P.evolve(tp, 1, fields=f)
+        else:
+            P.evolve_uniform(tp, 1, hx=hx, hy=hy, hz=0.0)
+    P.measure()
+    P.end()
+    return P
+
+
+def test_resident_batch_uneven_sizes_finite_pulses_match_oracle(ctx):
+    """The batched resident runner on realisations of L = 8..12 (L from the
+    index), finite XY-8 pulses: every realisation's rows within 1e-10 of the
+    oracle's, and a second run is bitwise identical."""
+    banner("resident batch uneven L XY-8")
+    obs = ((O.OBS_CORR_X, 0), (O.OBS_NORM, 0), (O.OBS_MAG_Z, 1), (O.OBS_ENERGY, 0))
+    n = 10
+    Ls = [8 + (i % 5) for i in range(n)]
+    b = spx.Batch(ctx, xy8(spx.Program(), 0, 1), obs, epsilon=1e-7, reference_axis=0)
+    want = []
+    for i, L in enumerate(Ls):
+        fields, c, psi0 = realisation_inputs(L, 20240726, i)
+        b.add(L, fields, c.lo, c.hi, c.jx, c.jy, c.jz, psi0)
+        cfg = O.Config(L=L, program=xy8(O.Program(), L, 1, oracle=True), observables=obs,
+                       field_disorder=0.5, min_separation=0.5, angular=True)
+        t, v = O.run_realization(cfg, i)
+        want.append(v)
+    gt, gv = b.run()
+    assert np.allclose(gt, t)
+    for i in range(n):
+        err = float(np.max(np.abs(gv[i] - want[i])))
+        assert err < 1e-10, (i, Ls[i], err)
+    _, gv2 = b.run()
+    assert np.array_equal(gv, gv2)
