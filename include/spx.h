@@ -196,6 +196,24 @@ int spx_batch_run(spx_batch* b, double* times, double* values);
 int spx_batch_width(const spx_batch* b, size_t* width);
 int spx_batch_clear(spx_batch* b);
 
+/* ---------------------------------------------- Disorder draws (a16)
+ * Host-side draws of realisation (seed, index) in runRealization's order
+ * (src/io/Experiment.cpp:10-55): the 2D dipolar lattice of
+ * generateDipolarLattice2D (src/ensemble/Ensemble.cpp:60-104; `angular`
+ * multiplies by 1 - 3 cos^2 of the angle to the in-plane x axis), then
+ * the Gaussian Sz disorder of applyFieldDisorder (Ensemble.cpp:106-112),
+ * from Rng::forRealization (include/spinsim/ensemble/Rng.hpp:25-58).
+ * Returns SPX_ECONVERGE when a spin cannot be placed at min_separation
+ * (the reference loops forever there). */
+int spx_dipolar_realisation(uint64_t seed, size_t index, size_t nspins, double scale,
+                            double min_separation, int angular, double disorder, uint32_t* lo,
+                            uint32_t* hi, double* jx, double* jy, double* jz, double* hz);
+/* Realisations first .. first+count-1 as dense symmetric J [count][L][L]
+ * (the secular jz of each pair) and fields h [count][L], drawn on a pool
+ * of host threads; the result does not depend on the pool size. */
+int spx_dipolar_graphs(uint64_t seed, size_t first, size_t count, size_t nspins, double scale,
+                       double min_separation, int angular, double disorder, double* J, double* h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,566 @@
+// The surrogate network of the control variate (include/spx_gnn.h):
+// forward, backward and Adam on the device. Every matrix product is one
+// strided GEMM kernel on BF16 MFMA (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation): a wavefront owns a 16x16 tile of C, four wavefronts a
+// 32x32 block, K is walked 16 at a time. Strides give the transposed
+// operands of the backward products without copies.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "spx_common.hpp"
+#include "spx_ensemble.hpp"
+#include "spx_gnn.h"
+
+namespace spx {
+namespace {
+
+constexpr int H = 128;       // hidden width
+constexpr int F = 16;        // padded input features
+constexpr int NLAYER = 3;
+constexpr size_t kPredictChunk = 4096;  // graphs per chunk of a bulk prediction
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return static_cast<uint16_t>(u >> 16);
+}
+
+struct GemmArgs {
+  const uint16_t* a;
+  long sam, sak;
+  const uint16_t* b;
+  long sbk, sbn;
+  float* c;
+  long scm, scn;
+  int M, N, K;
+  const float* bias;   // [N] or null
+  float* preact;       // store the pre-activation (before ReLU) or null
+  int relu;
+  int accumulate;
+};
+
+// C = A B (+ C) (+ bias), optional ReLU; A is M x K, B is K x N, bf16.
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m0 = blockIdx.y * 32 + (wave >> 1) * 16;
+  const int n0 = blockIdx.x * 32 + (wave & 1) * 16;
+  const int r = m0 + (lane & 15);
+  const int cidx = n0 + (lane & 15);
+  const int kq = (lane >> 4) * 4;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    bf16x4 av, bv;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = k0 + kq + i;
+      av[i] = (r < g.M && k < g.K) ? static_cast<short>(g.a[r * g.sam + k * g.sak]) : short(0);
+      bv[i] = (cidx < g.N && k < g.K) ? static_cast<short>(g.b[k * g.sbk + cidx * g.sbn]) : short(0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+  if (cidx >= g.N) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = m0 + kq + i;
+    if (row >= g.M) continue;
+    float v = acc[i];
+    if (g.bias) v += g.bias[cidx];
+    float* dst = g.c + row * g.scm + cidx * g.scn;
+    if (g.accumulate) v += *dst;
+    if (g.preact) g.preact[row * g.scm + cidx * g.scn] = v;
+    if (g.relu) v = v > 0.f ? v : 0.f;
+    *dst = v;
+  }
+}
+
+__global__ void k_cast(const float* __restrict__ x, uint16_t* __restrict__ y, long n) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = to_bf16(x[i]);
+}
+
+// Edge weights and node features of one graph per block, one thread per
+// spin (L <= 30): x = J / s_J, d_i = sum_j |x_ij|,
+// A_ij = x_ij / sqrt((1 + d_i)(1 + d_j)),
+// f_i = (h_i / s_h, d_i, log(1 + sum_j x_ij^2), 1, 0...) in bf16.
+__global__ __launch_bounds__(32) void k_graph_inputs(const float* __restrict__ J, const float* __restrict__ h,
+                                                     float* __restrict__ A, uint16_t* __restrict__ feat, int L,
+                                                     float inv_sj, float inv_sh) {
+  __shared__ float deg[32];
+  const int g = blockIdx.x, i = threadIdx.x;
+  const float* Jr = J + (static_cast<long>(g) * L + i) * L;
+  float d = 0.f, s2 = 0.f;
+  if (i < L) {
+    for (int j = 0; j < L; ++j) {
+      const float x = Jr[j] * inv_sj;
+      d += fabsf(x);
+      s2 += x * x;
+    }
+    uint16_t* f = feat + (static_cast<long>(g) * L + i) * F;
+    f[0] = to_bf16(h[static_cast<long>(g) * L + i] * inv_sh);
+    f[1] = to_bf16(d);
+    f[2] = to_bf16(log1pf(s2));
+    f[3] = to_bf16(1.f);
+    for (int k = 4; k < F; ++k) f[k] = 0;
+  }
+  deg[i] = d;
+  __syncthreads();
+  if (i >= L) return;
+  float* Ar = A + (static_cast<long>(g) * L + i) * L;
+  for (int j = 0; j < L; ++j) Ar[j] = Jr[j] * inv_sj / sqrtf((1.f + d) * (1.f + deg[j]));
+}
+
+// out[g,i,:] (+)= sum_j A[g,i,j] x[g,j,:]  (fp32)
+__global__ void k_mix(const float* __restrict__ J, const float* __restrict__ x, float* __restrict__ out,
+                      int L, int accumulate) {
+  const int g = blockIdx.x, i = blockIdx.y, c = threadIdx.x;
+  const float* Jr = J + (static_cast<long>(g) * L + i) * L;
+  const float* xg = x + static_cast<long>(g) * L * H;
+  float s = 0.f;
+  for (int j = 0; j < L; ++j) s += Jr[j] * xg[j * H + c];
+  float* o = out + (static_cast<long>(g) * L + i) * H + c;
+  *o = accumulate ? *o + s : s;
+}
+
+// [x | m] in bf16, rows of 2H
+__global__ void k_concat(const float* __restrict__ x, const float* __restrict__ m, uint16_t* __restrict__ cat,
+                         long rows) {
+  const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= rows * 2 * H) return;
+  const long row = t / (2 * H);
+  const int col = static_cast<int>(t % (2 * H));
+  cat[t] = to_bf16(col < H ? x[row * H + col] : m[row * H + col - H]);
+}
+
+__global__ void k_pool(const float* __restrict__ x, float* __restrict__ p, int L) {
+  const int g = blockIdx.x, c = threadIdx.x;
+  float s = 0.f;
+  for (int i = 0; i < L; ++i) s += x[(static_cast<long>(g) * L + i) * H + c];
+  p[g * H + c] = s / static_cast<float>(L);
+}
+
+__global__ void k_pool_bwd(const float* __restrict__ dp, float* __restrict__ dx, int L) {
+  const int g = blockIdx.x, i = blockIdx.y, c = threadIdx.x;
+  dx[(static_cast<long>(g) * L + i) * H + c] = dp[g * H + c] / static_cast<float>(L);
+}
+
+// d (MSE) / dy and the loss terms (summed on the host in a fixed order)
+__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, float* __restrict__ dy,
+                      float* __restrict__ sq, long n, float scale) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = y[i] - t[i];
+  dy[i] = 2.f * d * scale;
+  sq[i] = d * d;
+}
+
+__global__ void k_relu_bwd(const float* __restrict__ dx, const float* __restrict__ z, float* __restrict__ dz,
+                           long n) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dz[i] = z[i] > 0.f ? dx[i] : 0.f;
+}
+
+// column sums of a rows x cols matrix (fixed order per column)
+__global__ void k_colsum(const float* __restrict__ x, float* __restrict__ out, long rows, int cols) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (long r = 0; r < rows; ++r) s += x[r * cols + c];
+  out[c] = s;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long n, float lr, float b1t, float b2t) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  m[i] = 0.9f * m[i] + 0.1f * gi;
+  v[i] = 0.999f * v[i] + 0.001f * gi * gi;
+  const float mh = m[i] / (1.f - b1t), vh = v[i] / (1.f - b2t);
+  p[i] -= lr * mh / (sqrtf(vh) + 1e-8f);
+}
+
+inline unsigned nb(long n, int b = 256) { return static_cast<unsigned>((n + b - 1) / b); }
+
+}  // namespace
+}  // namespace spx
+
+using namespace spx;
+
+struct spx_gnn {
+  spx_ctx* ctx = nullptr;
+  int P = 0;
+  double sh = 1.0, sj = 1.0;
+  // parameter layout
+  size_t off_w[NLAYER + 2] = {}, off_b[NLAYER + 2] = {};
+  size_t nparam = 0;
+  float* p = nullptr;      // masters
+  float* grad = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  uint16_t* pbf = nullptr; // bf16 copy of the parameters
+  long step = 0;
+  // activations (sized for the largest batch seen)
+  long cap_rows = 0, cap_graphs = 0, cap_L = 0;
+  float *Jin = nullptr, *hin = nullptr, *J = nullptr;  // raw couplings/fields (FP32), edge weights
+  uint16_t* feat = nullptr;
+  float *z[NLAYER + 1] = {}, *x[NLAYER + 1] = {}, *mix[NLAYER + 1] = {};
+  uint16_t* cat[NLAYER + 1] = {};
+  float *pool = nullptr, *y = nullptr, *tgt = nullptr, *dy = nullptr, *sq = nullptr, *dpool = nullptr;
+  uint16_t *poolbf = nullptr, *dybf = nullptr, *dzbf = nullptr;
+  float *dx = nullptr, *dz = nullptr, *dcat = nullptr, *dm = nullptr;
+  // pinned double buffers of the bulk prediction
+  size_t pin_graphs = 0, pin_L = 0;
+  float *pin_J[2] = {}, *pin_h[2] = {}, *pin_y[2] = {};
+  hipEvent_t ev[2] = {};
+};
+
+namespace {
+
+size_t wsize(const spx_gnn* g, int l) {
+  if (l == 0) return static_cast<size_t>(H) * F;
+  if (l <= NLAYER) return static_cast<size_t>(H) * 2 * H;
+  return static_cast<size_t>(g->P) * H;
+}
+
+size_t bsize(const spx_gnn* g, int l) { return l <= NLAYER ? H : g->P; }
+
+int gemm(spx_ctx* c, const uint16_t* a, long sam, long sak, const uint16_t* b, long sbk, long sbn, float* out,
+         long scm, long scn, int M, int N, int K, const float* bias, float* preact, int relu, int acc) {
+  GemmArgs g{a, sam, sak, b, sbk, sbn, out, scm, scn, M, N, K, bias, preact, relu, acc};
+  const dim3 grid((N + 31) / 32, (M + 31) / 32);
+  k_gemm<<<grid, 256, 0, c->stream>>>(g);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+template <class T>
+int dalloc(T** p, long n) {
+  if (*p) SPX_HIP(hipFree(*p));
+  SPX_HIP(hipMalloc(p, (n > 0 ? n : 1) * sizeof(T)));
+  return SPX_OK;
+}
+
+int reserve(spx_gnn* g, long G, long L) {
+  const long rows = G * L;
+  if (rows <= g->cap_rows && G <= g->cap_graphs && L * L * G <= g->cap_L) return SPX_OK;
+  const long R = std::max(rows, g->cap_rows), GG = std::max(G, g->cap_graphs);
+  const long JJ = std::max(L * L * G, g->cap_L);
+  SPX_TRY(dalloc(&g->Jin, JJ));
+  SPX_TRY(dalloc(&g->hin, R));
+  SPX_TRY(dalloc(&g->J, JJ));
+  SPX_TRY(dalloc(&g->feat, R * F));
+  for (int l = 0; l <= NLAYER; ++l) {
+    SPX_TRY(dalloc(&g->z[l], R * H));
+    SPX_TRY(dalloc(&g->x[l], R * H));
+    SPX_TRY(dalloc(&g->mix[l], R * H));
+    SPX_TRY(dalloc(&g->cat[l], R * 2 * H));
+  }
+  SPX_TRY(dalloc(&g->pool, GG * H));
+  SPX_TRY(dalloc(&g->poolbf, GG * H));
+  SPX_TRY(dalloc(&g->y, GG * g->P));
+  SPX_TRY(dalloc(&g->tgt, GG * g->P));
+  SPX_TRY(dalloc(&g->dy, GG * g->P));
+  SPX_TRY(dalloc(&g->dybf, GG * g->P));
+  SPX_TRY(dalloc(&g->sq, GG * g->P));
+  SPX_TRY(dalloc(&g->dpool, GG * H));
+  SPX_TRY(dalloc(&g->dx, R * H));
+  SPX_TRY(dalloc(&g->dz, R * H));
+  SPX_TRY(dalloc(&g->dzbf, R * H));
+  SPX_TRY(dalloc(&g->dcat, R * 2 * H));
+  SPX_TRY(dalloc(&g->dm, R * H));
+  g->cap_rows = R;
+  g->cap_graphs = GG;
+  g->cap_L = JJ;
+  return SPX_OK;
+}
+
+// Upload the couplings (FP32), the fields and, when given, the targets of a
+// batch. The backward pass takes the message transpose as A itself, so J
+// must be symmetric.
+int load_batch(spx_gnn* g, long G, long L, const double* J, const double* h, const float* target) {
+  if (!J || !h) return fail(SPX_EINVAL, "null couplings or fields");
+  for (long gi = 0; gi < G; ++gi)
+    for (long i = 0; i < L; ++i)
+      for (long j = 0; j < i; ++j)
+        if (J[(gi * L + i) * L + j] != J[(gi * L + j) * L + i])
+          return fail(SPX_EINVAL, "coupling matrix of graph " + std::to_string(gi) + " is not symmetric");
+  SPX_TRY(reserve(g, G, L));
+  std::vector<float> Jf(G * L * L), hf(G * L);
+  for (long q = 0; q < G * L * L; ++q) Jf[q] = static_cast<float>(J[q]);
+  for (long q = 0; q < G * L; ++q) hf[q] = static_cast<float>(h[q]);
+  spx_ctx* c = g->ctx;
+  SPX_HIP(hipMemcpyAsync(g->Jin, Jf.data(), Jf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  SPX_HIP(hipMemcpyAsync(g->hin, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  if (target)
+    SPX_HIP(hipMemcpyAsync(g->tgt, target, G * g->P * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  SPX_HIP(hipStreamSynchronize(c->stream));  // pageable sources
+  return SPX_OK;
+}
+
+int forward(spx_gnn* g, long G, long L) {
+  spx_ctx* c = g->ctx;
+  const long R = G * L;
+  k_graph_inputs<<<static_cast<unsigned>(G), 32, 0, c->stream>>>(g->Jin, g->hin, g->J, g->feat, static_cast<int>(L),
+                                                                  static_cast<float>(1.0 / g->sj),
+                                                                  static_cast<float>(1.0 / g->sh));
+  k_cast<<<nb(static_cast<long>(g->nparam)), 256, 0, c->stream>>>(g->p, g->pbf, static_cast<long>(g->nparam));
+  // x0 = relu(feat W0^T + b0)
+  SPX_TRY(gemm(c, g->feat, F, 1, g->pbf + g->off_w[0], 1, F, g->x[0], H, 1, static_cast<int>(R), H, F,
+               g->p + g->off_b[0], g->z[0], 1, 0));
+  for (int l = 1; l <= NLAYER; ++l) {
+    k_mix<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(
+        g->J, g->x[l - 1], g->mix[l], static_cast<int>(L), 0);
+    k_concat<<<nb(R * 2 * H), 256, 0, c->stream>>>(g->x[l - 1], g->mix[l], g->cat[l], R);
+    SPX_TRY(gemm(c, g->cat[l], 2 * H, 1, g->pbf + g->off_w[l], 1, 2 * H, g->x[l], H, 1, static_cast<int>(R),
+                 H, 2 * H, g->p + g->off_b[l], g->z[l], 1, 0));
+  }
+  k_pool<<<static_cast<unsigned>(G), H, 0, c->stream>>>(g->x[NLAYER], g->pool, static_cast<int>(L));
+  k_cast<<<nb(G * H), 256, 0, c->stream>>>(g->pool, g->poolbf, G * H);
+  SPX_TRY(gemm(c, g->poolbf, H, 1, g->pbf + g->off_w[NLAYER + 1], 1, H, g->y, g->P, 1, static_cast<int>(G),
+               g->P, H, g->p + g->off_b[NLAYER + 1], nullptr, 0, 0));
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+int backward(spx_gnn* g, long G, long L, float* loss) {
+  spx_ctx* c = g->ctx;
+  const long R = G * L;
+  const int P = g->P;
+  k_mse<<<nb(G * P), 256, 0, c->stream>>>(g->y, g->tgt, g->dy, g->sq, G * P, 1.f / static_cast<float>(G * P));
+  k_cast<<<nb(G * P), 256, 0, c->stream>>>(g->dy, g->dybf, G * P);
+  float* gr = g->grad;
+  // readout
+  SPX_TRY(gemm(c, g->dybf, 1, P, g->poolbf, H, 1, gr + g->off_w[NLAYER + 1], H, 1, P, H, static_cast<int>(G),
+               nullptr, nullptr, 0, 0));
+  k_colsum<<<nb(P, 64), 64, 0, c->stream>>>(g->dy, gr + g->off_b[NLAYER + 1], G, P);
+  SPX_TRY(gemm(c, g->dybf, P, 1, g->pbf + g->off_w[NLAYER + 1], H, 1, g->dpool, H, 1, static_cast<int>(G), H, P,
+               nullptr, nullptr, 0, 0));
+  k_pool_bwd<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(g->dpool, g->dx,
+                                                                                           static_cast<int>(L));
+  for (int l = NLAYER; l >= 0; --l) {
+    k_relu_bwd<<<nb(R * H), 256, 0, c->stream>>>(g->dx, g->z[l], g->dz, R * H);
+    k_cast<<<nb(R * H), 256, 0, c->stream>>>(g->dz, g->dzbf, R * H);
+    k_colsum<<<nb(H, 64), 64, 0, c->stream>>>(g->dz, gr + g->off_b[l], R, H);
+    if (l == 0) {
+      SPX_TRY(gemm(c, g->dzbf, 1, H, g->feat, F, 1, gr + g->off_w[0], F, 1, H, F, static_cast<int>(R), nullptr,
+                   nullptr, 0, 0));
+      break;
+    }
+    SPX_TRY(gemm(c, g->dzbf, 1, H, g->cat[l], 2 * H, 1, gr + g->off_w[l], 2 * H, 1, H, 2 * H, static_cast<int>(R),
+                 nullptr, nullptr, 0, 0));
+    // d[x | m] = dz W_l; dx_{l-1} = first half + J^T (second half), J symmetric
+    SPX_TRY(gemm(c, g->dzbf, H, 1, g->pbf + g->off_w[l], 2 * H, 1, g->dcat, 2 * H, 1, static_cast<int>(R), 2 * H,
+                 H, nullptr, nullptr, 0, 0));
+    SPX_HIP(hipMemcpy2DAsync(g->dx, H * sizeof(float), g->dcat, 2 * H * sizeof(float), H * sizeof(float), R,
+                             hipMemcpyDeviceToDevice, c->stream));
+    SPX_HIP(hipMemcpy2DAsync(g->dm, H * sizeof(float), g->dcat + H, 2 * H * sizeof(float), H * sizeof(float), R,
+                             hipMemcpyDeviceToDevice, c->stream));
+    k_mix<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(g->J, g->dm, g->dx,
+                                                                                       static_cast<int>(L), 1);
+  }
+  SPX_HIP(hipGetLastError());
+  if (loss) {
+    std::vector<float> sq(G * P);
+    SPX_HIP(hipMemcpyAsync(sq.data(), g->sq, sq.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SPX_HIP(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (float v : sq) s += v;
+    *loss = static_cast<float>(s / static_cast<double>(G * P));
+  }
+  return SPX_OK;
+}
+
+int reserve_pinned(spx_gnn* g, size_t G, size_t L) {
+  if (G <= g->pin_graphs && L <= g->pin_L) return SPX_OK;
+  for (int b = 0; b < 2; ++b) {
+    for (float** q : {&g->pin_J[b], &g->pin_h[b], &g->pin_y[b]}) {
+      if (*q) SPX_HIP(hipHostFree(*q));
+      *q = nullptr;
+    }
+    SPX_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->pin_J[b]), G * L * L * sizeof(float)));
+    SPX_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->pin_h[b]), G * L * sizeof(float)));
+    SPX_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->pin_y[b]), G * g->P * sizeof(float)));
+  }
+  g->pin_graphs = G;
+  g->pin_L = L;
+  return SPX_OK;
+}
+
+int check_batch(const spx_gnn* g, size_t G, size_t L) {
+  if (!g) return fail(SPX_EINVAL, "null network");
+  if (G == 0 || L == 0 || L > kMaxSpins) return fail(SPX_EINVAL, "empty batch or too many spins");
+  return SPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spx_gnn_create(spx_ctx* ctx, size_t outputs, uint64_t seed, double scale_h, double scale_j, spx_gnn** out) {
+  if (!ctx || !out) return fail(SPX_EINVAL, "null argument");
+  if (outputs == 0 || outputs > 128) return fail(SPX_EINVAL, "outputs must be in [1, 128]");
+  if (!(scale_h > 0.0) || !(scale_j > 0.0)) return fail(SPX_EINVAL, "feature scales must be positive");
+  auto* g = new spx_gnn();
+  g->ctx = ctx;
+  g->P = static_cast<int>(outputs);
+  g->sh = scale_h;
+  g->sj = scale_j;
+  size_t off = 0;
+  for (int l = 0; l <= NLAYER + 1; ++l) {
+    g->off_w[l] = off;
+    off += wsize(g, l);
+    g->off_b[l] = off;
+    off += bsize(g, l);
+  }
+  g->nparam = off;
+  // He-uniform weights, zero biases, from a seeded host stream
+  std::vector<float> p(off, 0.f);
+  std::mt19937_64 rng(seed);
+  for (int l = 0; l <= NLAYER + 1; ++l) {
+    const double fan_in = l == 0 ? 4.0 : l <= NLAYER ? 2.0 * H : H;
+    const double lim = std::sqrt(6.0 / fan_in) * (l <= NLAYER ? 1.0 : 0.1);
+    std::uniform_real_distribution<double> u(-lim, lim);
+    for (size_t i = 0; i < wsize(g, l); ++i) {
+      const size_t col = l == 0 ? i % F : 0;
+      p[g->off_w[l] + i] = (l == 0 && col >= 4) ? 0.f : static_cast<float>(u(rng));
+    }
+  }
+  int rc = SPX_OK;
+  if (hipMalloc(&g->p, off * sizeof(float)) != hipSuccess || hipMalloc(&g->grad, off * sizeof(float)) != hipSuccess ||
+      hipMalloc(&g->m, off * sizeof(float)) != hipSuccess || hipMalloc(&g->v, off * sizeof(float)) != hipSuccess ||
+      hipMalloc(&g->pbf, off * sizeof(uint16_t)) != hipSuccess)
+    rc = fail(SPX_ENOMEM, "hipMalloc of network parameters failed");
+  else if (hipMemcpy(g->p, p.data(), off * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+           hipMemset(g->m, 0, off * sizeof(float)) != hipSuccess ||
+           hipMemset(g->v, 0, off * sizeof(float)) != hipSuccess)
+    rc = fail(SPX_EDEVICE, "initialising network parameters failed");
+  for (int b = 0; b < 2 && rc == SPX_OK; ++b)
+    if (hipEventCreateWithFlags(&g->ev[b], hipEventDisableTiming) != hipSuccess)
+      rc = fail(SPX_EDEVICE, "hipEventCreate failed");
+  if (rc != SPX_OK) {
+    spx_gnn_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return SPX_OK;
+}
+
+int spx_gnn_destroy(spx_gnn* g) {
+  if (!g) return SPX_OK;
+  for (void* q : {static_cast<void*>(g->p), static_cast<void*>(g->grad), static_cast<void*>(g->m),
+                  static_cast<void*>(g->v), static_cast<void*>(g->pbf), static_cast<void*>(g->J),
+                  static_cast<void*>(g->Jin), static_cast<void*>(g->hin), static_cast<void*>(g->feat),
+                  static_cast<void*>(g->pool), static_cast<void*>(g->y), static_cast<void*>(g->tgt),
+                  static_cast<void*>(g->dy), static_cast<void*>(g->sq), static_cast<void*>(g->dpool),
+                  static_cast<void*>(g->poolbf), static_cast<void*>(g->dybf), static_cast<void*>(g->dzbf),
+                  static_cast<void*>(g->dx), static_cast<void*>(g->dz), static_cast<void*>(g->dcat),
+                  static_cast<void*>(g->dm)})
+    if (q) (void)hipFree(q);
+  for (int l = 0; l <= NLAYER; ++l)
+    for (void* q : {static_cast<void*>(g->z[l]), static_cast<void*>(g->x[l]), static_cast<void*>(g->mix[l]),
+                    static_cast<void*>(g->cat[l])})
+      if (q) (void)hipFree(q);
+  for (int b = 0; b < 2; ++b) {
+    for (float* q : {g->pin_J[b], g->pin_h[b], g->pin_y[b]})
+      if (q) (void)hipHostFree(q);
+    if (g->ev[b]) (void)hipEventDestroy(g->ev[b]);
+  }
+  delete g;
+  return SPX_OK;
+}
+
+int spx_gnn_num_params(const spx_gnn* g, size_t* n) {
+  if (!g || !n) return fail(SPX_EINVAL, "null argument");
+  *n = g->nparam;
+  return SPX_OK;
+}
+
+int spx_gnn_get_params(const spx_gnn* g, float* host) {
+  if (!g || !host) return fail(SPX_EINVAL, "null argument");
+  SPX_HIP(hipMemcpy(host, g->p, g->nparam * sizeof(float), hipMemcpyDeviceToHost));
+  return SPX_OK;
+}
+
+int spx_gnn_set_params(spx_gnn* g, const float* host) {
+  if (!g || !host) return fail(SPX_EINVAL, "null argument");
+  SPX_HIP(hipMemcpy(g->p, host, g->nparam * sizeof(float), hipMemcpyHostToDevice));
+  return SPX_OK;
+}
+
+int spx_gnn_predict(spx_gnn* g, size_t G, size_t L, const double* J, const double* h, float* pred) {
+  SPX_TRY(check_batch(g, G, L));
+  if (!pred) return fail(SPX_EINVAL, "null prediction buffer");
+  SPX_TRY(load_batch(g, static_cast<long>(G), static_cast<long>(L), J, h, nullptr));
+  SPX_TRY(forward(g, static_cast<long>(G), static_cast<long>(L)));
+  SPX_HIP(hipMemcpyAsync(pred, g->y, G * g->P * sizeof(float), hipMemcpyDeviceToHost, g->ctx->stream));
+  SPX_HIP(hipStreamSynchronize(g->ctx->stream));
+  return SPX_OK;
+}
+
+int spx_gnn_predict_dipolar(spx_gnn* g, uint64_t seed, size_t first, size_t count, size_t nspins, double scale,
+                            double min_separation, int angular, double disorder, float* pred) {
+  if (!g || !pred) return fail(SPX_EINVAL, "null argument");
+  const DipolarSpec s{seed, nspins, scale, min_separation, angular != 0, disorder};
+  SPX_TRY(check_spec(s));
+  if (count == 0) return SPX_OK;
+  const size_t C = std::min(count, kPredictChunk), L = nspins, P = static_cast<size_t>(g->P);
+  SPX_TRY(reserve(g, static_cast<long>(C), static_cast<long>(L)));
+  SPX_TRY(reserve_pinned(g, C, L));
+  hipStream_t st = g->ctx->stream;
+  const size_t chunks = (count + C - 1) / C;
+  // Chunk k is drawn on the host while the device runs chunk k - 1.
+  for (size_t k = 0; k <= chunks; ++k) {
+    if (k < chunks) {
+      const int b = static_cast<int>(k & 1);
+      const size_t n = std::min(C, count - k * C);
+      SPX_HIP(hipEventSynchronize(g->ev[b]));
+      SPX_TRY(draw_graphs(s, first + k * C, n, nullptr, nullptr, g->pin_J[b], g->pin_h[b], 0));
+      SPX_HIP(hipMemcpyAsync(g->Jin, g->pin_J[b], n * L * L * sizeof(float), hipMemcpyHostToDevice, st));
+      SPX_HIP(hipMemcpyAsync(g->hin, g->pin_h[b], n * L * sizeof(float), hipMemcpyHostToDevice, st));
+      SPX_TRY(forward(g, static_cast<long>(n), static_cast<long>(L)));
+      SPX_HIP(hipMemcpyAsync(g->pin_y[b], g->y, n * P * sizeof(float), hipMemcpyDeviceToHost, st));
+      SPX_HIP(hipEventRecord(g->ev[b], st));
+    }
+    if (k >= 1) {
+      const int b = static_cast<int>((k - 1) & 1);
+      const size_t n = std::min(C, count - (k - 1) * C);
+      SPX_HIP(hipEventSynchronize(g->ev[b]));
+      std::memcpy(pred + (k - 1) * C * P, g->pin_y[b], n * P * sizeof(float));
+    }
+  }
+  return SPX_OK;
+}
+
+int spx_gnn_gradient(spx_gnn* g, size_t G, size_t L, const double* J, const double* h, const float* target,
+                     float* loss, float* grad) {
+  SPX_TRY(check_batch(g, G, L));
+  if (!target) return fail(SPX_EINVAL, "null targets");
+  SPX_TRY(load_batch(g, static_cast<long>(G), static_cast<long>(L), J, h, target));
+  SPX_TRY(forward(g, static_cast<long>(G), static_cast<long>(L)));
+  SPX_TRY(backward(g, static_cast<long>(G), static_cast<long>(L), loss));
+  if (grad) {
+    SPX_HIP(hipMemcpyAsync(grad, g->grad, g->nparam * sizeof(float), hipMemcpyDeviceToHost, g->ctx->stream));
+    SPX_HIP(hipStreamSynchronize(g->ctx->stream));
+  }
+  return SPX_OK;
+}
+
+int spx_gnn_train_step(spx_gnn* g, size_t G, size_t L, const double* J, const double* h, const float* target,
+                       float lr, float* loss) {
+  SPX_TRY(spx_gnn_gradient(g, G, L, J, h, target, loss, nullptr));
+  ++g->step;
+  const float b1t = static_cast<float>(std::pow(0.9, static_cast<double>(g->step)));
+  const float b2t = static_cast<float>(std::pow(0.999, static_cast<double>(g->step)));
+  k_adam<<<nb(static_cast<long>(g->nparam)), 256, 0, g->ctx->stream>>>(g->p, g->grad, g->m, g->v,
+                                                                       static_cast<long>(g->nparam), lr, b1t, b2t);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+}  // extern "C"

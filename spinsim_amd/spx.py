@@ -21,6 +21,7 @@ _u32p = C.POINTER(C.c_uint32)
 _szp = C.POINTER(C.c_size_t)
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
+_fp = C.POINTER(C.c_float)
 
 REAL, IMAGINARY = 0, 1
 AXES = {"x": 0, "y": 1, "z": 2}
@@ -137,6 +138,20 @@ SIGNATURES = {
     "spx_batch_run": (C.c_int, [_vp, _dp, _dp]),
     "spx_batch_width": (C.c_int, [_vp, _szp]),
     "spx_batch_clear": (C.c_int, [_vp]),
+    "spx_gnn_create": (C.c_int, [_vp, C.c_size_t, C.c_uint64, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "spx_gnn_destroy": (C.c_int, [_vp]),
+    "spx_gnn_num_params": (C.c_int, [_vp, _szp]),
+    "spx_gnn_get_params": (C.c_int, [_vp, _fp]),
+    "spx_gnn_set_params": (C.c_int, [_vp, _fp]),
+    "spx_gnn_predict": (C.c_int, [_vp, C.c_size_t, C.c_size_t, _dp, _dp, _fp]),
+    "spx_gnn_predict_dipolar": (C.c_int, [_vp, C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double,
+                                          C.c_double, C.c_int, C.c_double, _fp]),
+    "spx_dipolar_realisation": (C.c_int, [C.c_uint64, C.c_size_t, C.c_size_t, C.c_double, C.c_double, C.c_int,
+                                          C.c_double, _u32p, _u32p, _dp, _dp, _dp, _dp]),
+    "spx_dipolar_graphs": (C.c_int, [C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double, C.c_double,
+                                     C.c_int, C.c_double, _dp, _dp]),
+    "spx_gnn_gradient": (C.c_int, [_vp, C.c_size_t, C.c_size_t, _dp, _dp, _fp, _fp, _fp]),
+    "spx_gnn_train_step": (C.c_int, [_vp, C.c_size_t, C.c_size_t, _dp, _dp, _fp, C.c_float, _fp]),
 }
 
 _lib = None
@@ -538,3 +553,113 @@ class Batch:
     def clear(self):
         check(lib().spx_batch_clear(self.h))
         self.count = 0
+
+
+class Dipolar:
+    """One pair record of a drawn realisation: lo, hi, jx, jy, jz, hz."""
+
+    def __init__(self, lo, hi, jx, jy, jz, hz):
+        self.lo, self.hi, self.jx, self.jy, self.jz, self.hz = lo, hi, jx, jy, jz, hz
+
+
+def dipolar_realisation(seed: int, index: int, nspins: int, scale: float = 1.0, min_separation: float = 0.5,
+                        angular: bool = True, disorder: float = 0.5) -> Dipolar:
+    """Realisation (seed, index): couplings in generation order and Sz disorder."""
+    n = nspins * (nspins - 1) // 2
+    lo = np.zeros(n, np.uint32)
+    hi = np.zeros(n, np.uint32)
+    jx, jy, jz = np.zeros(n), np.zeros(n), np.zeros(n)
+    hz = np.zeros(nspins)
+    check(lib().spx_dipolar_realisation(seed, index, nspins, scale, min_separation, int(angular), disorder,
+                                        lo.ctypes.data_as(_u32p), hi.ctypes.data_as(_u32p), _d(jx), _d(jy),
+                                        _d(jz), _d(hz)))
+    return Dipolar(lo, hi, jx, jy, jz, hz)
+
+
+def dipolar_graphs(seed: int, first: int, count: int, nspins: int, scale: float = 1.0,
+                   min_separation: float = 0.5, angular: bool = True, disorder: float = 0.5):
+    """Realisations first .. first+count-1 as J [count, L, L] and h [count, L]."""
+    J = np.zeros((count, nspins, nspins))
+    h = np.zeros((count, nspins))
+    check(lib().spx_dipolar_graphs(seed, first, count, nspins, scale, min_separation, int(angular), disorder,
+                                   _d(J), _d(h)))
+    return J, h
+
+
+class Surrogate:
+    """The graph-network surrogate of an echo curve (spx_gnn_*): couplings
+    J [G, L, L] (symmetric) and fields h [G, L] in, P curve points out."""
+
+    def __init__(self, ctx: Context, outputs: int, seed: int = 0, scale_h: float = 1.0, scale_j: float = 1.0):
+        self.ctx = ctx
+        self.outputs = outputs
+        h = _vp()
+        check(lib().spx_gnn_create(ctx.h, outputs, seed, scale_h, scale_j, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().spx_gnn_destroy(self.h)
+        except Exception:
+            pass
+
+    def num_params(self) -> int:
+        v = C.c_size_t()
+        check(lib().spx_gnn_num_params(self.h, C.byref(v)))
+        return v.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        check(lib().spx_gnn_get_params(self.h, out.ctypes.data_as(_fp)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.num_params():
+            raise ValueError("parameter vector has the wrong length")
+        check(lib().spx_gnn_set_params(self.h, p.ctypes.data_as(_fp)))
+
+    @staticmethod
+    def _graphs(J, h):
+        J = np.ascontiguousarray(J, dtype=float)
+        h = np.ascontiguousarray(h, dtype=float)
+        if J.ndim != 3 or J.shape[1] != J.shape[2] or h.shape != J.shape[:2]:
+            raise ValueError("J must be [G, L, L] and h [G, L]")
+        return J, h
+
+    def predict(self, J, h) -> np.ndarray:
+        J, h = self._graphs(J, h)
+        out = np.zeros((J.shape[0], self.outputs), dtype=np.float32)
+        check(lib().spx_gnn_predict(self.h, J.shape[0], J.shape[1], _d(J), _d(h), out.ctypes.data_as(_fp)))
+        return out
+
+    def predict_dipolar(self, seed: int, first: int, count: int, nspins: int, scale: float = 1.0,
+                        min_separation: float = 0.5, angular: bool = True, disorder: float = 0.5) -> np.ndarray:
+        out = np.zeros((count, self.outputs), dtype=np.float32)
+        check(lib().spx_gnn_predict_dipolar(self.h, seed, first, count, nspins, scale, min_separation,
+                                            int(angular), disorder, out.ctypes.data_as(_fp)))
+        return out
+
+    def _targets(self, J, target):
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        if t.shape != (J.shape[0], self.outputs):
+            raise ValueError("targets must be [G, outputs]")
+        return t
+
+    def gradient(self, J, h, target):
+        J, h = self._graphs(J, h)
+        t = self._targets(J, target)
+        loss = C.c_float()
+        grad = np.zeros(self.num_params(), dtype=np.float32)
+        check(lib().spx_gnn_gradient(self.h, J.shape[0], J.shape[1], _d(J), _d(h), t.ctypes.data_as(_fp),
+                                     C.byref(loss), grad.ctypes.data_as(_fp)))
+        return loss.value, grad
+
+    def train_step(self, J, h, target, lr: float) -> float:
+        J, h = self._graphs(J, h)
+        t = self._targets(J, target)
+        loss = C.c_float()
+        check(lib().spx_gnn_train_step(self.h, J.shape[0], J.shape[1], _d(J), _d(h), t.ctypes.data_as(_fp),
+                                       lr, C.byref(loss)))
+        return loss.value

@@ -15,8 +15,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_symbols():
-    text = open(os.path.join(ROOT, "include", "spx.h")).read()
-    return set(re.findall(r"^\s*(?:int|const char\*)\s+(spx_\w+)\s*\(", text, re.M))
+    syms = set()
+    for name in ("spx.h", "spx_gnn.h"):
+        text = open(os.path.join(ROOT, "include", name)).read()
+        syms |= set(re.findall(r"^\s*(?:int|const char\*)\s+(spx_\w+)\s*\(", text, re.M))
+    return syms
 
 
 def test_library_exports_every_declared_symbol():
@@ -72,3 +75,39 @@ def test_program_counts_measurements_through_nested_repeats():
         q.measurements()
     with pytest.raises(spx.InvalidArgument):
         spx.Program().end()
+
+
+def test_dipolar_draws_match_oracle_bitwise():
+    """The host-side disorder draws of libspx against the oracle's pinned
+    stream: couplings in generation order, then the Sz disorder."""
+    for L, angular, disorder in ((2, False, 0.0), (7, True, 0.5), (12, True, 0.3), (16, False, 1.0)):
+        for index in (0, 3, 1000):
+            c, hz, _ = O.dipolar_lattice(L, 20240726, index, scale=1.3, min_sep=0.5, angular=angular,
+                                         disorder=disorder)
+            d = spx.dipolar_realisation(20240726, index, L, 1.3, 0.5, angular, disorder)
+            assert np.array_equal(d.lo, c.lo) and np.array_equal(d.hi, c.hi)
+            assert np.array_equal(d.jz, c.jz) and np.array_equal(d.jx, c.jx) and np.array_equal(d.jy, c.jy)
+            assert np.array_equal(d.hz, hz)
+
+
+def test_dipolar_graphs_are_the_realisations_whatever_the_pool():
+    L = 9
+    J, h = spx.dipolar_graphs(7, 100, 1500, L)      # a pool of threads
+    J2, h2 = spx.dipolar_graphs(7, 1100, 40, L)     # one thread
+    assert np.array_equal(J[1000:1040], J2) and np.array_equal(h[1000:1040], h2)
+    assert np.array_equal(J, np.swapaxes(J, 1, 2))
+    for k in (0, 777, 1499):
+        d = spx.dipolar_realisation(7, 100 + k, L)
+        want = np.zeros((L, L))
+        want[d.lo, d.hi] = d.jz
+        want[d.hi, d.lo] = d.jz
+        assert np.array_equal(J[k], want) and np.array_equal(h[k], d.hz)
+
+
+def test_dipolar_draw_errors():
+    with pytest.raises(spx.InvalidArgument):
+        spx.dipolar_realisation(1, 0, 0)
+    with pytest.raises(spx.InvalidArgument):
+        spx.dipolar_realisation(1, 0, 31)
+    with pytest.raises(spx.ConvergenceError):   # 9 spins cannot sit 2 apart in a 3 x 3 box
+        spx.dipolar_realisation(1, 0, 9, min_separation=2.0)
