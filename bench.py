@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "H·ψ GB/s vs HBM peak at L=18; s to ±1e-3 echo error bar, CV vs plain MC"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the general-order kernel at L=18 (tile bits 10):
+# FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from the in-place scale
+# kernel of known size), profiles/k_dense_L18_order.md.
+MEASURED_TRAFFIC = {("k_dense<K_ORDERN, 18, 10>"): 42417 * 1024}
 BYTES_PER_AMP_ORDER = 88          # read T_{n-1}, T_{n-2}, diag; write T_n; read+write psi
 BYTES_FIRST, BYTES_SECOND = 40, 72  # order 1 (psi -> T_1), order 2 (fused psi update)
 
@@ -39,30 +43,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def synthetic_system(L: int, seed: int = 20240726, sigma: float = 0.5, min_sep: float = 0.5):
-    """Dipolar geometry + Hz disorder of the workload's shape (numpy stream)."""
-    g = np.random.default_rng(seed)
-    hw = 0.5 * math.sqrt(L)
-    x, y = [0.0], [0.0]
-    while len(x) < L:
-        px, py = hw * (2 * g.random() - 1), hw * (2 * g.random() - 1)
-        if all(not (abs(px - x[j]) < min_sep and abs(py - y[j]) < min_sep) for j in range(len(x))):
-            x.append(px)
-            y.append(py)
-    lo, hi, jz = [], [], []
-    for i in range(1, L):
-        for j in range(i):
-            dx, dy = x[i] - x[j], y[i] - y[j]
-            r2 = dx * dx + dy * dy
-            lo.append(j)
-            hi.append(i)
-            jz.append((1.0 - 3.0 * dx * dx / r2) / (r2 * math.sqrt(r2)))
-    jz = np.array(jz)
+def synthetic_system(L: int, seed: int = 20240726, index: int = 0, sigma: float = 0.5, min_sep: float = 0.5):
+    """Realisation (seed, index) of the dipolar ensemble, drawn by libspx in
+    the reference's order (secular (1 - 3cos^2 theta)/r^3 couplings at unit
+    density, min spacing 0.5, Gaussian Hz of spread sigma); a normalised
+    Gaussian initial state from numpy."""
+    from spinsim_amd import spx
+    d = spx.dipolar_realisation(seed, index, L, 1.0, min_sep, True, sigma)
     fields = np.zeros((L, 3))
-    fields[:, 2] = sigma * g.standard_normal(L)
-    psi = g.standard_normal(2 << L)
+    fields[:, 2] = d.hz
+    psi = np.random.default_rng(seed).standard_normal(2 << L)
     psi /= np.linalg.norm(psi)
-    return fields, np.array(lo, np.uint32), np.array(hi, np.uint32), -0.5 * jz, -0.5 * jz, jz, psi
+    return fields, d.lo, d.hi, d.jx, d.jy, d.jz, psi
 
 
 def step_bytes(order: int, n: int) -> int:
@@ -106,6 +98,116 @@ def cpu_baseline(L, fields, lo, hi, jx, jy, jz, order_bytes, target_s=15.0):
             "ms_per_order_per_thread": round(wall / orders * 1e3, 3)}
 
 
+# ------------------------------------------------ control-variate leg
+
+CV_SEED = 20240726
+
+
+def cpmg_program(spx, L, cycles=32, steps=5):
+    """pi/2_x on every spin, then `cycles` x [tau, pi_y, tau, measure],
+    tau = steps x 0.05."""
+    P = spx.Program()
+    allspins = list(range(L))
+    P.pulse((1, 0, 0), math.pi / 2, allspins)
+    P.repeat(cycles)
+    P.evolve(0.05, steps)
+    P.pulse((0, 1, 0), math.pi, allspins)
+    P.evolve(0.05, steps)
+    P.measure()
+    P.end()
+    return P
+
+
+def exact_curves(spx, ctx, L, first, count):
+    """Echo curves (mean <sigma_y> over spins, from the all-down state) of
+    realisations first .. first+count-1, by the resident batch runner; the
+    run's wall time (the host-side adds are not timed)."""
+    obs = [(spx.OBS_MAG_Y, s) for s in range(L)]
+    b = spx.Batch(ctx, cpmg_program(spx, L), obs, epsilon=1e-7, reference_axis=0)
+    psi0 = np.zeros(2 << L)
+    psi0[0] = 1.0
+    f = np.zeros((L, 3))
+    for i in range(first, first + count):
+        d = spx.dipolar_realisation(CV_SEED, i, L)
+        f[:, 2] = d.hz
+        b.add(L, f, d.lo, d.hi, d.jx, d.jy, d.jz, psi0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, v = b.run()
+    return v.mean(axis=2), time.perf_counter() - t0
+
+
+def control_variate_leg(spx, ctx, L=10, target=1e-3):
+    """Device seconds to a +-target error bar on every point of the mean echo
+    curve: plain batched Monte-Carlo against the control variate (exact
+    correction + surrogate draws), both actually run at the sample sizes the
+    held-out variances prescribe. Training cost is reported beside it."""
+    from spinsim_amd import surrogate as S
+    n_tr, n_va = 12288, 2048
+    log(f"[bench] control variate: training set of {n_tr} + {n_va} exact L={L} curves")
+    Ytr, t_ytr = exact_curves(spx, ctx, L, 0, n_tr)
+    Yva, t_yva = exact_curves(spx, ctx, L, n_tr, n_va)
+    Jtr, htr = spx.dipolar_graphs(CV_SEED, 0, n_tr, L)
+    Jva, hva = spx.dipolar_graphs(CV_SEED, n_tr, n_va, L)
+    model = S.Model(ctx, Jtr, htr, Ytr, seed=11)
+    t0 = time.perf_counter()
+    model.train(Jtr, htr, Ytr, epochs=60, batch=128, lr=2e-3, seed=1, decay=0.97)
+    t_train = time.perf_counter() - t0
+    gva = model.predict(Jva, hva)
+    beta = S.fit_beta(Yva, gva)
+    var_y = Yva.var(axis=0, ddof=1)
+    var_r = (Yva - beta * gva).var(axis=0, ddof=1)
+    var_g = beta * beta * gva.var(axis=0, ddof=1)
+    se2 = target * target
+    n_p = int(math.ceil(var_y.max() / se2))
+    n_c = int(math.ceil(var_r.max() / (0.9 * se2)))
+    m_c = int(math.ceil(var_g.max() / (0.1 * se2)))
+    log(f"[bench] plain: {n_p} exact runs")
+    Yp, t_plain = exact_curves(spx, ctx, L, 1_000_000, n_p)
+    plain = S.plain_estimate(Yp)
+    log(f"[bench] control variate: {n_c} exact runs + {m_c} surrogate draws")
+    Yc, t_exact_c = exact_curves(spx, ctx, L, 2_000_000, n_c)
+    Jc, hc = spx.dipolar_graphs(CV_SEED, 2_000_000, n_c, L)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    gc = model.predict(Jc, hc)
+    gx = model.predict_dipolar(CV_SEED, 3_000_000, m_c, L)
+    t_sur = time.perf_counter() - t0
+    cv = S.cv_estimate(Yc, gc, gx, beta)
+    t_cv = t_exact_c + t_sur
+    z = float(np.max(np.abs(cv.mean - plain.mean) / np.sqrt(cv.stderr ** 2 + plain.stderr ** 2)))
+    return {
+        "workload": f"L={L} dipolar XXZ (min spacing 0.5, Hz sigma 0.5), CPMG pi/2_x + 32 x [tau=0.25, pi_y, "
+                    f"tau], mean <sigma_y> per echo from the all-down state; +-{target:g} on every point",
+        "plain": {"s": round(t_plain, 3), "exact_runs": n_p, "max_stderr": float(plain.stderr.max())},
+        "control_variate": {"s": round(t_cv, 3), "exact_runs": n_c, "surrogate_draws": m_c,
+                            "max_stderr": float(cv.stderr.max()), "surrogate_s": round(t_sur, 3)},
+        "speedup": round(t_plain / t_cv, 2),
+        "worst_point_variance_reduction": round(float(var_y.max() / var_r.max()), 2),
+        "max_abs_z_cv_vs_plain": round(z, 2),
+        "training": {"exact_runs": n_tr + n_va, "exact_s": round(t_ytr + t_yva, 3), "train_s": round(t_train, 3),
+                     "note": "one-off; reused by every later estimate of this ensemble"},
+    }
+
+
+def copy_probe(spx, L, reps=50):
+    """In-place scale (read + write 16 B per amplitude): the plain streaming
+    kernel that calibrates the HBM counters and the achievable bandwidth."""
+    ctx = spx.Context(0)
+    st = spx.StateVector(ctx, L, np.ones(2 << L) / math.sqrt(float(2 << L)))
+    for _ in range(5):
+        st.scale(1.0)
+    ctx.record(0)
+    for _ in range(reps):
+        st.scale(1.0)
+    ctx.record(1)
+    ctx.synchronize()
+    us = ctx.elapsed_ms(0, 1) * 1e3 / reps
+    nbytes = 32 * (1 << L)
+    print(json.dumps({"probe": "k_scale", "nspins": L, "bytes_per_launch": nbytes, "us_per_launch": round(us, 3),
+                      "GB/s": round(nbytes / (us * 1e-6) / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +218,9 @@ def main():
     ap.add_argument("--dt", type=float, default=0.05)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="L",
+                    help="only time the in-place scale kernel on 2^L amplitudes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -124,6 +229,9 @@ def main():
         sys.exit(2)
 
     from spinsim_amd import spx
+    if args.copy_probe:
+        copy_probe(spx, args.copy_probe)
+        return
     L = args.nspins
     n = 1 << L
     fields, lo, hi, jx, jy, jz, psi0 = synthetic_system(L)
@@ -166,6 +274,7 @@ def main():
     t_long, t_short = ctx.elapsed_ms(2, 3), ctx.elapsed_ms(3, 4)
     per_order_ms = (t_long - t_short) / (reps * (order - (short.size - 1)))
     kernel_bytes = BYTES_PER_AMP_ORDER * n
+    kernel_name = ham.kernel_name()
     achieved = kernel_bytes / (per_order_ms * 1e-3) / 1e9
     log(f"[bench] general order: {per_order_ms * 1e3:.2f} us -> {achieved:.1f} GB/s")
 
@@ -192,10 +301,13 @@ def main():
                    "spectral_bound": round(bound, 4)},
         "hpsi_per_s": round(args.steps * order / (ms * 1e-3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_dense<K_ORDERN, 18, 10> (general Chebyshev order)", "bytes_per_launch": kernel_bytes,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get(kernel_name),
+                     "kernel": kernel_name + " (general Chebyshev order)", "bytes_per_launch": kernel_bytes,
                      "us_per_launch": round(per_order_ms * 1e3, 3)},
     }
+    if not args.no_cv:
+        line["control_variate"] = control_variate_leg(spx, ctx)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(L, fields, lo, hi, jx, jy, jz, kernel_bytes,

@@ -24,6 +24,10 @@ _vp = C.c_void_p
 _fp = C.POINTER(C.c_float)
 
 REAL, IMAGINARY = 0, 1
+# observable codes of the batched runner (include/spx.h)
+OBS_CORR_X, OBS_CORR_Y, OBS_CORR_Z = 0, 1, 2
+OBS_MAG_X, OBS_MAG_Y, OBS_MAG_Z = 3, 4, 5
+OBS_NORM, OBS_ENERGY = 6, 7
 AXES = {"x": 0, "y": 1, "z": 2}
 
 OK, EINVAL, ERANGE, EDOMAIN, ECONVERGE, EOVERFLOW, ERUNTIME, EDEVICE, ENOMEM = range(9)
