@@ -105,6 +105,9 @@ int spx_ham_spectral_bound(const spx_ham* h, double* bound);                   /
 int spx_ham_apply(spx_ham* h, int kind, double bound, const spx_state* in, spx_state* out);
 /* The cached sigma_z diagonal (rebuildDiagonal), downloaded to host [2^L]. */
 int spx_ham_diagonal(spx_ham* h, double* host);
+/* Name of the order kernel the most recently prepared scale dispatches to
+ * (no reference counterpart: a schedule report for tuning and benchmarks). */
+int spx_ham_schedule(const spx_ham* h, char* buf, size_t len);
 
 /* ------------------------------------------------ Propagator (a8)
  * include/spinsim/core/Propagator.hpp:21-55, src/core/Propagator.cpp:38-100 */

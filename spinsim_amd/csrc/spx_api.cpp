@@ -1,6 +1,7 @@
 // The extern "C" surface of libspx (include/spx.h): contexts, device state
 // vectors, Hamiltonians, the Chebyshev propagator driver, gates,
 // observables, pulse programs and the batched ensemble runner.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -351,6 +352,22 @@ int spx_ham_diagonal(spx_ham* h, double* host) {
   const size_t n = static_cast<size_t>(1) << h->nspins;
   SPX_HIP(hipMemcpyAsync(host, h->d_diag, n * sizeof(double), hipMemcpyDeviceToHost, h->ctx->stream));
   SPX_HIP(hipStreamSynchronize(h->ctx->stream));
+  return SPX_OK;
+}
+
+int spx_ham_schedule(const spx_ham* h, char* buf, size_t len) {
+  if (!h || !buf || len == 0) return fail(SPX_EINVAL, "null argument");
+  std::string name;
+  if (!h->terms_valid) name = "unprepared";
+  else if (h->dense_ok)
+    name = "k_dense<K_ORDERN, " + std::to_string(h->nspins) + ", " + std::to_string(dense_tile_bits()) + ">";
+  else if (h->terms_variant == 3)
+    name = "k_order3<K_ORDERN, G=" + std::to_string(h->terms_g) + ">, tile bits " + std::to_string(h->terms_tb);
+  else
+    name = "k_horder<K_ORDERN>, tile bits " + std::to_string(h->terms_tb);
+  const size_t n = std::min(len - 1, name.size());
+  std::memcpy(buf, name.data(), n);
+  buf[n] = '\0';
   return SPX_OK;
 }
 

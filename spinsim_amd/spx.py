@@ -105,6 +105,7 @@ SIGNATURES = {
     "spx_ham_spectral_bound": (C.c_int, [_vp, _dp]),
     "spx_ham_apply": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _vp]),
     "spx_ham_diagonal": (C.c_int, [_vp, _dp]),
+    "spx_ham_schedule": (C.c_int, [_vp, C.c_char_p, C.c_size_t]),
     "spx_prop_create": (C.c_int, [_vp, C.c_size_t, C.c_double, C.POINTER(_vp)]),
     "spx_prop_destroy": (C.c_int, [_vp]),
     "spx_prop_evolve": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_double, C.c_size_t, C.c_int]),
@@ -363,6 +364,12 @@ class Hamiltonian:
         v = C.c_double()
         check(lib().spx_ham_spectral_bound(self.h, C.byref(v)))
         return v.value
+
+    def kernel_name(self) -> str:
+        """The order kernel the most recently prepared scale dispatches to."""
+        buf = C.create_string_buffer(128)
+        check(lib().spx_ham_schedule(self.h, buf, 128))
+        return buf.value.decode()
 
     def apply_physical(self, inp: StateVector, out: StateVector):
         check(lib().spx_ham_apply(self.h, 0, 0.0, inp.h, out.h))
