@@ -219,6 +219,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="L",
                     help="only time the in-place scale kernel on 2^L amplitudes")
     args = ap.parse_args()
@@ -242,6 +243,10 @@ def main():
     ham.set_couplings(lo, hi, jx, jy, jz)
     bound = ham.spectral_bound()
     coeffs, eff = spx.cheb_build(spx.REAL, bound, args.dt, args.epsilon)
+    tune_report = None
+    if not args.no_tune:
+        tune_report = ham.autotune(eff)
+        log(f"[bench] autotune: {tune_report[:300]}")
     order = coeffs.size - 1
     psi = spx.StateVector(ctx, L, psi0)
     prop = spx.Propagator(ctx, L, args.epsilon)
@@ -298,7 +303,8 @@ def main():
         "config": {"workload": f"L={L} dipolar XXZ real-time Chebyshev steps, dt={args.dt}, "
                                f"eps={args.epsilon:g}, autonormalize off (single large state)",
                    "nspins": L, "pairs": int(lo.size), "order": int(order),
-                   "spectral_bound": round(bound, 4)},
+                   "spectral_bound": round(bound, 4),
+                   "schedule": tune_report.split(";")[0] if tune_report else "default"},
         "hpsi_per_s": round(args.steps * order / (ms * 1e-3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

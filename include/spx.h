@@ -109,6 +109,21 @@ int spx_ham_diagonal(spx_ham* h, double* host);
  * (no reference counterpart: a schedule report for tuning and benchmarks). */
 int spx_ham_schedule(const spx_ham* h, char* buf, size_t len);
 
+/* ------------------------------------------------- Autotuner (a16)
+ * No reference counterpart. Times every eligible launch schedule of the
+ * order kernels (all-pairs kernel at tile bits 10/9, table kernels at tile
+ * bits 8..10, with and without the XCD-aware tile order, the general kernel
+ * at two block sizes) on this Hamiltonian at the step bound `bound`, keeps
+ * the fastest and caches it per problem shape (L, all-pairs eligibility,
+ * flip-term count) for the process; `report` receives the timings. */
+int spx_ham_autotune(spx_ham* h, double bound, char* report, size_t len);
+/* 0: default schedules (the default); 1: every evolve on a Hamiltonian
+ * without a schedule adopts its shape's cached one, tuning it first if
+ * absent. Schedules reorder floating-point additions only. */
+int spx_autotune_mode(int mode);
+/* Fix a schedule by hand; 0 / -1 fields keep the defaults. */
+int spx_ham_set_schedule(spx_ham* h, int tile_bits, int block, int xcd, int dense, int dense_tile_bits);
+
 /* ------------------------------------------------ Propagator (a8)
  * include/spinsim/core/Propagator.hpp:21-55, src/core/Propagator.cpp:38-100 */
 int spx_prop_create(spx_ctx* ctx, size_t nspins, double epsilon, spx_prop** out);

@@ -360,7 +360,7 @@ int spx_ham_schedule(const spx_ham* h, char* buf, size_t len) {
   std::string name;
   if (!h->terms_valid) name = "unprepared";
   else if (h->dense_ok)
-    name = "k_dense<K_ORDERN, " + std::to_string(h->nspins) + ", " + std::to_string(dense_tile_bits()) + ">";
+    name = "k_dense<K_ORDERN, " + std::to_string(h->nspins) + ", " + std::to_string(dense_tile_bits(h)) + ">";
   else if (h->terms_variant == 3)
     name = "k_order3<K_ORDERN, G=" + std::to_string(h->terms_g) + ">, tile bits " + std::to_string(h->terms_tb);
   else
@@ -506,6 +506,7 @@ int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, siz
   if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
   SPX_TRY(set_device(p->ctx));
   SPX_TRY(ensure_scratch(p, nstates < kMaxMany ? nstates : kMaxMany));
+  SPX_TRY(autotune_before_evolve(h, bound));
   SPX_TRY(ham_prepare(h, -2.0 / bound));
   for (size_t step = 0; step < nsteps; ++step) {
     for (size_t first = 0; first < nstates; first += kMaxMany) {

@@ -55,6 +55,17 @@ struct FlipTerm {
   double pad;
 };
 
+// Launch schedule of the order kernels for one Hamiltonian; zero or -1
+// fields keep the library default (environment overridable).
+struct Schedule {
+  int tb = 0;        // tile bits of the table kernels
+  int block = 0;     // threads per workgroup of the per-amplitude-group kernel
+  int xcd = -1;      // XCD-aware tile order
+  int dense = -1;    // all-pairs kernel allowed
+  int dense_tb = 0;  // its tile bits (9 or 10)
+  int table = 0;     // 0: best eligible, 2: force the per-amplitude-group kernel
+};
+
 // ----------------------------------------------------------------- objects
 }  // namespace spx
 
@@ -103,6 +114,8 @@ struct spx_ham {
   double* d_tab = nullptr;   // diagonal build table
   double2* d_scratch = nullptr;  // H psi for the energy observable
   size_t tab_cap = 0;
+  spx::Schedule sched;           // launch schedule (autotuned or default)
+  bool sched_set = false;        // chosen by the autotuner (or its cache)
 };
 
 struct spx_prop {
@@ -154,10 +167,13 @@ int reduce_corr(spx_ctx* c, int axis, const double2* state, const double2* ref, 
 int reduce_mag(spx_ctx* c, int axis, uint32_t spin, const double2* state, size_t L, double* out);
 int reduce_energy(spx_ham* h, const double2* state, double* out);
 
-// tunables (environment overridable for sweeps)
-int tile_bits_for(size_t L);
-int block_threads();
-int xcd_remap();
-int dense_tile_bits();
+// tunables: the Hamiltonian's schedule, else the environment (for sweeps),
+// else the defaults
+int tile_bits_for(const spx_ham* h);
+int block_threads(const spx_ham* h);
+int xcd_remap(const spx_ham* h);
+int dense_tile_bits(const spx_ham* h);
+// the autotuner's process-wide mode and its hook in the propagator
+int autotune_before_evolve(spx_ham* h, double bound);
 
 }  // namespace spx

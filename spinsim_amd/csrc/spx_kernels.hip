@@ -965,42 +965,44 @@ static int env_int(const char* name, int fallback) {
 }
 
 // Tile bits (2^tb amplitudes staged in LDS per workgroup), threads per
-// workgroup and the XCD-aware tile order; SPX_TILE_BITS / SPX_BLOCK /
-// SPX_XCD_REMAP override them for sweeps.
-int tile_bits_for(size_t L) {
+// workgroup and the XCD-aware tile order: the Hamiltonian's schedule when
+// set, else SPX_TILE_BITS / SPX_BLOCK / SPX_XCD_REMAP (sweeps), else 10 /
+// 256 / off.
+int tile_bits_for(const spx_ham* h) {
   static const int forced = env_int("SPX_TILE_BITS", 0);
-  int tb = forced > 0 ? forced : 10;
+  int tb = h->sched.tb > 0 ? h->sched.tb : forced > 0 ? forced : 10;
   if (tb > 12) tb = 12;
   if (tb < 6) tb = 6;
-  if (static_cast<size_t>(tb) > L) tb = static_cast<int>(L);
+  if (static_cast<size_t>(tb) > h->nspins) tb = static_cast<int>(h->nspins);
   return tb;
 }
 
-int block_threads() {
-  static const int b = env_int("SPX_BLOCK", 256);
+int block_threads(const spx_ham* h) {
+  static const int env = env_int("SPX_BLOCK", 256);
+  const int b = h->sched.block > 0 ? h->sched.block : env;
   return b >= 64 && b <= 1024 ? b : 256;
 }
 
-int xcd_remap() {
-  static const int r = env_int("SPX_XCD_REMAP", 0);
-  return r;
+int xcd_remap(const spx_ham* h) {
+  static const int env = env_int("SPX_XCD_REMAP", 0);
+  return h->sched.xcd >= 0 ? h->sched.xcd : env;
 }
 
-// Diagonal (when hz or the couplings changed) and the flip table at `scale`,
-// with the reference's skip rule applied to the scaled coefficients.
 // Dense-pair eligibility and its weight table (pair order (0,1),(0,2),...):
-// L = 14..18 with the default tile, no transverse field and no flip-both
-// weight at this scale. SPX_DENSE=0 switches the specialised kernel off.
-int dense_tile_bits() {
-  static const int tb = env_int("SPX_DENSE_TB", 10);
+// L = 14..18, no transverse field and no flip-both weight at this scale.
+// SPX_DENSE=0 (or a schedule with dense = 0) switches the kernel off.
+int dense_tile_bits(const spx_ham* h) {
+  static const int env = env_int("SPX_DENSE_TB", 10);
+  const int tb = h->sched.dense_tb > 0 ? h->sched.dense_tb : env;
   return tb == 9 ? 9 : 10;
 }
 
 static int prepare_dense(spx_ham* h, double scale) {
-  static const int enabled = env_int("SPX_DENSE", 1);
+  static const int env = env_int("SPX_DENSE", 1);
+  const int enabled = h->sched.dense >= 0 ? h->sched.dense : env;
   const size_t L = h->nspins;
   h->dense_ok = false;
-  if (!enabled || L < 14 || L > 18 || tile_bits_for(L) != 10) return SPX_OK;
+  if (!enabled || L < 14 || L > 18) return SPX_OK;
   for (size_t s = 0; s < L; ++s)
     if (std::abs(scale * 0.5 * h->hx[s]) > kNegligible || std::abs(scale * 0.5 * h->hy[s]) > kNegligible)
       return SPX_OK;
@@ -1067,11 +1069,11 @@ int ham_prepare(spx_ham* h, double scale) {
   if (h->terms_valid && h->terms_scale == scale) return SPX_OK;
   SPX_TRY(prepare_dense(h, scale));
 
-  const int tb = tile_bits_for(L);
+  const int tb = tile_bits_for(h);
   const int g = static_cast<int>(L) - tb;
   // the padded table is at most L fields + L(L-1)/2 pairs plus kChunk per segment
   const size_t table_max = sizeof(FlipTerm) * (L + L * (L - 1) / 2 + kChunk * (4 + 2 * (g > 0 ? g : 0)));
-  if (tb <= 10 && g <= kMaxG3 &&
+  if (h->sched.table != 2 && tb <= 10 && g <= kMaxG3 &&
       static_cast<size_t>(1 + (g > 0 ? g : 0)) * (sizeof(double2) << tb) + table_max <= 160 * 1024) {
     // v3 layout (see K3Args::off)
     std::vector<FlipTerm> seg[3];
@@ -1266,20 +1268,21 @@ static void launch_dense_lt(spx_ctx* c, const KDArgs& a, unsigned nvec) {
 // Tile of 2^10 amplitudes (one workgroup of 1024 threads per CU) or 2^9
 // (two of 512); SPX_DENSE_TB selects.
 template <int KIND, int L>
-static void launch_dense_l(spx_ctx* c, const KDArgs& a, unsigned nvec) {
-  if (dense_tile_bits() == 9) launch_dense_lt<KIND, L, 9>(c, a, nvec);
+static void launch_dense_l(spx_ctx* c, const KDArgs& a, unsigned nvec, int tb) {
+  if (tb == 9) launch_dense_lt<KIND, L, 9>(c, a, nvec);
   else launch_dense_lt<KIND, L, 10>(c, a, nvec);
 }
 
 template <int KIND>
 static int launch_dense_kind(spx_ham* h, const KDArgs& a, unsigned nvec) {
   spx_ctx* c = h->ctx;
+  const int tb = dense_tile_bits(h);
   switch (h->nspins) {
-    case 14: launch_dense_l<KIND, 14>(c, a, nvec); break;
-    case 15: launch_dense_l<KIND, 15>(c, a, nvec); break;
-    case 16: launch_dense_l<KIND, 16>(c, a, nvec); break;
-    case 17: launch_dense_l<KIND, 17>(c, a, nvec); break;
-    default: launch_dense_l<KIND, 18>(c, a, nvec); break;
+    case 14: launch_dense_l<KIND, 14>(c, a, nvec, tb); break;
+    case 15: launch_dense_l<KIND, 15>(c, a, nvec, tb); break;
+    case 16: launch_dense_l<KIND, 16>(c, a, nvec, tb); break;
+    case 17: launch_dense_l<KIND, 17>(c, a, nvec, tb); break;
+    default: launch_dense_l<KIND, 18>(c, a, nvec, tb); break;
   }
   SPX_HIP(hipGetLastError());
   return SPX_OK;
@@ -1316,7 +1319,7 @@ static int launch_k(spx_ham* h, int kind, const KArgs& a, unsigned nvec) {
   spx_ctx* c = h->ctx;
   const size_t n = static_cast<size_t>(1) << h->nspins;
   const uint32_t T = 1u << a.tb;
-  unsigned block = static_cast<unsigned>(block_threads());
+  unsigned block = static_cast<unsigned>(block_threads(h));
   if (block > T) block = T;
   int amps = static_cast<int>(T / block);
   if (amps > 16) {
@@ -1344,7 +1347,7 @@ static KArgs base_args(spx_ham* h, double sdiag) {
   if (h->terms_variant == 2)
     for (int r = 0; r <= h->terms_g; ++r) a.grp[r] = h->terms_grp[r];
   a.tb = h->terms_tb;
-  a.xcd_remap = xcd_remap();
+  a.xcd_remap = xcd_remap(h);
   a.sdiag = sdiag;
   return a;
 }

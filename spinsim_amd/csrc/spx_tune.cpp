@@ -1,0 +1,229 @@
+// Autotuner of the order-kernel schedule (no reference counterpart).
+//
+// For one Hamiltonian at one step bound it times every eligible schedule —
+// the all-pairs kernel at tile bits 10 and 9, the table kernels at tile bits
+// 8..10, each with and without the XCD-aware tile order, and the
+// per-amplitude-group kernel at two block sizes — on the general Chebyshev
+// order (T_n = 2Ĥ T_{n-1} - T_{n-2}, psi += c T_n) and keeps the fastest.
+// Winners are cached per problem shape (L, whether the all-pairs kernel is
+// eligible, number of flip terms), so the realisations of an ensemble pay
+// for one tuning. Schedules change the order of floating-point additions, not
+// the terms, so results agree to rounding across schedules.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "spx_common.hpp"
+
+namespace spx {
+namespace {
+
+using Key = std::tuple<size_t, bool, size_t>;
+
+std::mutex g_mu;
+std::map<Key, Schedule> g_cache;
+int g_mode = 0;
+
+// The shape a schedule depends on, at the scale of one step.
+Key shape_of(const spx_ham* h, double scale) {
+  bool fields = false, flipboth = false;
+  size_t terms = 0;
+  for (size_t s = 0; s < h->nspins; ++s) {
+    const bool on = std::abs(scale * 0.5 * h->hx[s]) > kNegligible || std::abs(scale * 0.5 * h->hy[s]) > kNegligible;
+    fields |= on;
+    terms += on ? 1 : 0;
+  }
+  for (size_t p = 0; p < h->lo.size(); ++p) {
+    const double ff = scale * 0.25 * (h->jx[p] + h->jy[p]);
+    const double fb = scale * 0.25 * (h->jx[p] - h->jy[p]);
+    flipboth |= std::abs(fb) > kNegligible;
+    terms += (std::abs(ff) > kNegligible || std::abs(fb) > kNegligible) ? 1 : 0;
+  }
+  const bool dense = h->nspins >= 14 && h->nspins <= 18 && !fields && !flipboth;
+  return Key{h->nspins, dense, terms};
+}
+
+std::string describe(const Schedule& s, const spx_ham* h) {
+  if (h->dense_ok) return "all-pairs tb=" + std::to_string(dense_tile_bits(h)) + " xcd=" + std::to_string(xcd_remap(h));
+  const std::string kind = h->terms_variant == 3 ? "table" : "group";
+  std::string d = kind + " tb=" + std::to_string(h->terms_tb) + " xcd=" + std::to_string(xcd_remap(h));
+  if (h->terms_variant != 3) d += " block=" + std::to_string(block_threads(h));
+  (void)s;
+  return d;
+}
+
+std::vector<Schedule> candidates(const spx_ham* h, bool dense) {
+  std::vector<Schedule> c;
+  const int L = static_cast<int>(h->nspins);
+  if (dense)
+    for (int tb : {10, 9})
+      for (int xcd : {0, 1}) {
+        Schedule s;
+        s.dense = 1;
+        s.dense_tb = tb;
+        s.xcd = xcd;
+        c.push_back(s);
+      }
+  for (int tb : {10, 9, 8}) {
+    if (tb > L) continue;
+    for (int xcd : {0, 1}) {
+      if (xcd == 1 && L - tb < 3) continue;
+      Schedule s;
+      s.dense = 0;
+      s.tb = tb;
+      s.xcd = xcd;
+      c.push_back(s);
+    }
+  }
+  for (int tb : {10, 9}) {
+    if (tb > L) continue;
+    for (int block : {256, 1024}) {
+      Schedule s;
+      s.dense = 0;
+      s.tb = tb;
+      s.table = 2;
+      s.block = block;
+      c.push_back(s);
+    }
+  }
+  return c;
+}
+
+struct Scratch {
+  double2* v[3] = {};
+  ~Scratch() {
+    for (double2* p : v)
+      if (p) (void)hipFree(p);
+  }
+};
+
+int tune(spx_ham* h, double bound, std::string* report) {
+  if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
+  const double scale = -2.0 / bound;
+  const Key key = shape_of(h, scale);
+  spx_ctx* c = h->ctx;
+  const size_t n = static_cast<size_t>(1) << h->nspins;
+  Scratch sc;
+  for (double2*& p : sc.v) {
+    SPX_HIP(hipMalloc(&p, n * sizeof(double2)));
+    SPX_HIP(hipMemsetAsync(p, 0, n * sizeof(double2), c->stream));
+  }
+  hipEvent_t e0, e1;
+  SPX_HIP(hipEventCreate(&e0));
+  SPX_HIP(hipEventCreate(&e1));
+  const int reps = n >= (size_t(1) << 16) ? 12 : 40;
+  double best = 1e300;
+  Schedule best_s;
+  std::string log;
+  int rc = SPX_OK;
+  for (const Schedule& s : candidates(h, std::get<1>(key))) {
+    h->sched = s;
+    h->terms_valid = false;
+    if ((rc = ham_prepare(h, scale)) != SPX_OK) break;
+    if (s.dense == 1 && !h->dense_ok) continue;
+    OrderLaunch v{};
+    v.in[0] = sc.v[0];
+    v.out[0] = sc.v[1];
+    v.psi[0] = sc.v[2];
+    v.nvec = 1;
+    for (int i = 0; i < 2 && rc == SPX_OK; ++i) rc = launch_order(h, 3, v, 0.0, 0.0, 0.0, 1e-3, 0.0);
+    if (rc == SPX_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(SPX_EDEVICE, "hipEventRecord failed");
+    for (int i = 0; i < reps && rc == SPX_OK; ++i) rc = launch_order(h, 3, v, 0.0, 0.0, 0.0, 1e-3, 0.0);
+    if (rc == SPX_OK && hipEventRecord(e1, c->stream) != hipSuccess) rc = fail(SPX_EDEVICE, "hipEventRecord failed");
+    float ms = 0.f;
+    if (rc == SPX_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = fail(SPX_EDEVICE, "timing the candidate schedule failed");
+    if (rc != SPX_OK) break;
+    const double us = 1e3 * ms / reps;
+    log += describe(s, h) + ": " + std::to_string(us).substr(0, 7) + " us; ";
+    if (us < best) {
+      best = us;
+      best_s = s;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != SPX_OK) return rc;
+  h->sched = best_s;
+  h->sched_set = true;
+  h->terms_valid = false;
+  SPX_TRY(ham_prepare(h, scale));
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache[key] = best_s;
+  }
+  if (report) *report = "chosen " + describe(best_s, h) + " (" + std::to_string(best).substr(0, 7) + " us/order); " + log;
+  return SPX_OK;
+}
+
+}  // namespace
+
+int autotune_before_evolve(spx_ham* h, double bound) {
+  int mode;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    mode = g_mode;
+  }
+  if (mode == 0 || h->sched_set) return SPX_OK;
+  const Key key = shape_of(h, -2.0 / bound);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) {
+      h->sched = it->second;
+      h->sched_set = true;
+      h->terms_valid = false;
+      return SPX_OK;
+    }
+  }
+  return tune(h, bound, nullptr);
+}
+
+}  // namespace spx
+
+using namespace spx;
+
+extern "C" {
+
+int spx_autotune_mode(int mode) {
+  if (mode != 0 && mode != 1) return fail(SPX_EINVAL, "autotune mode must be 0 or 1");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_mode = mode;
+  return SPX_OK;
+}
+
+int spx_ham_autotune(spx_ham* h, double bound, char* report, size_t len) {
+  if (!h) return fail(SPX_EINVAL, "null Hamiltonian");
+  std::string r;
+  SPX_TRY(tune(h, bound, &r));
+  if (report && len > 0) {
+    const size_t k = std::min(len - 1, r.size());
+    std::copy(r.begin(), r.begin() + static_cast<long>(k), report);
+    report[k] = '\0';
+  }
+  return SPX_OK;
+}
+
+int spx_ham_set_schedule(spx_ham* h, int tile_bits, int block, int xcd, int dense, int dense_tile_bits) {
+  if (!h) return fail(SPX_EINVAL, "null Hamiltonian");
+  if (tile_bits != 0 && (tile_bits < 6 || tile_bits > 12)) return fail(SPX_EINVAL, "tile bits must be 0 or 6..12");
+  if (block != 0 && block != 256 && block != 512 && block != 1024) return fail(SPX_EINVAL, "block must be 0, 256, 512 or 1024");
+  if (dense_tile_bits != 0 && dense_tile_bits != 9 && dense_tile_bits != 10)
+    return fail(SPX_EINVAL, "all-pairs tile bits must be 0, 9 or 10");
+  Schedule s;
+  s.tb = tile_bits;
+  s.block = block;
+  s.xcd = xcd < 0 ? -1 : (xcd ? 1 : 0);
+  s.dense = dense < 0 ? -1 : (dense ? 1 : 0);
+  s.dense_tb = dense_tile_bits;
+  h->sched = s;
+  h->sched_set = true;
+  h->terms_valid = false;
+  return SPX_OK;
+}
+
+}  // extern "C"

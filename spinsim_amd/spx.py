@@ -106,6 +106,9 @@ SIGNATURES = {
     "spx_ham_apply": (C.c_int, [_vp, C.c_int, C.c_double, _vp, _vp]),
     "spx_ham_diagonal": (C.c_int, [_vp, _dp]),
     "spx_ham_schedule": (C.c_int, [_vp, C.c_char_p, C.c_size_t]),
+    "spx_ham_autotune": (C.c_int, [_vp, C.c_double, C.c_char_p, C.c_size_t]),
+    "spx_autotune_mode": (C.c_int, [C.c_int]),
+    "spx_ham_set_schedule": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "spx_prop_create": (C.c_int, [_vp, C.c_size_t, C.c_double, C.POINTER(_vp)]),
     "spx_prop_destroy": (C.c_int, [_vp]),
     "spx_prop_evolve": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_double, C.c_size_t, C.c_int]),
@@ -365,6 +368,16 @@ class Hamiltonian:
         check(lib().spx_ham_spectral_bound(self.h, C.byref(v)))
         return v.value
 
+    def autotune(self, bound: float) -> str:
+        """Time the eligible order-kernel schedules at this step bound and
+        keep the fastest; returns the timing report."""
+        buf = C.create_string_buffer(4096)
+        check(lib().spx_ham_autotune(self.h, bound, buf, 4096))
+        return buf.value.decode()
+
+    def set_schedule(self, tile_bits=0, block=0, xcd=-1, dense=-1, dense_tile_bits=0):
+        check(lib().spx_ham_set_schedule(self.h, tile_bits, block, xcd, dense, dense_tile_bits))
+
     def kernel_name(self) -> str:
         """The order kernel the most recently prepared scale dispatches to."""
         buf = C.create_string_buffer(128)
@@ -427,6 +440,11 @@ class Propagator:
         v = C.c_double()
         check(lib().spx_prop_epsilon(self.h, C.byref(v)))
         return v.value
+
+
+def autotune_mode(mode: int) -> None:
+    """0: default schedules; 1: tune each new problem shape on first evolve."""
+    check(lib().spx_autotune_mode(mode))
 
 
 def rotate(psi: StateVector, axis, angle: float, targets: Iterable[int]):
