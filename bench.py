@@ -190,6 +190,52 @@ def control_variate_leg(spx, ctx, L=10, target=1e-3):
     }
 
 
+def xy8_program(spx, blocks=16, omega=50.0, free_steps=10):
+    """XY-8 (X Y X Y Y X Y X) with finite pulses: each pulse an evolve of
+    tau_p = pi/omega under hx or hy = omega on every spin, preceded by free
+    evolution free_steps x 0.05; a measure after each block."""
+    P = spx.Program()
+    tp = math.pi / omega
+    P.measure()
+    P.repeat(blocks)
+    for ax in "XYXYYXYX":
+        P.evolve(0.05, free_steps)
+        hx, hy = (omega, 0.0) if ax == "X" else (0.0, omega)
+        P.evolve_uniform(tp, 1, hx=hx, hy=hy, hz=0.0)
+    P.measure()
+    P.end()
+    return P
+
+
+def batched_leg(spx, ctx, count=4096, seed=20240726):
+    """One launch of `count` realisations at uneven L in {8..12} (L from a
+    hash of the index, no draws taken from the realisation stream), XY-8
+    repeated 16 times, corr:x and norm per block: realisations per second."""
+    obs = [(spx.OBS_CORR_X, 0), (spx.OBS_NORM, 0)]
+    b = spx.Batch(ctx, xy8_program(spx), obs, epsilon=1e-7, reference_axis=0)
+    g = np.random.default_rng(seed)
+    Ls = []
+    for i in range(count):
+        L = 8 + ((i * 2654435761) >> 16) % 5
+        d = spx.dipolar_realisation(seed, i, L)
+        f = np.zeros((L, 3))
+        f[:, 2] = d.hz
+        psi = g.standard_normal(2 << L)
+        psi /= np.linalg.norm(psi)
+        b.add(L, f, d.lo, d.hi, d.jx, d.jy, d.jz, psi)
+        Ls.append(L)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, v = b.run()
+    dt = time.perf_counter() - t0
+    if not np.isfinite(v).all():
+        raise SystemExit("non-finite batched ensemble output")
+    return {"workload": f"{count} realisations, L uniform in 8..12, XY-8 (finite pulses, Omega=50, tau=0.5) x 16, "
+                        "corr:x + norm per block, one persistent launch",
+            "realisations": count, "s": round(dt, 4), "realisations_per_s": round(count / dt, 1),
+            "mean_L": round(float(np.mean(Ls)), 2)}
+
+
 def copy_probe(spx, L, reps=50):
     """In-place scale (read + write 16 B per amplitude): the plain streaming
     kernel that calibrates the HBM counters and the achievable bandwidth."""
@@ -314,6 +360,8 @@ def main():
     }
     if not args.no_cv:
         line["control_variate"] = control_variate_leg(spx, ctx)
+        log("[bench] batched ensemble leg ...")
+        line["batched_ensemble"] = batched_leg(spx, ctx)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(L, fields, lo, hi, jx, jy, jz, kernel_bytes,

@@ -59,6 +59,17 @@ int spx_gnn_gradient(spx_gnn* g, size_t ngraphs, size_t nspins, const double* J,
 /* One Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8) on one batch. */
 int spx_gnn_train_step(spx_gnn* g, size_t ngraphs, size_t nspins, const double* J, const double* h,
                        const float* target, float lr, float* loss);
+/* Keep a training set resident on the device: n graphs of `nspins`
+ * (J [n][L][L] symmetric, h [n][L], targets [n][P]); replaces any earlier
+ * set. */
+int spx_gnn_set_training_set(spx_gnn* g, size_t n, size_t nspins, const double* J, const double* h,
+                             const float* target);
+/* One epoch over the resident set: floor(n / batch) Adam steps, step s on
+ * graphs order[s*batch .. (s+1)*batch) (order: a permutation of 0..n-1),
+ * gathered on the device; the same updates as spx_gnn_train_step on those
+ * batches, with one host synchronisation per epoch. mean_loss: the mean
+ * of the steps' losses. */
+int spx_gnn_train_epoch(spx_gnn* g, size_t batch, const uint32_t* order, float lr, float* mean_loss);
 
 #ifdef __cplusplus
 }

@@ -165,13 +165,26 @@ __global__ void k_relu_bwd(const float* __restrict__ dx, const float* __restrict
   if (i < n) dz[i] = z[i] > 0.f ? dx[i] : 0.f;
 }
 
-// column sums of a rows x cols matrix (fixed order per column)
-__global__ void k_colsum(const float* __restrict__ x, float* __restrict__ out, long rows, int cols) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// Column sums of a rows x cols matrix: a block owns 64 columns; its 16 row
+// lanes sum rows r = lane (mod 16) (coalesced 256-byte rows), then a fixed
+// tree over the lanes, so the result does not depend on scheduling.
+__global__ __launch_bounds__(1024) void k_colsum(const float* __restrict__ x, float* __restrict__ out, long rows,
+                                                 int cols) {
+  __shared__ float part[16][64];
+  const int cl = threadIdx.x & 63, lane = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
   float s = 0.f;
-  for (long r = 0; r < rows; ++r) s += x[r * cols + c];
-  out[c] = s;
+  if (c < cols) {
+#pragma unroll 4
+    for (long r = lane; r < rows; r += 16) s += x[r * cols + c];
+  }
+  part[lane][cl] = s;
+  __syncthreads();
+  for (int w = 8; w > 0; w >>= 1) {
+    if (lane < w) part[lane][cl] += part[lane + w][cl];
+    __syncthreads();
+  }
+  if (lane == 0 && c < cols) out[c] = part[0][cl];
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -183,6 +196,33 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   v[i] = 0.999f * v[i] + 0.001f * gi * gi;
   const float mh = m[i] / (1.f - b1t), vh = v[i] / (1.f - b2t);
   p[i] -= lr * mh / (sqrtf(vh) + 1e-8f);
+}
+
+// Minibatch rows of the resident training set: graph i of the batch is
+// graph idx[i] of the set (couplings, fields, targets).
+__global__ void k_gather(const float* __restrict__ sJ, const float* __restrict__ sh, const float* __restrict__ st,
+                         const uint32_t* __restrict__ idx, int L, int P, float* __restrict__ J, float* __restrict__ h,
+                         float* __restrict__ t) {
+  const int i = blockIdx.x;
+  const long src = idx[i];
+  for (int q = threadIdx.x; q < L * L; q += blockDim.x) J[static_cast<long>(i) * L * L + q] = sJ[src * L * L + q];
+  for (int q = threadIdx.x; q < L; q += blockDim.x) h[static_cast<long>(i) * L + q] = sh[src * L + q];
+  for (int q = threadIdx.x; q < P; q += blockDim.x) t[static_cast<long>(i) * P + q] = st[src * P + q];
+}
+
+// acc += sum(sq[0..n)) in a fixed order (one block, strided partial sums,
+// then a tree), so the epoch loss does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_loss_acc(const float* __restrict__ sq, long n, double* __restrict__ acc) {
+  __shared__ double part[256];
+  double s = 0.0;
+  for (long i = threadIdx.x; i < n; i += 256) s += sq[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc += part[0];
 }
 
 inline unsigned nb(long n, int b = 256) { return static_cast<unsigned>((n + b - 1) / b); }
@@ -214,6 +254,11 @@ struct spx_gnn {
   float *pool = nullptr, *y = nullptr, *tgt = nullptr, *dy = nullptr, *sq = nullptr, *dpool = nullptr;
   uint16_t *poolbf = nullptr, *dybf = nullptr, *dzbf = nullptr;
   float *dx = nullptr, *dz = nullptr, *dcat = nullptr, *dm = nullptr;
+  // resident training set
+  float *set_J = nullptr, *set_h = nullptr, *set_t = nullptr;
+  uint32_t* set_idx = nullptr;
+  double* loss_acc = nullptr;
+  size_t set_n = 0, set_L = 0;
   // pinned double buffers of the bulk prediction
   size_t pin_graphs = 0, pin_L = 0;
   float *pin_J[2] = {}, *pin_h[2] = {}, *pin_y[2] = {};
@@ -338,7 +383,7 @@ int backward(spx_gnn* g, long G, long L, float* loss) {
   // readout
   SPX_TRY(gemm(c, g->dybf, 1, P, g->poolbf, H, 1, gr + g->off_w[NLAYER + 1], H, 1, P, H, static_cast<int>(G),
                nullptr, nullptr, 0, 0));
-  k_colsum<<<nb(P, 64), 64, 0, c->stream>>>(g->dy, gr + g->off_b[NLAYER + 1], G, P);
+  k_colsum<<<nb(P, 64), 1024, 0, c->stream>>>(g->dy, gr + g->off_b[NLAYER + 1], G, P);
   SPX_TRY(gemm(c, g->dybf, P, 1, g->pbf + g->off_w[NLAYER + 1], H, 1, g->dpool, H, 1, static_cast<int>(G), H, P,
                nullptr, nullptr, 0, 0));
   k_pool_bwd<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(g->dpool, g->dx,
@@ -346,7 +391,7 @@ int backward(spx_gnn* g, long G, long L, float* loss) {
   for (int l = NLAYER; l >= 0; --l) {
     k_relu_bwd<<<nb(R * H), 256, 0, c->stream>>>(g->dx, g->z[l], g->dz, R * H);
     k_cast<<<nb(R * H), 256, 0, c->stream>>>(g->dz, g->dzbf, R * H);
-    k_colsum<<<nb(H, 64), 64, 0, c->stream>>>(g->dz, gr + g->off_b[l], R, H);
+    k_colsum<<<nb(H, 64), 1024, 0, c->stream>>>(g->dz, gr + g->off_b[l], R, H);
     if (l == 0) {
       SPX_TRY(gemm(c, g->dzbf, 1, H, g->feat, F, 1, gr + g->off_w[0], F, 1, H, F, static_cast<int>(R), nullptr,
                    nullptr, 0, 0));
@@ -471,6 +516,9 @@ int spx_gnn_destroy(spx_gnn* g) {
       if (q) (void)hipHostFree(q);
     if (g->ev[b]) (void)hipEventDestroy(g->ev[b]);
   }
+  for (void* q : {static_cast<void*>(g->set_J), static_cast<void*>(g->set_h), static_cast<void*>(g->set_t),
+                  static_cast<void*>(g->set_idx), static_cast<void*>(g->loss_acc)})
+    if (q) (void)hipFree(q);
   delete g;
   return SPX_OK;
 }
@@ -560,6 +608,69 @@ int spx_gnn_train_step(spx_gnn* g, size_t G, size_t L, const double* J, const do
   k_adam<<<nb(static_cast<long>(g->nparam)), 256, 0, g->ctx->stream>>>(g->p, g->grad, g->m, g->v,
                                                                        static_cast<long>(g->nparam), lr, b1t, b2t);
   SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+int spx_gnn_set_training_set(spx_gnn* g, size_t n, size_t L, const double* J, const double* h,
+                             const float* target) {
+  if (!g || !J || !h || !target) return fail(SPX_EINVAL, "null argument");
+  if (n == 0 || L == 0 || L > kMaxSpins) return fail(SPX_EINVAL, "empty set or too many spins");
+  for (size_t gi = 0; gi < n; ++gi)
+    for (size_t i = 0; i < L; ++i)
+      for (size_t j = 0; j < i; ++j)
+        if (J[(gi * L + i) * L + j] != J[(gi * L + j) * L + i])
+          return fail(SPX_EINVAL, "coupling matrix of graph " + std::to_string(gi) + " is not symmetric");
+  for (void* q : {static_cast<void*>(g->set_J), static_cast<void*>(g->set_h), static_cast<void*>(g->set_t),
+                  static_cast<void*>(g->set_idx)})
+    if (q) SPX_HIP(hipFree(q));
+  g->set_J = g->set_h = g->set_t = nullptr;
+  g->set_idx = nullptr;
+  g->set_n = 0;
+  std::vector<float> Jf(n * L * L), hf(n * L);
+  for (size_t q = 0; q < Jf.size(); ++q) Jf[q] = static_cast<float>(J[q]);
+  for (size_t q = 0; q < hf.size(); ++q) hf[q] = static_cast<float>(h[q]);
+  SPX_HIP(hipMalloc(&g->set_J, Jf.size() * sizeof(float)));
+  SPX_HIP(hipMalloc(&g->set_h, hf.size() * sizeof(float)));
+  SPX_HIP(hipMalloc(&g->set_t, n * g->P * sizeof(float)));
+  SPX_HIP(hipMalloc(&g->set_idx, n * sizeof(uint32_t)));
+  if (!g->loss_acc) SPX_HIP(hipMalloc(&g->loss_acc, sizeof(double)));
+  SPX_HIP(hipMemcpy(g->set_J, Jf.data(), Jf.size() * sizeof(float), hipMemcpyHostToDevice));
+  SPX_HIP(hipMemcpy(g->set_h, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice));
+  SPX_HIP(hipMemcpy(g->set_t, target, n * g->P * sizeof(float), hipMemcpyHostToDevice));
+  g->set_n = n;
+  g->set_L = L;
+  return SPX_OK;
+}
+
+int spx_gnn_train_epoch(spx_gnn* g, size_t batch, const uint32_t* order, float lr, float* mean_loss) {
+  if (!g || !order) return fail(SPX_EINVAL, "null argument");
+  if (g->set_n == 0) return fail(SPX_EINVAL, "no training set");
+  if (batch == 0 || batch > g->set_n) return fail(SPX_EINVAL, "batch must be in [1, set size]");
+  for (size_t i = 0; i < g->set_n; ++i)
+    if (order[i] >= g->set_n) return fail(SPX_ERANGE, "order index out of range");
+  const long G = static_cast<long>(batch), L = static_cast<long>(g->set_L);
+  SPX_TRY(reserve(g, G, L));
+  spx_ctx* c = g->ctx;
+  SPX_HIP(hipMemcpyAsync(g->set_idx, order, g->set_n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  SPX_HIP(hipMemsetAsync(g->loss_acc, 0, sizeof(double), c->stream));
+  const size_t steps = g->set_n / batch;
+  for (size_t st = 0; st < steps; ++st) {
+    k_gather<<<static_cast<unsigned>(G), 128, 0, c->stream>>>(g->set_J, g->set_h, g->set_t, g->set_idx + st * batch,
+                                                              static_cast<int>(L), g->P, g->Jin, g->hin, g->tgt);
+    SPX_TRY(forward(g, G, L));
+    SPX_TRY(backward(g, G, L, nullptr));
+    k_loss_acc<<<1, 256, 0, c->stream>>>(g->sq, G * g->P, g->loss_acc);
+    ++g->step;
+    const float b1t = static_cast<float>(std::pow(0.9, static_cast<double>(g->step)));
+    const float b2t = static_cast<float>(std::pow(0.999, static_cast<double>(g->step)));
+    k_adam<<<nb(static_cast<long>(g->nparam)), 256, 0, c->stream>>>(g->p, g->grad, g->m, g->v,
+                                                                    static_cast<long>(g->nparam), lr, b1t, b2t);
+  }
+  SPX_HIP(hipGetLastError());
+  double acc = 0.0;
+  SPX_HIP(hipMemcpyAsync(&acc, g->loss_acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  SPX_HIP(hipStreamSynchronize(c->stream));
+  if (mean_loss) *mean_loss = static_cast<float>(acc / static_cast<double>(steps * batch * g->P));
   return SPX_OK;
 }
 

@@ -152,6 +152,8 @@ SIGNATURES = {
     "spx_gnn_get_params": (C.c_int, [_vp, _fp]),
     "spx_gnn_set_params": (C.c_int, [_vp, _fp]),
     "spx_gnn_predict": (C.c_int, [_vp, C.c_size_t, C.c_size_t, _dp, _dp, _fp]),
+    "spx_gnn_set_training_set": (C.c_int, [_vp, C.c_size_t, C.c_size_t, _dp, _dp, _fp]),
+    "spx_gnn_train_epoch": (C.c_int, [_vp, C.c_size_t, _u32p, C.c_float, _fp]),
     "spx_gnn_predict_dipolar": (C.c_int, [_vp, C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double,
                                           C.c_double, C.c_int, C.c_double, _fp]),
     "spx_dipolar_realisation": (C.c_int, [C.c_uint64, C.c_size_t, C.c_size_t, C.c_double, C.c_double, C.c_int,
@@ -684,6 +686,21 @@ class Surrogate:
         check(lib().spx_gnn_gradient(self.h, J.shape[0], J.shape[1], _d(J), _d(h), t.ctypes.data_as(_fp),
                                      C.byref(loss), grad.ctypes.data_as(_fp)))
         return loss.value, grad
+
+    def set_training_set(self, J, h, target) -> None:
+        J, h = self._graphs(J, h)
+        t = self._targets(J, target)
+        check(lib().spx_gnn_set_training_set(self.h, J.shape[0], J.shape[1], _d(J), _d(h),
+                                             t.ctypes.data_as(_fp)))
+        self.set_n = J.shape[0]
+
+    def train_epoch(self, order, batch: int, lr: float) -> float:
+        o = np.ascontiguousarray(order, dtype=np.uint32)
+        if o.size != self.set_n:
+            raise ValueError("order must be a permutation of the training set")
+        loss = C.c_float()
+        check(lib().spx_gnn_train_epoch(self.h, batch, o.ctypes.data_as(_u32p), lr, C.byref(loss)))
+        return loss.value
 
     def train_step(self, J, h, target, lr: float) -> float:
         J, h = self._graphs(J, h)

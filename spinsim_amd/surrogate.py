@@ -67,14 +67,13 @@ class Model:
         if n < batch:
             raise ValueError("fewer training graphs than one batch")
         rng = np.random.default_rng(seed)
+        self.net.set_training_set(J, h, T)
         history = []
         for _ in range(epochs):
             order = rng.permutation(n)
-            losses = []
             for s in range(0, n - batch + 1, batch):
-                idx = np.sort(order[s:s + batch])
-                losses.append(self.net.train_step(J[idx], h[idx], T[idx], lr))
-            history.append(float(np.mean(losses)))
+                order[s:s + batch] = np.sort(order[s:s + batch])
+            history.append(self.net.train_epoch(order, batch, lr))
             lr *= decay
         return history
 

@@ -92,6 +92,30 @@ def test_gnn_adam_step_matches_oracle(ctx):
     assert np.array_equal(sur.get_params(), p0.astype(np.float32))
 
 
+def test_resident_epoch_equals_step_by_step_training(ctx):
+    """An epoch over the device-resident training set applies exactly the
+    updates of train_step on the same minibatches (bitwise parameters)."""
+    banner("gnn resident epoch vs train_step")
+    P, L, n, batch = 8, 9, 64, 16
+    J, h = random_graphs(n, L, 8)
+    t = np.random.default_rng(9).normal(size=(n, P)).astype(np.float32)
+    a = spx.Surrogate(ctx, P, seed=2)
+    b = spx.Surrogate(ctx, P, seed=2)
+    a.set_training_set(J, h, t)
+    rng = np.random.default_rng(1)
+    for epoch in range(2):
+        order = rng.permutation(n).astype(np.uint32)
+        loss_a = a.train_epoch(order, batch, 1e-3)
+        losses = [b.train_step(J[order[s:s + batch]], h[order[s:s + batch]], t[order[s:s + batch]], 1e-3)
+                  for s in range(0, n, batch)]
+        assert np.array_equal(a.get_params(), b.get_params()), epoch
+        assert abs(loss_a - np.mean(losses)) <= 1e-5 * max(1.0, abs(loss_a))
+    with pytest.raises(spx.OutOfRange):
+        a.train_epoch(np.full(n, n, np.uint32), batch, 1e-3)
+    with pytest.raises(spx.InvalidArgument):
+        a.train_epoch(np.arange(n, dtype=np.uint32), n + 1, 1e-3)
+
+
 def test_gnn_argument_errors(ctx):
     banner("gnn argument errors")
     with pytest.raises(spx.InvalidArgument):

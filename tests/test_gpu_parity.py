@@ -398,3 +398,30 @@ def test_resident_batch_uneven_sizes_finite_pulses_match_oracle(ctx):
         assert err < 1e-10, (i, Ls[i], err)
     _, gv2 = b.run()
     assert np.array_equal(gv, gv2)
+
+
+def test_imaginary_time_relaxes_to_the_ground_state(ctx):
+    """Imaginary-time propagation of -H with autonormalisation (the ground-state
+    run of the single-large-state workload) reaches the lowest eigenvalue of H
+    from a dense eigensolver, L=10, and its eigenvector."""
+    L = 10
+    banner("imaginary time ground state L=10")
+    for index in range(7, 40):  # a realisation with a gap, so the relaxation converges fast
+        fields, c = dipolar_system(L, index)
+        H = D.hamiltonian(L, fields, c.lo, c.hi, c.jx, c.jy, c.jz)
+        w, V = np.linalg.eigh(H)
+        if w[1] - w[0] > 0.05:
+            break
+    assert w[1] - w[0] > 0.05
+    hneg = device_ham(ctx, L, -fields, type(c)(c.lo, c.hi, -c.jx, -c.jy, -c.jz))
+    hpos = device_ham(ctx, L, fields, c)
+    s = spx.StateVector(ctx, L, O.random_state(L, 3))
+    prop = spx.Propagator(ctx, L, 1e-12)
+    tau = 0.1
+    steps = int(math.ceil(40.0 / ((w[1] - w[0]) * tau)))
+    prop.evolve(hneg, spx.IMAGINARY, s, tau, steps, autonormalize=True)
+    e = spx.measure_energy(hpos, s)
+    assert abs(s.norm() - 1.0) < 1e-12
+    assert abs(e - w[0]) < 1e-9 * max(1.0, abs(w[0])), (e, w[0])
+    overlap = abs(np.vdot(V[:, 0], D.to_complex(s.download())))
+    assert overlap > 1.0 - 1e-9
