@@ -266,6 +266,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule")
+    ap.add_argument("--graphs", action="store_true", help="replay each step as a captured HIP graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="L",
                     help="only time the in-place scale kernel on 2^L amplitudes")
     args = ap.parse_args()
@@ -296,6 +297,7 @@ def main():
     order = coeffs.size - 1
     psi = spx.StateVector(ctx, L, psi0)
     prop = spx.Propagator(ctx, L, args.epsilon)
+    prop.set_graphs(args.graphs)
     log(f"[bench] bound={bound:.3f} alpha={bound * args.dt:.3f} order={order}")
 
     prop.steps(ham, spx.REAL, psi, coeffs, eff, args.warmup)

@@ -112,7 +112,8 @@ int spx_ham_schedule(const spx_ham* h, char* buf, size_t len);
 /* ------------------------------------------------- Autotuner (a16)
  * No reference counterpart. Times every eligible launch schedule of the
  * order kernels (all-pairs kernel at tile bits 10/9, table kernels at tile
- * bits 8..10, with and without the XCD-aware tile order, the general kernel
+ * bits 6..10 (6 and 7 up to L = 16), with and without the XCD-aware tile
+ * order, the general kernel
  * at two block sizes) on this Hamiltonian at the step bound `bound`, keeps
  * the fastest and caches it per problem shape (L, all-pairs eligibility,
  * flip-term count) for the process; `report` receives the timings. */
@@ -137,6 +138,12 @@ int spx_prop_evolve(spx_prop* p, spx_ham* h, int mode, spx_state* psi, double ta
 int spx_prop_evolve_many(spx_prop* p, spx_ham* h, int mode, spx_state* const* states,
                          size_t nstates, double tau, size_t nsteps, int autonormalize);
 int spx_prop_last_order(const spx_prop* p, size_t* order);                      /* lastOrder */
+/* No reference counterpart: with enable = 1, evolve calls of at least 4
+ * steps without autonormalisation record one step as a HIP graph and replay
+ * it; with enable = 0 (the default) every order is launched from the host.
+ * Both run the same kernels with the same arguments (bitwise the same
+ * result); on this part the host launches already keep the device busy. */
+int spx_prop_set_graphs(spx_prop* p, int enable);
 int spx_prop_epsilon(const spx_prop* p, double* epsilon);                       /* epsilon */
 /* Hot loop with a prebuilt expansion: `nsteps` applyOneStep calls of
  * `order` fused Chebyshev orders (coeffs[0..order]) at `bound`. */

@@ -373,6 +373,12 @@ int spx_ham_schedule(const spx_ham* h, char* buf, size_t len) {
 
 // --------------------------------------------------------------- Propagator
 
+int spx_prop_set_graphs(spx_prop* p, int enable) {
+  if (!p) return fail(SPX_EINVAL, "null propagator");
+  p->graphs = enable ? 1 : 0;
+  return SPX_OK;
+}
+
 int spx_prop_create(spx_ctx* c, size_t nspins, double epsilon, spx_prop** out) {
   if (!c || !out) return fail(SPX_EINVAL, "null argument");
   if (!(epsilon > 0.0)) return fail(SPX_EINVAL, "epsilon must be positive");
@@ -419,6 +425,8 @@ int ensure_scratch(spx_prop* p, size_t nvec) {
 // T_1 into B; T_2 into A with psi = a0 T_0 + c1 T_1 + c2 T_2 (psi still holds
 // T_0 until then, so no copy of it is made); from there T_n overwrites
 // T_{n-2} in place and psi += c_n T_n in the same pass.
+constexpr size_t kGraphMinSteps = 4;
+
 int one_step(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t nvec,
              const std::vector<double>& a) {
   const size_t order = a.size() - 1;
@@ -508,6 +516,31 @@ int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, siz
   SPX_TRY(ensure_scratch(p, nstates < kMaxMany ? nstates : kMaxMany));
   SPX_TRY(autotune_before_evolve(h, bound));
   SPX_TRY(ham_prepare(h, -2.0 / bound));
+  // Without autonormalisation every step is the same sequence of launches
+  // (same expansion, same buffers): record it once as a HIP graph and replay
+  // it, so the host issues one launch per step instead of one per order.
+  if (!autonormalize && p->graphs && nsteps >= kGraphMinSteps) {
+    hipStream_t st = p->ctx->stream;
+    SPX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = SPX_OK;
+    for (size_t first = 0; first < nstates && rc == SPX_OK; first += kMaxMany) {
+      const size_t cnt = nstates - first < static_cast<size_t>(kMaxMany) ? nstates - first : kMaxMany;
+      rc = one_step(p, h, mode, states + first, cnt, coeffs);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (rc != SPX_OK || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc != SPX_OK ? rc : fail(SPX_EDEVICE, std::string("capturing a step: ") + hipGetErrorString(ec));
+    }
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    for (size_t step = 0; step < nsteps && e == hipSuccess; ++step) e = hipGraphLaunch(exec, st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(SPX_EDEVICE, std::string("replaying a step: ") + hipGetErrorString(e));
+    return SPX_OK;
+  }
   for (size_t step = 0; step < nsteps; ++step) {
     for (size_t first = 0; first < nstates; first += kMaxMany) {
       const size_t cnt = nstates - first < static_cast<size_t>(kMaxMany) ? nstates - first : kMaxMany;

@@ -124,6 +124,7 @@ struct spx_prop {
   double epsilon = 1e-7;
   size_t last_order = 0;
   double2* d_scratch[2 * spx::kMaxMany] = {};
+  int graphs = 0;   // record one step as a HIP graph and replay it
 };
 
 namespace spx {

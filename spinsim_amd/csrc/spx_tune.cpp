@@ -2,9 +2,11 @@
 //
 // For one Hamiltonian at one step bound it times every eligible schedule —
 // the all-pairs kernel at tile bits 10 and 9, the table kernels at tile bits
-// 8..10, each with and without the XCD-aware tile order, and the
+// 6..10, each with and without the XCD-aware tile order, and the
 // per-amplitude-group kernel at two block sizes — on the general Chebyshev
 // order (T_n = 2Ĥ T_{n-1} - T_{n-2}, psi += c T_n) and keeps the fastest.
+// Small tiles (6, 7 bits) are tried up to L = 16, where few large tiles
+// would leave most compute units idle.
 // Winners are cached per problem shape (L, whether the all-pairs kernel is
 // eligible, number of flip terms), so the realisations of an ensemble pay
 // for one tuning. Schedules change the order of floating-point additions, not
@@ -68,8 +70,8 @@ std::vector<Schedule> candidates(const spx_ham* h, bool dense) {
         s.xcd = xcd;
         c.push_back(s);
       }
-  for (int tb : {10, 9, 8}) {
-    if (tb > L) continue;
+  for (int tb : {10, 9, 8, 7, 6}) {
+    if (tb > L || (tb < 8 && L > 16)) continue;
     for (int xcd : {0, 1}) {
       if (xcd == 1 && L - tb < 3) continue;
       Schedule s;

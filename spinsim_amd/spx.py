@@ -115,6 +115,7 @@ SIGNATURES = {
     "spx_prop_evolve_many": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), C.c_size_t, C.c_double,
                                        C.c_size_t, C.c_int]),
     "spx_prop_last_order": (C.c_int, [_vp, _szp]),
+    "spx_prop_set_graphs": (C.c_int, [_vp, C.c_int]),
     "spx_prop_epsilon": (C.c_int, [_vp, _dp]),
     "spx_prop_steps": (C.c_int, [_vp, _vp, C.c_int, _vp, _dp, C.c_size_t, C.c_double, C.c_size_t,
                                  C.c_int]),
@@ -420,6 +421,9 @@ class Propagator:
     def evolve(self, ham: Hamiltonian, mode: int, psi: StateVector, tau: float, nsteps: int = 1,
                autonormalize: bool = False):
         check(lib().spx_prop_evolve(self.h, ham.h, mode, psi.h, tau, nsteps, int(autonormalize)))
+
+    def set_graphs(self, enable: bool) -> None:
+        check(lib().spx_prop_set_graphs(self.h, int(enable)))
 
     def evolve_many(self, ham: Hamiltonian, mode: int, states: Sequence[StateVector], tau: float,
                     nsteps: int = 1, autonormalize: bool = False):

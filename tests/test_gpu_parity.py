@@ -425,3 +425,34 @@ def test_imaginary_time_relaxes_to_the_ground_state(ctx):
     assert abs(e - w[0]) < 1e-9 * max(1.0, abs(w[0])), (e, w[0])
     overlap = abs(np.vdot(V[:, 0], D.to_complex(s.download())))
     assert overlap > 1.0 - 1e-9
+
+
+@pytest.mark.parametrize("L", [9, 14, 16])
+def test_step_graph_replays_equal_eager_launches_bitwise(ctx, L):
+    """Evolve calls that record one step as a HIP graph and replay it agree
+    bitwise with the same calls launched order by order, over several calls
+    with eager work (gates, measurements, host syncs) between them; also for
+    several states at once and in imaginary time."""
+    banner(f"graph replay vs eager L={L}")
+    fields, c = dipolar_system(L, 21, transverse=(L == 9))
+    h = device_ham(ctx, L, fields, c)
+    psi0, ref0 = O.random_state(L, 5), O.random_state(L, 6)
+    pg, pe = spx.Propagator(ctx, L, 1e-10), spx.Propagator(ctx, L, 1e-10)
+    pg.set_graphs(True)
+    pe.set_graphs(False)
+    sg, se = spx.StateVector(ctx, L, psi0), spx.StateVector(ctx, L, psi0)
+    rg, re = spx.StateVector(ctx, L, ref0), spx.StateVector(ctx, L, ref0)
+    for rep in range(3):
+        pg.evolve(h, spx.REAL, sg, 0.05, 6)
+        pe.evolve(h, spx.REAL, se, 0.05, 6)
+        ctx.synchronize()
+        assert np.array_equal(sg.download(), se.download()), rep
+        spx.rotate(sg, (0, 1, 0), math.pi / 3, list(range(L)))
+        spx.rotate(se, (0, 1, 0), math.pi / 3, list(range(L)))
+        assert spx.measure_mag(0, 0, sg) == spx.measure_mag(0, 0, se)
+        pg.evolve_many(h, spx.REAL, [sg, rg], 0.05, 5)
+        pe.evolve_many(h, spx.REAL, [se, re], 0.05, 5)
+        assert np.array_equal(sg.download(), se.download()) and np.array_equal(rg.download(), re.download())
+    pg.evolve(h, spx.IMAGINARY, sg, 0.02, 8)
+    pe.evolve(h, spx.IMAGINARY, se, 0.02, 8)
+    assert np.array_equal(sg.download(), se.download())
