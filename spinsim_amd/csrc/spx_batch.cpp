@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -123,35 +124,109 @@ struct Walker {
 }  // namespace
 
 // Per-realisation device objects, one kernel sequence per program step.
-int batch_run_streaming(spx_batch* b, size_t first, size_t count, double* times, double* values) {
-  const size_t points = program_measurements(b->program, 0, b->program.ops.size());
-  for (size_t i = first; i < first + count; ++i) {
-    const Realisation& r = b->items[i];
-    spx_ham* h = nullptr;
-    spx_state* state = nullptr;
-    spx_state* ref = nullptr;
-    spx_prop* prop = nullptr;
-    int rc = spx_ham_create(b->ctx, r.nspins, &h);
-    if (rc == SPX_OK) rc = spx_state_create(b->ctx, r.nspins, &state);
-    if (rc == SPX_OK) rc = spx_state_create(b->ctx, r.nspins, &ref);
-    if (rc == SPX_OK) rc = spx_prop_create(b->ctx, r.nspins, b->epsilon, &prop);
-    for (size_t s = 0; rc == SPX_OK && s < r.nspins; ++s)
-      rc = spx_ham_set_field(h, s, r.fields[3 * s], r.fields[3 * s + 1], r.fields[3 * s + 2]);
-    for (size_t p = 0; rc == SPX_OK && p < r.lo.size(); ++p)
-      rc = spx_ham_set_coupling(h, r.lo[p], r.hi[p], r.jx[p], r.jy[p], r.jz[p]);
-    if (rc == SPX_OK) rc = spx_state_upload(state, r.psi0.data());
-    if (rc == SPX_OK) rc = spx_apply_total_spin(b->reference_axis, state, ref);
-    if (rc == SPX_OK) {
-      Walker w{b, &r, h, state, ref, prop, 0.0, (i == first) ? times : nullptr,
-               values + (i - first) * points * b->width};
-      rc = w.run(0, b->program.ops.size());
-    }
+// Device objects of one worker, reused across its realisations (allocation
+// and release synchronise the device, which would serialise the workers);
+// rebuilt only when L changes.
+struct WorkerObjects {
+  spx_ctx* c = nullptr;
+  size_t L = 0;
+  spx_ham* h = nullptr;
+  spx_state* state = nullptr;
+  spx_state* ref = nullptr;
+  spx_prop* prop = nullptr;
+
+  void release() {
     spx_prop_destroy(prop);
     spx_state_destroy(ref);
     spx_state_destroy(state);
     spx_ham_destroy(h);
-    SPX_TRY(rc);
+    prop = nullptr;
+    ref = state = nullptr;
+    h = nullptr;
+    L = 0;
   }
+  ~WorkerObjects() { release(); }
+
+  int prepare(size_t nspins, double epsilon) {
+    if (L == nspins) {
+      // same size: clear the Hamiltonian's fields and couplings (its device
+      // caches are keyed on them and rebuild on first use)
+      std::fill(h->hx.begin(), h->hx.end(), 0.0);
+      std::fill(h->hy.begin(), h->hy.end(), 0.0);
+      std::fill(h->hz.begin(), h->hz.end(), 0.0);
+      h->lo.clear();
+      h->hi.clear();
+      h->jx.clear();
+      h->jy.clear();
+      h->jz.clear();
+      h->terms_valid = false;
+      h->sched_set = false;
+      return SPX_OK;
+    }
+    release();
+    SPX_TRY(spx_ham_create(c, nspins, &h));
+    SPX_TRY(spx_state_create(c, nspins, &state));
+    SPX_TRY(spx_state_create(c, nspins, &ref));
+    SPX_TRY(spx_prop_create(c, nspins, epsilon, &prop));
+    L = nspins;
+    return SPX_OK;
+  }
+};
+
+// One realisation on a worker's objects, rows written at `values`.
+static int run_one(spx_batch* b, WorkerObjects& o, const Realisation& r, double* times, double* values) {
+  SPX_TRY(o.prepare(r.nspins, b->epsilon));
+  for (size_t s = 0; s < r.nspins; ++s)
+    SPX_TRY(spx_ham_set_field(o.h, s, r.fields[3 * s], r.fields[3 * s + 1], r.fields[3 * s + 2]));
+  for (size_t p = 0; p < r.lo.size(); ++p)
+    SPX_TRY(spx_ham_set_coupling(o.h, r.lo[p], r.hi[p], r.jx[p], r.jy[p], r.jz[p]));
+  SPX_TRY(spx_state_upload(o.state, r.psi0.data()));
+  SPX_TRY(spx_apply_total_spin(b->reference_axis, o.state, o.ref));
+  Walker w{b, &r, o.h, o.state, o.ref, o.prop, 0.0, times, values};
+  SPX_TRY(w.run(0, b->program.ops.size()));
+  return spx_ctx_synchronize(o.c);
+}
+
+// Realisations one after another on the batch's context, or, with several
+// workers, realisation i on worker i mod W, each worker a host thread with
+// its own context (stream), as LocalRunner's pool runs jobs concurrently;
+// the kernels of different realisations then overlap on the device. Every
+// realisation runs the same kernels on its own rows, so the output does not
+// depend on the number of workers.
+int batch_run_streaming(spx_batch* b, size_t first, size_t count, double* times, double* values) {
+  const size_t points = program_measurements(b->program, 0, b->program.ops.size());
+  static const int env_workers = [] {
+    const char* e = std::getenv("SPX_BATCH_WORKERS");
+    return e ? std::atoi(e) : 4;
+  }();
+  const size_t workers = std::min<size_t>(count, static_cast<size_t>(std::max(1, std::min(env_workers, 8))));
+  if (workers <= 1) {
+    WorkerObjects o;
+    o.c = b->ctx;
+    for (size_t i = first; i < first + count; ++i)
+      SPX_TRY(run_one(b, o, b->items[i], (i == first) ? times : nullptr, values + (i - first) * points * b->width));
+    return SPX_OK;
+  }
+  std::vector<int> rcs(workers, SPX_OK);
+  std::vector<std::string> msgs(workers);
+  std::vector<std::thread> pool;
+  for (size_t w = 0; w < workers; ++w)
+    pool.emplace_back([&, w] {
+      spx_ctx* c = nullptr;
+      int rc = spx_ctx_create(b->ctx->device, &c);
+      {
+        WorkerObjects o;
+        o.c = c;
+        for (size_t i = first + w; rc == SPX_OK && i < first + count; i += workers)
+          rc = run_one(b, o, b->items[i], (i == first) ? times : nullptr, values + (i - first) * points * b->width);
+        if (rc != SPX_OK) msgs[w] = spx_last_error();
+      }
+      if (c) spx_ctx_destroy(c);
+      rcs[w] = rc;
+    });
+  for (auto& t : pool) t.join();
+  for (size_t w = 0; w < workers; ++w)
+    if (rcs[w] != SPX_OK) return fail(rcs[w], msgs[w]);
   return SPX_OK;
 }
 

@@ -4,6 +4,7 @@ one-pass gather order, so agreement is to rounding (stated per test), except
 where the arithmetic is identical by construction (diagonal, Bessel,
 Chebyshev coefficients, identity steps), which is checked bitwise."""
 import math
+import time
 
 import numpy as np
 import pytest
@@ -456,3 +457,29 @@ def test_step_graph_replays_equal_eager_launches_bitwise(ctx, L):
     pg.evolve(h, spx.IMAGINARY, sg, 0.02, 8)
     pe.evolve(h, spx.IMAGINARY, se, 0.02, 8)
     assert np.array_equal(sg.download(), se.download())
+
+
+def test_streaming_batch_mid_size_matches_oracle_and_repeats(ctx):
+    """Realisations above the LDS-resident size (L = 13, 14) go through the
+    streaming batch path (worker contexts running realisations concurrently):
+    every realisation's rows match the oracle's within 1e-10, and a second
+    run is bitwise identical."""
+    banner("streaming batch L=13/14")
+    obs = ((O.OBS_CORR_X, 0), (O.OBS_NORM, 0), (O.OBS_MAG_Z, 2))
+    Ls = [13, 14, 13, 14, 13, 14]
+    b = spx.Batch(ctx, cpmg(spx.Program(), 0, cycles=3), obs, epsilon=1e-7, reference_axis=0)
+    want = []
+    for i, L in enumerate(Ls):
+        fields, c, psi0 = realisation_inputs(L, 20240726, 500 + i)
+        b.add(L, fields, c.lo, c.hi, c.jx, c.jy, c.jz, psi0)
+        cfg = O.Config(L=L, program=cpmg(O.Program(), L, cycles=3), observables=obs, field_disorder=0.5,
+                       min_separation=0.5, angular=True)
+        want.append(O.run_realization(cfg, 500 + i)[1])
+    t0 = time.perf_counter()
+    _, gv = b.run()
+    banner(f"streaming batch of {len(Ls)}: {time.perf_counter() - t0:.3f} s")
+    for i in range(len(Ls)):
+        err = float(np.max(np.abs(gv[i] - want[i])))
+        assert err < 1e-10, (i, Ls[i], err)
+    _, gv2 = b.run()
+    assert np.array_equal(gv, gv2)
