@@ -1,17 +1,19 @@
 #!/usr/bin/env python3
 """Headline benchmark: the L=18 FP64 Chebyshev propagator on one MI355X.
 
-Workload (the single-large-state configuration): 2^18 complex FP64
-amplitudes, all-to-all secular dipolar XXZ couplings (153 pairs) from random
-2D positions at unit density (min spacing 0.5, (1 - 3cos^2 theta)/r^3) plus
-Gaussian Hz disorder (sigma 0.5), real-time steps of dt = 0.05 at
-epsilon = 1e-12, autonormalisation off. One bench "step" is one Chebyshev
-time step: `order` fused orders, each one HBM pass of
-T_{n-1}, T_{n-2}, T_n, the diagonal and psi.
+Workload (the single-large-state configuration, whose 64 realisations run
+four at a time): 2^18 complex FP64 amplitudes per realisation, all-to-all
+secular dipolar XXZ couplings (153 pairs) from random 2D positions at unit
+density (min spacing 0.5, (1 - 3cos^2 theta)/r^3) plus Gaussian Hz disorder
+(sigma 0.5), real-time steps of dt = 0.05 at epsilon = 1e-12,
+autonormalisation off. One bench "step" is one Chebyshev time step of each of
+the `--set` realisations (default 4), stepped together: every order is one
+launch sequence for the set, each realisation with its own bound and order.
 
 Prints ONE JSON line. `value` is algorithmic GB/s of the whole timed region
-(inputs resident in HBM); `roofline` prices the dominant kernel (a general
-Chebyshev order, 88 B per amplitude) against the 8 TB/s HBM peak; the CPU
+(inputs resident in HBM, 88 B per amplitude per general order); `roofline`
+prices the dominant launch (a general Chebyshev order over the set) against
+the 8 TB/s HBM peak; `single_state` gives one realisation alone; the CPU
 baseline runs the same per-order work through the reference's own kernels
 (oracle/_ref) or the oracle restatement, on the host's cores.
 """
@@ -31,10 +33,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "H·ψ GB/s vs HBM peak at L=18; s to ±1e-3 echo error bar, CV vs plain MC"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the general-order kernel at L=18 (tile bits 10):
-# FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from the in-place scale
-# kernel of known size), profiles/k_dense_L18_order.md.
-MEASURED_TRAFFIC = {("k_dense<K_ORDERN, 18, 10>"): 42417 * 1024}
+# HBM bytes per general-order launch at L=18, keyed by (order kernel, states
+# per launch): FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from the
+# in-place scale kernel of known size), profiles/k_dense_L18_order.md.
+MEASURED_TRAFFIC = {("k_dense<K_ORDERN, 18, 10>", 1): 42417 * 1024}
 BYTES_PER_AMP_ORDER = 88          # read T_{n-1}, T_{n-2}, diag; write T_n; read+write psi
 BYTES_FIRST, BYTES_SECOND = 40, 72  # order 1 (psi -> T_1), order 2 (fused psi update)
 
@@ -43,7 +45,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def synthetic_system(L: int, seed: int = 20240726, index: int = 0, sigma: float = 0.5, min_sep: float = 0.5):
+SEED = 20240726
+
+
+def synthetic_system(L: int, seed: int = SEED, index: int = 0, sigma: float = 0.5, min_sep: float = 0.5):
     """Realisation (seed, index) of the dipolar ensemble, drawn by libspx in
     the reference's order (secular (1 - 3cos^2 theta)/r^3 couplings at unit
     density, min spacing 0.5, Gaussian Hz of spread sigma); a normalised
@@ -52,7 +57,7 @@ def synthetic_system(L: int, seed: int = 20240726, index: int = 0, sigma: float 
     d = spx.dipolar_realisation(seed, index, L, 1.0, min_sep, True, sigma)
     fields = np.zeros((L, 3))
     fields[:, 2] = d.hz
-    psi = np.random.default_rng(seed).standard_normal(2 << L)
+    psi = np.random.default_rng([seed, index]).standard_normal(2 << L)
     psi /= np.linalg.norm(psi)
     return fields, d.lo, d.hi, d.jx, d.jy, d.jz, psi
 
@@ -236,6 +241,39 @@ def batched_leg(spx, ctx, count=4096, seed=20240726):
             "mean_L": round(float(np.mean(Ls)), 2)}
 
 
+def single_state_leg(spx, ctx, L, system, args):
+    """The first realisation alone (one state per launch, the schedule tuned
+    at width 1): algorithmic GB/s of its steps and its general order."""
+    fields, lo, hi, jx, jy, jz, psi0 = system
+    ham = spx.Hamiltonian(ctx, L)
+    ham.set_fields(fields)
+    ham.set_couplings(lo, hi, jx, jy, jz)
+    coeffs, eff = spx.cheb_build(spx.REAL, ham.spectral_bound(), args.dt, args.epsilon)
+    if not args.no_tune:
+        ham.autotune(eff, nvec=1)
+    order = coeffs.size - 1
+    n = 1 << L
+    psi = spx.StateVector(ctx, L, psi0)
+    prop = spx.Propagator(ctx, L, args.epsilon)
+    prop.steps(ham, spx.REAL, psi, coeffs, eff, args.warmup)
+    ctx.record(5)
+    prop.steps(ham, spx.REAL, psi, coeffs, eff, args.steps)
+    ctx.record(6)
+    reps = max(20, args.steps // 4)
+    short = coeffs[: max(3, order // 2) + 1]
+    prop.steps(ham, spx.REAL, psi, coeffs, eff, reps)
+    ctx.record(7)
+    prop.steps(ham, spx.REAL, psi, short, eff, reps)
+    ctx.record(8)
+    ctx.synchronize()
+    ms = ctx.elapsed_ms(5, 6)
+    per_order_ms = (ctx.elapsed_ms(6, 7) - ctx.elapsed_ms(7, 8)) / (reps * (order - (short.size - 1)))
+    return {"value": round(step_bytes(order, n) * args.steps / (ms * 1e-3) / 1e9, 2), "unit": "GB/s",
+            "order": int(order), "kernel": ham.kernel_name(),
+            "us_per_order": round(per_order_ms * 1e3, 3),
+            "order_GBs": round(BYTES_PER_AMP_ORDER * n / (per_order_ms * 1e-3) / 1e9, 2)}
+
+
 def copy_probe(spx, L, reps=50):
     """In-place scale (read + write 16 B per amplitude): the plain streaming
     kernel that calibrates the HBM counters and the achievable bandwidth."""
@@ -266,6 +304,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule")
+    ap.add_argument("--set", type=int, default=4, metavar="S",
+                    help="realisations stepped together, one launch per order (1..4; 1 = a single state)")
     ap.add_argument("--graphs", action="store_true", help="replay each step as a captured HIP graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="L",
                     help="only time the in-place scale kernel on 2^L amplitudes")
@@ -282,58 +322,78 @@ def main():
         return
     L = args.nspins
     n = 1 << L
-    fields, lo, hi, jx, jy, jz, psi0 = synthetic_system(L)
-    log(f"[bench] L={L} pairs={lo.size} building device objects")
+    S = args.set
+    if not 1 <= S <= 4:
+        raise SystemExit("--set takes 1..4 realisations per launch")
+    systems = [synthetic_system(L, index=i) for i in range(S)]
+    fields, lo, hi, jx, jy, jz, _ = systems[0]
+    log(f"[bench] L={L} pairs={lo.size} realisations={S} building device objects")
     ctx = spx.Context(0)
-    ham = spx.Hamiltonian(ctx, L)
-    ham.set_fields(fields)
-    ham.set_couplings(lo, hi, jx, jy, jz)
-    bound = ham.spectral_bound()
-    coeffs, eff = spx.cheb_build(spx.REAL, bound, args.dt, args.epsilon)
+    hams, states = [], []
+    for f, l, h_, x, y, z, p0 in systems:
+        ham = spx.Hamiltonian(ctx, L)
+        ham.set_fields(f)
+        ham.set_couplings(l, h_, x, y, z)
+        hams.append(ham)
+        states.append(spx.StateVector(ctx, L, p0))
+    built = [spx.cheb_build(spx.REAL, h.spectral_bound(), args.dt, args.epsilon) for h in hams]
+    coeffs = [c for c, _ in built]
+    effs = [e for _, e in built]
+    orders = [c.size - 1 for c in coeffs]
     tune_report = None
     if not args.no_tune:
-        tune_report = ham.autotune(eff)
-        log(f"[bench] autotune: {tune_report[:300]}")
-    order = coeffs.size - 1
-    psi = spx.StateVector(ctx, L, psi0)
+        # the schedule for launches of S states; the other realisations adopt
+        # it from the tuner's cache (same shape and width)
+        tune_report = hams[0].autotune(effs[0], nvec=S)
+        spx.autotune_mode(1)
+        log(f"[bench] autotune (width {S}): {tune_report[:300]}")
     prop = spx.Propagator(ctx, L, args.epsilon)
     prop.set_graphs(args.graphs)
-    log(f"[bench] bound={bound:.3f} alpha={bound * args.dt:.3f} order={order}")
+    log(f"[bench] bounds={[round(h.spectral_bound(), 3) for h in hams]} orders={orders}")
 
-    prop.steps(ham, spx.REAL, psi, coeffs, eff, args.warmup)
+    prop.steps_set(hams, spx.REAL, states, coeffs, effs, args.warmup)
     ctx.synchronize()
     log(f"[bench] warmup {args.warmup} steps done")
 
     ctx.record(0)
-    prop.steps(ham, spx.REAL, psi, coeffs, eff, args.steps)
+    prop.steps_set(hams, spx.REAL, states, coeffs, effs, args.steps)
     ctx.record(1)
     ctx.synchronize()
     ms = ctx.elapsed_ms(0, 1)
-    log(f"[bench] timed {args.steps} steps: {ms:.2f} ms")
-    total_bytes = step_bytes(order, n) * args.steps
+    log(f"[bench] timed {args.steps} steps of {S} realisations: {ms:.2f} ms")
+    total_bytes = sum(step_bytes(o, n) for o in orders) * args.steps
     value = total_bytes / (ms * 1e-3) / 1e9
 
-    # Dominant kernel: a general order (T_n = 2 Hhat T_{n-1} - T_{n-2}; psi += c T_n).
-    # Its per-launch time is the difference between two step lengths of the
-    # same stream of launches, so the first/second-order kernels cancel out.
+    # Dominant launch: a general order over the S states (T_n = 2 Hhat T_{n-1}
+    # - T_{n-2}; psi += c T_n). Its time is the difference between two runs of
+    # the same launch stream whose expansions are cut to common lengths m and
+    # s < m, so every launch carries all S states and the first- and
+    # second-order launches cancel out.
     reps = max(20, args.steps // 4)
-    short = coeffs[: max(3, order // 2) + 1]
+    m = min(orders)
+    sh = max(3, m // 2)
     ctx.record(2)
-    prop.steps(ham, spx.REAL, psi, coeffs, eff, reps)
+    prop.steps_set(hams, spx.REAL, states, [c[: m + 1] for c in coeffs], effs, reps)
     ctx.record(3)
-    prop.steps(ham, spx.REAL, psi, short, eff, reps)
+    prop.steps_set(hams, spx.REAL, states, [c[: sh + 1] for c in coeffs], effs, reps)
     ctx.record(4)
     ctx.synchronize()
     t_long, t_short = ctx.elapsed_ms(2, 3), ctx.elapsed_ms(3, 4)
-    per_order_ms = (t_long - t_short) / (reps * (order - (short.size - 1)))
-    kernel_bytes = BYTES_PER_AMP_ORDER * n
-    kernel_name = ham.kernel_name()
+    per_order_ms = (t_long - t_short) / (reps * (m - sh))
+    kernel_bytes = BYTES_PER_AMP_ORDER * n * S
+    kernel_name = hams[0].kernel_name()
     achieved = kernel_bytes / (per_order_ms * 1e-3) / 1e9
-    log(f"[bench] general order: {per_order_ms * 1e3:.2f} us -> {achieved:.1f} GB/s")
+    log(f"[bench] general order over {S} states: {per_order_ms * 1e3:.2f} us -> {achieved:.1f} GB/s")
+    spx.autotune_mode(0)
 
-    psi_host = psi.download()
-    if not np.isfinite(psi_host).all():
-        raise SystemExit("non-finite state after the timed run")
+    single = None
+    if S > 1:
+        single = single_state_leg(spx, ctx, L, systems[0], args)
+        log(f"[bench] single state: {single}")
+
+    for st in states:
+        if not np.isfinite(st.download()).all():
+            raise SystemExit("non-finite state after the timed run")
 
     line = {
         "metric": METRIC,
@@ -349,24 +409,27 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"L={L} dipolar XXZ real-time Chebyshev steps, dt={args.dt}, "
-                               f"eps={args.epsilon:g}, autonormalize off (single large state)",
-                   "nspins": L, "pairs": int(lo.size), "order": int(order),
-                   "spectral_bound": round(bound, 4),
+                               f"eps={args.epsilon:g}, autonormalize off; {S} realisations (seed {SEED}, "
+                               f"index 0..{S - 1}) stepped together, one launch per order for the set",
+                   "nspins": L, "pairs": int(lo.size), "realisations": S, "orders": [int(o) for o in orders],
+                   "spectral_bounds": [round(h.spectral_bound(), 4) for h in hams],
                    "schedule": tune_report.split(";")[0] if tune_report else "default"},
-        "hpsi_per_s": round(args.steps * order / (ms * 1e-3), 1),
+        "hpsi_per_s": round(args.steps * sum(orders) / (ms * 1e-3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get(kernel_name),
-                     "kernel": kernel_name + " (general Chebyshev order)", "bytes_per_launch": kernel_bytes,
-                     "us_per_launch": round(per_order_ms * 1e3, 3)},
+                     "traffic": MEASURED_TRAFFIC.get((kernel_name, S)),
+                     "kernel": kernel_name + f" (general Chebyshev order, {S} states per launch)",
+                     "bytes_per_launch": kernel_bytes, "us_per_launch": round(per_order_ms * 1e3, 3)},
     }
+    if single:
+        line["single_state"] = single
     if not args.no_cv:
         line["control_variate"] = control_variate_leg(spx, ctx)
         log("[bench] batched ensemble leg ...")
         line["batched_ensemble"] = batched_leg(spx, ctx)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(L, fields, lo, hi, jx, jy, jz, kernel_bytes,
+        line["cpu_baseline"] = cpu_baseline(L, fields, lo, hi, jx, jy, jz, BYTES_PER_AMP_ORDER * n,
                                             args.cpu_seconds)
     print(json.dumps(line), flush=True)
 

@@ -111,18 +111,26 @@ int spx_ham_schedule(const spx_ham* h, char* buf, size_t len);
 
 /* ------------------------------------------------- Autotuner (a16)
  * No reference counterpart. Times every eligible launch schedule of the
- * order kernels (all-pairs kernel at tile bits 10/9, table kernels at tile
- * bits 6..10 (6 and 7 up to L = 16), with and without the XCD-aware tile
- * order, the general kernel
- * at two block sizes) on this Hamiltonian at the step bound `bound`, keeps
- * the fastest and caches it per problem shape (L, all-pairs eligibility,
- * flip-term count) for the process; `report` receives the timings. */
+ * order kernels (the Hadamard-sandwich path, the all-pairs kernel at tile
+ * bits 10/9, table kernels at tile bits 6..10 (6 and 7 up to L = 16), with
+ * and without the XCD-aware tile order, the general kernel at two block
+ * sizes) on this Hamiltonian at the step bound `bound`, keeps the fastest
+ * and caches it per problem shape (L, whether the couplings are flip-flop
+ * only, flip-term count) for the process; `report` receives the timings. */
 int spx_ham_autotune(spx_ham* h, double bound, char* report, size_t len);
+/* The same with `nvec` states (1..4) sharing every timed launch, the width of
+ * a batched evolve (spx_prop_evolve_many / spx_prop_evolve_set); winners are
+ * cached per width as well. */
+int spx_ham_autotune_batch(spx_ham* h, double bound, int nvec, char* report, size_t len);
 /* 0: default schedules (the default); 1: every evolve on a Hamiltonian
- * without a schedule adopts its shape's cached one, tuning it first if
- * absent. Schedules reorder floating-point additions only. */
+ * without a schedule adopts the cached one of its shape and launch width,
+ * tuning it first if absent. Schedules reorder floating-point additions only. */
 int spx_autotune_mode(int mode);
-/* Fix a schedule by hand; 0 / -1 fields keep the defaults. */
+/* Fix a schedule by hand; 0 / -1 fields keep the defaults. `dense`: -1
+ * default, 0 table kernels only, 1 the all-pairs gather kernel (L = 14..18),
+ * 2 the Hadamard-sandwich path (L = 14..20); both need flip-flop-only
+ * couplings (no transverse field, jx = jy) and otherwise fall back to the
+ * table kernels. */
 int spx_ham_set_schedule(spx_ham* h, int tile_bits, int block, int xcd, int dense, int dense_tile_bits);
 
 /* ------------------------------------------------ Propagator (a8)
@@ -137,6 +145,19 @@ int spx_prop_evolve(spx_prop* p, spx_ham* h, int mode, spx_state* psi, double ta
  * sharing one expansion and one diagonal read per order. */
 int spx_prop_evolve_many(spx_prop* p, spx_ham* h, int mode, spx_state* const* states,
                          size_t nstates, double tau, size_t nsteps, int autonormalize);
+/* State i evolved under hams[i] (a realisation each, as in LocalRunner::run,
+ * src/ensemble/EnsembleRunner.cpp:27-88, which evolves them one per thread):
+ * each with its own spectral bound and expansion order, up to 4 states per
+ * order launch where their Hamiltonians take the Hadamard or all-pairs path
+ * together (else one by one). Every state's result is bitwise that of
+ * spx_prop_evolve on it alone; last_order reports the longest expansion. */
+int spx_prop_evolve_set(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* states,
+                        size_t count, double tau, size_t nsteps, int autonormalize);
+/* spx_prop_evolve_set over prebuilt expansions (spx_prop_steps for a set):
+ * state i takes coeffs[i][0..orders[i]] at spectral bound bounds[i]. */
+int spx_prop_steps_set(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* states,
+                       size_t count, const double* const* coeffs, const size_t* orders,
+                       const double* bounds, size_t nsteps, int autonormalize);
 int spx_prop_last_order(const spx_prop* p, size_t* order);                      /* lastOrder */
 /* No reference counterpart: with enable = 1, evolve calls of at least 4
  * steps without autonormalisation record one step as a HIP graph and replay

@@ -270,6 +270,8 @@ int spx_ham_destroy(spx_ham* h) {
   if (h->d_tab) hipFree(h->d_tab);
   if (h->d_scratch) hipFree(h->d_scratch);
   if (h->d_coef) hipFree(h->d_coef);
+  if (h->d_wht_e) hipFree(h->d_wht_e);
+  if (h->d_wht) hipFree(h->d_wht);
   delete h;
   return SPX_OK;
 }
@@ -359,6 +361,8 @@ int spx_ham_schedule(const spx_ham* h, char* buf, size_t len) {
   if (!h || !buf || len == 0) return fail(SPX_EINVAL, "null argument");
   std::string name;
   if (!h->terms_valid) name = "unprepared";
+  else if (h->wht_ok)
+    name = "k_wht_rows_fwd + k_wht_cols<" + std::to_string(h->nspins - 10) + "> + k_wht_rows_bwd<K_ORDERN>";
   else if (h->dense_ok)
     name = "k_dense<K_ORDERN, " + std::to_string(h->nspins) + ", " + std::to_string(dense_tile_bits(h)) + ">";
   else if (h->terms_variant == 3)
@@ -514,7 +518,7 @@ int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, siz
   if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
   SPX_TRY(set_device(p->ctx));
   SPX_TRY(ensure_scratch(p, nstates < kMaxMany ? nstates : kMaxMany));
-  SPX_TRY(autotune_before_evolve(h, bound));
+  SPX_TRY(autotune_before_evolve(h, bound, static_cast<int>(nstates < kMaxMany ? nstates : kMaxMany)));
   SPX_TRY(ham_prepare(h, -2.0 / bound));
   // Without autonormalisation every step is the same sequence of launches
   // (same expansion, same buffers): record it once as a HIP graph and replay
@@ -552,6 +556,75 @@ int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, siz
   return SPX_OK;
 }
 
+// One step of a set of states, state i under hs[i] with its own expansion
+// *a[i] (order a[i]->size() - 1 >= 1), sorted longest expansion first so that
+// the states still expanding at order n are a prefix of the launch. Every
+// state sees exactly the arithmetic of one_step on its own.
+int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* states, size_t nvec,
+                 const std::vector<double>* const* a) {
+  double2* A[kMaxMany];
+  double2* B[kMaxMany];
+  for (size_t i = 0; i < nvec; ++i) {
+    A[i] = p->d_scratch[2 * i];
+    B[i] = p->d_scratch[2 * i + 1];
+  }
+  double wre = mode == 0 ? 0.0 : -1.0, wim = mode == 0 ? -1.0 : 0.0;
+  auto advance = [&] {
+    if (mode == 0) {
+      const double pr = wre;
+      wre = wim;
+      wim = -pr;
+    } else {
+      wre = -wre;
+      wim = -wim;
+    }
+  };
+  auto order_of = [&](size_t i) { return a[i]->size() - 1; };
+  OrderLaunch v{};
+  OrderWeights w[kMaxMany] = {};
+  v.nvec = static_cast<int>(nvec);
+  for (size_t i = 0; i < nvec; ++i) {
+    v.in[i] = states[i]->d;
+    v.out[i] = B[i];
+    v.psi[i] = states[i]->d;
+  }
+  SPX_TRY(launch_order_set(hs, 1, v, w));
+  double c1r[kMaxMany], c1i[kMaxMany];
+  for (size_t i = 0; i < nvec; ++i) {
+    c1r[i] = 2.0 * (*a[i])[1] * wre;
+    c1i[i] = 2.0 * (*a[i])[1] * wim;
+  }
+  advance();
+  size_t act = nvec;
+  while (act > 0 && order_of(act - 1) == 1) {
+    --act;
+    SPX_TRY(launch_axpby(p->ctx, states[act]->d, (*a[act])[0], B[act], c1r[act], c1i[act], states[act]->n));
+  }
+  if (act == 0) return SPX_OK;
+  v.nvec = static_cast<int>(act);
+  for (size_t i = 0; i < act; ++i) {
+    v.in[i] = B[i];
+    v.out[i] = A[i];
+    w[i] = OrderWeights{(*a[i])[0], c1r[i], c1i[i], 2.0 * (*a[i])[2] * wre, 2.0 * (*a[i])[2] * wim};
+  }
+  SPX_TRY(launch_order_set(hs, 2, v, w));
+  advance();
+  const size_t top = order_of(0);
+  for (size_t n = 3; n <= top; ++n) {
+    while (act > 0 && order_of(act - 1) < n) --act;
+    v.nvec = static_cast<int>(act);
+    for (size_t i = 0; i < act; ++i) {
+      const bool odd = (n % 2) == 1;
+      v.in[i] = odd ? A[i] : B[i];
+      v.out[i] = odd ? B[i] : A[i];
+      w[i] = OrderWeights{0.0, 0.0, 0.0, 2.0 * (*a[i])[n] * wre, 2.0 * (*a[i])[n] * wim};
+    }
+    SPX_TRY(launch_order_set(hs, 3, v, w));
+    advance();
+  }
+  return SPX_OK;
+}
+
 }  // namespace
 
 int spx_prop_evolve(spx_prop* p, spx_ham* h, int mode, spx_state* psi, double tau, size_t nsteps,
@@ -563,6 +636,115 @@ int spx_prop_evolve(spx_prop* p, spx_ham* h, int mode, spx_state* psi, double ta
 int spx_prop_evolve_many(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t nstates,
                          double tau, size_t nsteps, int autonormalize) {
   return evolve_impl(p, h, mode, states, nstates, nullptr, 0.0, tau, nsteps, autonormalize);
+}
+
+namespace {
+
+// The set evolve over given expansions (coeffs[i], bounds[i] per state).
+int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* states, size_t count,
+                    const std::vector<std::vector<double>>& coeffs, const std::vector<double>& bounds,
+                    size_t nsteps, int autonormalize) {
+  size_t top = 0;
+  for (size_t i = 0; i < count; ++i) top = std::max(top, coeffs[i].size() - 1);
+  p->last_order = top;
+  SPX_TRY(set_device(p->ctx));
+  SPX_TRY(ensure_scratch(p, count < kMaxMany ? count : kMaxMany));
+  // longest expansion first (ties by position), identities (order 0) dropped
+  std::vector<size_t> idx;
+  for (size_t i = 0; i < count; ++i)
+    if (coeffs[i].size() > 1) {
+      if (!(bounds[i] > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
+      idx.push_back(i);
+    }
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](size_t x, size_t y) { return coeffs[x].size() > coeffs[y].size(); });
+  const int width = static_cast<int>(idx.size() < static_cast<size_t>(kMaxMany) ? idx.size() : kMaxMany);
+  for (size_t i : idx) {
+    SPX_TRY(autotune_before_evolve(hams[i], bounds[i], width));
+    SPX_TRY(ham_prepare(hams[i], -2.0 / bounds[i]));
+  }
+  // groups of up to kMaxMany consecutive states whose Hamiltonians can share launches
+  std::vector<std::pair<size_t, size_t>> groups;
+  for (size_t g = 0; g < idx.size();) {
+    size_t e = g + 1;
+    while (e < idx.size() && e - g < static_cast<size_t>(kMaxMany)) {
+      spx_ham* hs[kMaxMany];
+      for (size_t k = g; k <= e; ++k) hs[k - g] = hams[idx[k]];
+      if (!order_set_compatible(hs, e - g + 1)) break;
+      ++e;
+    }
+    groups.emplace_back(g, e);
+    g = e;
+  }
+  for (size_t step = 0; step < nsteps; ++step) {
+    for (const auto& [g, e] : groups) {
+      spx_ham* hs[kMaxMany];
+      spx_state* ss[kMaxMany];
+      const std::vector<double>* cs[kMaxMany];
+      for (size_t k = g; k < e; ++k) {
+        hs[k - g] = hams[idx[k]];
+        ss[k - g] = states[idx[k]];
+        cs[k - g] = &coeffs[idx[k]];
+      }
+      SPX_TRY(one_step_set(p, hs, mode, ss, e - g, cs));
+    }
+    if (autonormalize)
+      for (size_t i = 0; i < count; ++i) SPX_TRY(spx_state_normalize(states[i], nullptr));
+  }
+  return SPX_OK;
+}
+
+int check_set(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* states, size_t count) {
+  if (!p || !hams || !states) return fail(SPX_EINVAL, "null argument");
+  for (size_t i = 0; i < count; ++i) {
+    if (!hams[i]) return fail(SPX_EINVAL, "null Hamiltonian");
+    SPX_TRY(check_state(states[i]));
+    if (states[i]->nspins != hams[i]->nspins) return fail(SPX_EINVAL, "state and Hamiltonian disagree on spin count");
+    if (states[i]->nspins != p->nspins) return fail(SPX_EINVAL, "state does not match the propagator's size");
+    if (hams[i]->ctx != p->ctx) return fail(SPX_EINVAL, "Hamiltonian belongs to another context");
+    for (size_t j = 0; j < i; ++j)
+      if (states[j] == states[i]) return fail(SPX_EINVAL, "a state appears twice");
+  }
+  if (mode != 0 && mode != 1) return fail(SPX_EINVAL, "unknown time mode");
+  return SPX_OK;
+}
+
+}  // namespace
+
+// LocalRunner::run (src/ensemble/EnsembleRunner.cpp:27-88) evolves each
+// realisation under its own Hamiltonian on a pool of threads; here up to
+// kMaxMany of them share every order launch (the Hadamard and all-pairs
+// paths take per-state Hamiltonians), each with its own spectral bound and
+// expansion order. Results are bitwise those of spx_prop_evolve per state.
+int spx_prop_evolve_set(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* states, size_t count,
+                        double tau, size_t nsteps, int autonormalize) {
+  SPX_TRY(check_set(p, hams, mode, states, count));
+  if (count == 0 || nsteps == 0) return SPX_OK;
+  std::vector<std::vector<double>> coeffs(count);
+  std::vector<double> bounds(count);
+  for (size_t i = 0; i < count; ++i) {
+    double sb = 0.0;
+    spx_ham_spectral_bound(hams[i], &sb);
+    SPX_TRY(cheb_build(mode, sb, tau, p->epsilon, 6000, coeffs[i], bounds[i]));
+  }
+  return evolve_set_impl(p, hams, mode, states, count, coeffs, bounds, nsteps, autonormalize);
+}
+
+// The same over prebuilt expansions (spx_prop_steps for a set): state i takes
+// coeffs[i][0..orders[i]] at bound bounds[i].
+int spx_prop_steps_set(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* states, size_t count,
+                       const double* const* coeffs, const size_t* orders, const double* bounds, size_t nsteps,
+                       int autonormalize) {
+  SPX_TRY(check_set(p, hams, mode, states, count));
+  if (count == 0 || nsteps == 0) return SPX_OK;
+  if (!coeffs || !orders || !bounds) return fail(SPX_EINVAL, "null expansion");
+  std::vector<std::vector<double>> c(count);
+  std::vector<double> b(bounds, bounds + count);
+  for (size_t i = 0; i < count; ++i) {
+    if (!coeffs[i]) return fail(SPX_EINVAL, "null expansion");
+    c[i].assign(coeffs[i], coeffs[i] + orders[i] + 1);
+  }
+  return evolve_set_impl(p, hams, mode, states, count, c, b, nsteps, autonormalize);
 }
 
 int spx_prop_steps(spx_prop* p, spx_ham* h, int mode, spx_state* psi, const double* coeffs,

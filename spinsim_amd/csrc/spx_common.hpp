@@ -62,7 +62,7 @@ struct Schedule {
   int block = 0;     // threads per workgroup of the per-amplitude-group kernel
   int xcd = -1;      // XCD-aware tile order
   int dense = -1;    // all-pairs kernel allowed
-  int dense_tb = 0;  // its tile bits (9 or 10)
+  int dense_tb = 0;  // its tile bits (9 or 10); dense = 2 selects the Hadamard path
   int table = 0;     // 0: best eligible, 2: force the per-amplitude-group kernel
 };
 
@@ -108,6 +108,9 @@ struct spx_ham {
   int terms_flipboth = 0;
   bool dense_ok = false;         // specialised all-pairs kernel usable
   double* d_coef = nullptr;      // its weights
+  bool wht_ok = false;           // Hadamard-sandwich flip-flop path selected
+  double* d_wht_e = nullptr;     // its Hadamard-basis diagonal E(k), 2^L
+  double2* d_wht = nullptr;      // its transform scratch, 2 * kMaxMany vectors of 2^L
   double terms_scale = 0.0;
   bool terms_valid = false;
   std::vector<spx::FlipTerm> h_terms;
@@ -143,11 +146,40 @@ struct OrderLaunch {
 };
 
 int ham_prepare(spx_ham* h, double scale);   // diagonal + flip table at scale
+
+// Flip-flop-only Hamiltonians through Walsh-Hadamard transforms
+// (spx_hadamard.hip): three passes per application, any L in range.
+constexpr size_t kHadamardMinSpins = 14, kHadamardMaxSpins = 20;
+// Chebyshev weights of one vector of an order launch (see launch_order).
+struct OrderWeights {
+  double a0, c1r, c1i, cr, ci;
+};
+// Vector i of a launch is propagated under ham[i] (prepared at its own
+// scale sdiag[i]) and uses scratch pair slot[i] of that Hamiltonian.
+struct HadamardLaunch {
+  const double2* in[kMaxMany];
+  double2* out[kMaxMany];
+  double2* psi[kMaxMany];
+  const spx_ham* ham[kMaxMany];
+  int slot[kMaxMany];
+  double sdiag[kMaxMany];
+  OrderWeights w[kMaxMany];
+};
+bool hadamard_eligible(size_t L);
+// coef: the L(L-1)/2 flip-flop weights at the apply scale, pair order (0,1),(0,2),...
+int hadamard_prepare(spx_ham* h, const std::vector<double>& coef);
+int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec);
 int launch_apply(spx_ham* h, double scale, const double2* in, double2* out, bool accumulate);
 // kind: 1 = first order (out = 0.5 acc), 2 = second order (fused psi
 // a0 T0 + c1 T1 + c2 T2), 3 = general order (T = acc - out; psi += c T)
 int launch_order(spx_ham* h, int kind, const OrderLaunch& v, double a0, double c1r, double c1i,
                  double cr, double ci);
+// The same over a set of states, vector i under hs[i] (prepared at its own
+// scale) with weights w[i]. The Hadamard and all-pairs paths take distinct
+// Hamiltonians in one launch; order_set_compatible says whether these
+// prepared Hamiltonians can share launches (else they are stepped one by one).
+bool order_set_compatible(spx_ham* const* hs, size_t n);
+int launch_order_set(spx_ham* const* hs, int kind, const OrderLaunch& v, const OrderWeights* w);
 int launch_axpby(spx_ctx* c, double2* y, double a, const double2* x, double br, double bi, size_t n);
 int launch_fill(spx_ctx* c, double2* y, size_t n);
 int launch_scale(spx_ctx* c, double2* y, double f, size_t n);
@@ -175,6 +207,6 @@ int block_threads(const spx_ham* h);
 int xcd_remap(const spx_ham* h);
 int dense_tile_bits(const spx_ham* h);
 // the autotuner's process-wide mode and its hook in the propagator
-int autotune_before_evolve(spx_ham* h, double bound);
+int autotune_before_evolve(spx_ham* h, double bound, int nvec);
 
 }  // namespace spx

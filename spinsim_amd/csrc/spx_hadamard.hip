@@ -1,0 +1,381 @@
+// Flip-flop couplings through Walsh-Hadamard transforms (gfx950, FP64).
+//
+// For a Hamiltonian whose only off-diagonal terms are flip-flop pairs
+// (weights w_q = scale (jx + jy) / 4, no transverse field, no flip-both
+// weight), the pair sum the reference applies term by term
+// (src/core/Hamiltonian.cpp:95-127) is
+//
+//   H_ff = sum_q w_q (s+_i s-_j + s-_i s+_j) = (H_XX + P H_XX P) / 2,
+//   H_XX = sum_q w_q X_i X_j = W D W / 2^L,
+//
+// with W the unnormalised Walsh-Hadamard transform on all L bits, D(s) =
+// sum_q w_q z_i(s) z_j(s) (z = +1 for bit 0, -1 for bit 1) and P the
+// diagonal sign (-1)^floor(popcount / 2): Y_i Y_j = P X_i X_j P on the
+// popcount-parity sectors that X_i X_j preserves, and (X X + Y Y) / 2 is the
+// flip-flop term. With E = D / 2^(L+1) (a power-of-two scale, exact):
+//
+//   H_ff psi = W E W psi + P (W E W (P psi)).
+//
+// This costs O(L) butterflies per amplitude instead of one partner gather
+// per coupled pair (L(L-1)/2 of them for all-to-all dipolar couplings), and
+// every butterfly is unmasked. The 2^L index is split into a row part (the
+// low TB = 10 bits, a contiguous tile) and a column part (the high G = L - TB
+// bits); W = W_high W_low, so one Chebyshev order is three passes:
+//   1. rows:    U = W_low psi, V = W_low (P psi)              per tile, in LDS
+//   2. columns: X = W_high E W_high X for X in {U, V}         8 columns per workgroup
+//   3. rows:    H psi = sdiag diag psi + W_low U + P W_low V, fused with the
+//               same Chebyshev / accumulate epilogues as the gather kernels.
+// The transforms are exact sums and differences; results agree with the
+// gather kernels to rounding (a few ulps of the amplitude scale).
+#include <cmath>
+#include <utility>
+
+#include "spx_common.hpp"
+
+namespace spx {
+namespace {
+
+constexpr int kTB = 10;                // row (tile) bits
+constexpr int kCols = 8;               // columns per workgroup of the column pass
+constexpr int kRowThreads = 1 << (kTB - 2);   // two vectors x 2^(TB-3) threads
+
+// Everything per vector (blockIdx.y): a launch may carry the states of
+// several Hamiltonians of one size.
+struct WhtArgs {
+  const double2* in[kMaxMany];
+  double2* out[kMaxMany];
+  double2* psi[kMaxMany];
+  double2* u[kMaxMany];         // scratch: W E W psi
+  double2* w[kMaxMany];         // scratch: W E W P psi
+  const double* diag[kMaxMany];
+  const double* e[kMaxMany];    // E(k), length 2^L
+  double sdiag[kMaxMany];
+  double a0[kMaxMany], c1r[kMaxMany], c1i[kMaxMany], cr[kMaxMany], ci[kMaxMany];
+};
+
+enum { K_APPLY = 0, K_ACCUM = 1, K_ORDER1 = 2, K_ORDER2 = 3, K_ORDERN = 4 };
+
+// (-1)^floor(popcount(k) / 2)
+__device__ __forceinline__ double psign(uint32_t k) { return (__popc(k) & 2u) ? -1.0 : 1.0; }
+
+// LDS slot of element i of a row tile: bits 0-2 XOR bits 3-5, so that in
+// every round eight consecutive lanes touch eight distinct 16-byte slots of
+// one 128-byte bank line.
+__device__ __forceinline__ uint32_t rslot(uint32_t i) { return i ^ ((i >> 3) & 7u); }
+
+struct RowSlot {
+  __device__ __forceinline__ uint32_t operator()(uint32_t i, uint32_t) const { return rslot(i); }
+};
+struct ColSlot {
+  __device__ __forceinline__ uint32_t operator()(uint32_t i, uint32_t c) const { return (i << 3) | c; }
+};
+
+// In-place unnormalised Walsh-Hadamard transform over NB index bits of
+// values in LDS, three bits per round (eight values per thread). Thread t
+// of a group of 2^(CB + NB - 3) threads owns batch column c = t mod 2^CB and
+// the NB - 3 index bits t >> CB; round b0 takes index bits [lo, lo + 3) into
+// registers (lo = b0, or NB - 3 for a short last round, whose lower bits are
+// already done). Every round ends with a workgroup barrier, so all threads
+// of the workgroup call this the same number of times.
+template <int NB, int CB, typename Slot>
+__device__ __forceinline__ void wht_rounds(double2* __restrict__ S, uint32_t t, Slot slot) {
+  static_assert(NB >= 3, "at least three transform bits");
+  const uint32_t c = t & ((1u << CB) - 1u);
+  const uint32_t u = t >> CB;
+#pragma unroll
+  for (int b0 = 0; b0 < NB; b0 += 3) {
+    const int lo = b0 + 3 <= NB ? b0 : NB - 3;
+    const uint32_t base = ((u >> lo) << (lo + 3)) | (u & ((1u << lo) - 1u));
+    double2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = S[slot(base | (static_cast<uint32_t>(r) << lo), c)];
+#pragma unroll
+    for (int s = b0 - lo; s < 3; ++s) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        if (r & (1 << s)) continue;
+        const double2 a = v[r], b = v[r | (1 << s)];
+        v[r] = make_double2(a.x + b.x, a.y + b.y);
+        v[r | (1 << s)] = make_double2(a.x - b.x, a.y - b.y);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) S[slot(base | (static_cast<uint32_t>(r) << lo), c)] = v[r];
+    __syncthreads();
+  }
+}
+
+// Global loads of a pass are all issued into registers before the first
+// dependent LDS write or global store: the compiler cannot prove that the
+// state pointers do not alias, and would otherwise serialise the memory
+// round trips (load, wait, store, next load).
+
+// Pass 1: one workgroup per row tile x (2^TB consecutive amplitudes) and
+// vector; threads [0, 128) transform psi, [128, 256) transform P psi.
+__global__ __launch_bounds__(kRowThreads) void k_wht_rows_fwd(WhtArgs a) {
+  constexpr uint32_t T = 1u << kTB, NT = kRowThreads, H = NT / 2, M = T / NT;
+  extern __shared__ double2 smem[];
+  const uint32_t t = threadIdx.x;
+  const int v = blockIdx.y;
+  const size_t base = static_cast<size_t>(blockIdx.x) << kTB;
+  const double2* __restrict__ in = a.in[v];
+  double2 x[M];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) x[m] = in[base + t + m * NT];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint32_t i = t + m * NT;
+    const double p = psign(static_cast<uint32_t>(base + i));
+    smem[rslot(i)] = x[m];
+    smem[T + rslot(i)] = make_double2(p * x[m].x, p * x[m].y);
+  }
+  __syncthreads();
+  const uint32_t half = t >= H ? 1u : 0u;
+  wht_rounds<kTB, 0>(smem + half * T, t - half * H, RowSlot{});
+  double2 yu[M], yv[M];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    yu[m] = smem[rslot(t + m * NT)];
+    yv[m] = smem[T + rslot(t + m * NT)];
+  }
+  double2* __restrict__ U = a.u[v];
+  double2* __restrict__ V = a.w[v];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    U[base + t + m * NT] = yu[m];
+    V[base + t + m * NT] = yv[m];
+  }
+}
+
+// Pass 2: one workgroup per group of 8 columns (low indices) and per scratch
+// vector; 2^G threads hold 8 columns x 2^G rows in LDS, transform the rows,
+// multiply by E and transform again. Each row segment is 128 contiguous bytes.
+template <int G>
+__global__ __launch_bounds__(1 << G) void k_wht_cols(WhtArgs a) {
+  constexpr uint32_t NT = 1u << G, NV = kCols << G, M = NV / NT;
+  extern __shared__ double2 smem[];
+  const uint32_t t = threadIdx.x;
+  const int v = static_cast<int>(blockIdx.y >> 1);
+  double2* __restrict__ X = (blockIdx.y & 1u) ? a.w[v] : a.u[v];
+  const uint32_t col0 = blockIdx.x * kCols;
+  const double* __restrict__ e = a.e[v];
+  double2 x[M];
+  double s[M];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint32_t f = t + m * NT;
+    x[m] = X[(static_cast<size_t>(f >> 3) << kTB) | (col0 + (f & 7u))];
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint32_t f = t + m * NT;
+    s[m] = e[(static_cast<size_t>(f >> 3) << kTB) | (col0 + (f & 7u))];
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) smem[t + m * NT] = x[m];
+  __syncthreads();
+  wht_rounds<G, 3>(smem, t, ColSlot{});
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const double2 y = smem[t + m * NT];
+    smem[t + m * NT] = make_double2(s[m] * y.x, s[m] * y.y);
+  }
+  __syncthreads();
+  wht_rounds<G, 3>(smem, t, ColSlot{});
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) x[m] = smem[t + m * NT];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint32_t f = t + m * NT;
+    X[(static_cast<size_t>(f >> 3) << kTB) | (col0 + (f & 7u))] = x[m];
+  }
+}
+
+// Pass 3: the row transforms back, the diagonal term and the epilogue of
+// the gather kernels (apply / accumulate / first, second and general
+// Chebyshev order, src/core/Propagator.cpp:58-72). The epilogue's operands
+// are loaded before the transform so that their latency overlaps it.
+template <int KIND>
+__global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
+  constexpr uint32_t T = 1u << kTB, NT = kRowThreads, H = NT / 2, M = T / NT;
+  extern __shared__ double2 smem[];
+  const uint32_t t = threadIdx.x;
+  const int v = blockIdx.y;
+  const size_t base = static_cast<size_t>(blockIdx.x) << kTB;
+  const double2* __restrict__ U = a.u[v];
+  const double2* __restrict__ V = a.w[v];
+  const double2* __restrict__ in = a.in[v];
+  double2* __restrict__ out = a.out[v];
+  double2* __restrict__ psi = a.psi[v];
+  const double* __restrict__ diag = a.diag[v];
+  const double sdiag = a.sdiag[v], a0 = a.a0[v], c1r = a.c1r[v], c1i = a.c1i[v], cr = a.cr[v], ci = a.ci[v];
+  double2 xu[M], xw[M], xv[M], prev[M], pv[M];
+  double dg[M];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    xu[m] = U[base + t + m * NT];
+    xw[m] = V[base + t + m * NT];
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const size_t k = base + t + m * NT;
+    xv[m] = in[k];
+    dg[m] = diag[k];
+    if constexpr (KIND == K_ACCUM || KIND == K_ORDERN) prev[m] = out[k];
+    if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN) pv[m] = psi[k];
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    smem[rslot(t + m * NT)] = xu[m];
+    smem[T + rslot(t + m * NT)] = xw[m];
+  }
+  __syncthreads();
+  const uint32_t half = t >= H ? 1u : 0u;
+  wht_rounds<kTB, 0>(smem + half * T, t - half * H, RowSlot{});
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    xu[m] = smem[rslot(t + m * NT)];
+    xw[m] = smem[T + rslot(t + m * NT)];
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const size_t k = base + t + m * NT;
+    const double p = psign(static_cast<uint32_t>(k));
+    const double f = sdiag * dg[m];
+    double2 acc;
+    acc.x = fma(f, xv[m].x, xu[m].x + p * xw[m].x);
+    acc.y = fma(f, xv[m].y, xu[m].y + p * xw[m].y);
+    if constexpr (KIND == K_APPLY) {
+      out[k] = acc;
+    } else if constexpr (KIND == K_ACCUM) {
+      out[k] = make_double2(prev[m].x + acc.x, prev[m].y + acc.y);
+    } else if constexpr (KIND == K_ORDER1) {
+      out[k] = make_double2(0.5 * acc.x, 0.5 * acc.y);
+    } else if constexpr (KIND == K_ORDER2) {
+      const double2 t0 = pv[m], t1 = xv[m];
+      const double2 t2 = make_double2(acc.x - t0.x, acc.y - t0.y);
+      out[k] = t2;
+      double2 q = make_double2(a0 * t0.x, a0 * t0.y);
+      q.x += c1r * t1.x - c1i * t1.y;
+      q.y += c1r * t1.y + c1i * t1.x;
+      q.x += cr * t2.x - ci * t2.y;
+      q.y += cr * t2.y + ci * t2.x;
+      psi[k] = q;
+    } else {
+      const double2 tn = make_double2(acc.x - prev[m].x, acc.y - prev[m].y);
+      out[k] = tn;
+      double2 q = pv[m];
+      q.x += cr * tn.x - ci * tn.y;
+      q.y += cr * tn.y + ci * tn.x;
+      psi[k] = q;
+    }
+  }
+}
+
+// E(k) = sum_q c_q z_i(k) z_j(k), c_q already divided by 2^(L+1); the
+// table holds (mask, c) pairs with the mask as a double.
+__global__ __launch_bounds__(256) void k_wht_energy(double* __restrict__ e, size_t n,
+                                                    const double* __restrict__ tab, int np) {
+  const size_t k = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (k >= n) return;
+  double s = 0.0;
+  for (int i = 0; i < np; ++i) {
+    const uint32_t m = static_cast<uint32_t>(tab[2 * i]);
+    const double c = tab[2 * i + 1];
+    s += (__popc(static_cast<uint32_t>(k) & m) != 1) ? c : -c;
+  }
+  e[k] = s;
+}
+
+template <int G>
+void launch_cols(hipStream_t st, const WhtArgs& a, unsigned nvec) {
+  const dim3 grid((1u << kTB) / kCols, 2 * nvec);
+  k_wht_cols<G><<<grid, 1u << G, sizeof(double2) * (kCols << G), st>>>(a);
+}
+
+template <int KIND>
+void launch_bwd(hipStream_t st, const WhtArgs& a, dim3 grid) {
+  k_wht_rows_bwd<KIND><<<grid, kRowThreads, sizeof(double2) * (2u << kTB), st>>>(a);
+}
+
+}  // namespace
+
+bool hadamard_eligible(size_t L) { return L >= kHadamardMinSpins && L <= kHadamardMaxSpins; }
+
+int hadamard_prepare(spx_ham* h, const std::vector<double>& coef) {
+  const size_t L = h->nspins, n = static_cast<size_t>(1) << L;
+  if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard path needs 14 <= L <= 20");
+  std::vector<double> tab;
+  size_t q = 0;
+  for (size_t lo = 0; lo < L; ++lo)
+    for (size_t hi = lo + 1; hi < L; ++hi, ++q) {
+      if (coef[q] == 0.0) continue;
+      tab.push_back(static_cast<double>((1u << lo) | (1u << hi)));
+      tab.push_back(std::ldexp(coef[q], -static_cast<int>(L + 1)));
+    }
+  spx_ctx* c = h->ctx;
+  if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, n * sizeof(double)));
+  if (!h->d_wht) SPX_HIP(hipMalloc(&h->d_wht, 2 * kMaxMany * n * sizeof(double2)));
+  const size_t tbytes = (tab.size() + 2) * sizeof(double);
+  if (tbytes > h->tab_cap) {
+    if (h->d_tab) SPX_HIP(hipFree(h->d_tab));
+    SPX_HIP(hipMalloc(&h->d_tab, tbytes));
+    h->tab_cap = tbytes;
+  }
+  if (!tab.empty())
+    SPX_HIP(hipMemcpyAsync(h->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_wht_energy<<<static_cast<unsigned>((n + 255) / 256), 256, 0, c->stream>>>(h->d_wht_e, n, h->d_tab,
+                                                                               static_cast<int>(tab.size() / 2));
+  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipStreamSynchronize(c->stream));  // `tab` is pageable and local
+  return SPX_OK;
+}
+
+int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
+  if (nvec < 1 || nvec > static_cast<unsigned>(kMaxMany)) return fail(SPX_EINVAL, "Hadamard launch out of range");
+  const spx_ham* h0 = b.ham[0];
+  const size_t L = h0->nspins, n = static_cast<size_t>(1) << L;
+  if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard launch out of range");
+  WhtArgs a{};
+  for (unsigned i = 0; i < nvec; ++i) {
+    const spx_ham* h = b.ham[i];
+    if (!h || h->nspins != L || !h->wht_ok || !h->d_wht || b.slot[i] < 0 || b.slot[i] >= kMaxMany)
+      return fail(SPX_EINVAL, "Hadamard launch mixes sizes or unprepared Hamiltonians");
+    a.in[i] = b.in[i];
+    a.out[i] = b.out[i];
+    a.psi[i] = b.psi[i];
+    a.u[i] = h->d_wht + (2 * static_cast<size_t>(b.slot[i])) * n;
+    a.w[i] = h->d_wht + (2 * static_cast<size_t>(b.slot[i]) + 1) * n;
+    a.diag[i] = h->d_diag;
+    a.e[i] = h->d_wht_e;
+    a.sdiag[i] = b.sdiag[i];
+    a.a0[i] = b.w[i].a0;
+    a.c1r[i] = b.w[i].c1r;
+    a.c1i[i] = b.w[i].c1i;
+    a.cr[i] = b.w[i].cr;
+    a.ci[i] = b.w[i].ci;
+  }
+  hipStream_t st = h0->ctx->stream;
+  const dim3 rows(static_cast<unsigned>(n >> kTB), nvec);
+  k_wht_rows_fwd<<<rows, kRowThreads, sizeof(double2) * (2u << kTB), st>>>(a);
+  switch (L - kTB) {
+    case 4: launch_cols<4>(st, a, nvec); break;
+    case 5: launch_cols<5>(st, a, nvec); break;
+    case 6: launch_cols<6>(st, a, nvec); break;
+    case 7: launch_cols<7>(st, a, nvec); break;
+    case 8: launch_cols<8>(st, a, nvec); break;
+    case 9: launch_cols<9>(st, a, nvec); break;
+    default: launch_cols<10>(st, a, nvec); break;
+  }
+  switch (kind) {
+    case K_APPLY: launch_bwd<K_APPLY>(st, a, rows); break;
+    case K_ACCUM: launch_bwd<K_ACCUM>(st, a, rows); break;
+    case K_ORDER1: launch_bwd<K_ORDER1>(st, a, rows); break;
+    case K_ORDER2: launch_bwd<K_ORDER2>(st, a, rows); break;
+    default: launch_bwd<K_ORDERN>(st, a, rows); break;
+  }
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+}  // namespace spx

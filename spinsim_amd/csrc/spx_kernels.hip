@@ -561,15 +561,17 @@ __device__ __forceinline__ void dense_all(std::integer_sequence<int, CH...>, std
   (dense_tail<L, TB, R>(d, coef, acc, slot), ...);
 }
 
+// Per vector (blockIdx.y): a launch may carry the states of several
+// Hamiltonians of one size.
 struct KDArgs {
   const double2* in[kMaxMany];
   double2* out[kMaxMany];
   double2* psi[kMaxMany];
-  const double* diag;
-  const double* coef;   // [L(L-1)/2] flip-flop weights, pair order (0,1),(0,2),...
+  const double* diag[kMaxMany];
+  const double* coef[kMaxMany];   // [L(L-1)/2] flip-flop weights, pair order (0,1),(0,2),...
   int xcd_remap;
-  double sdiag;
-  double a0, c1r, c1i, cr, ci;
+  double sdiag[kMaxMany];
+  double a0[kMaxMany], c1r[kMaxMany], c1i[kMaxMany], cr[kMaxMany], ci[kMaxMany];
 };
 
 template <int KIND, int L, int TB>
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(1 << TB) void k_dense(KDArgs a) {
   double2 nbv[G > 0 ? G : 1];
 #pragma unroll
   for (int r = 0; r < G; ++r) nbv[r] = in[(static_cast<size_t>(x ^ (1u << r)) << TB) + j];
-  const double dg = a.diag[k];
+  const double dg = a.diag[v][k];
   double2 prev = make_double2(0.0, 0.0), psiv = make_double2(0.0, 0.0);
   if constexpr (KIND == K_ACCUM || KIND == K_ORDERN) prev = a.out[v][k];
   if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN) psiv = a.psi[v][k];
@@ -599,11 +601,11 @@ __global__ __launch_bounds__(1 << TB) void k_dense(KDArgs a) {
   for (int r = 0; r < G; ++r) smem[(1 + r) * T + j] = nbv[r];
   __syncthreads();
 
-  const double f = a.sdiag * dg;
+  const double f = a.sdiag[v] * dg;
   double2 acc = make_double2(f * xv.x, f * xv.y);
   const DenseLane d{j, wj, x, smem, in};
   dense_all<L, TB>(std::make_integer_sequence<int, DenseShape<L, TB>::NCL>{},
-                   std::make_integer_sequence<int, DenseShape<L, TB>::NCR>{}, d, a.coef, acc);
+                   std::make_integer_sequence<int, DenseShape<L, TB>::NCR>{}, d, a.coef[v], acc);
 
   if constexpr (KIND == K_APPLY) {
     a.out[v][k] = acc;
@@ -615,17 +617,17 @@ __global__ __launch_bounds__(1 << TB) void k_dense(KDArgs a) {
     const double2 t0 = psiv, t1 = xv;
     const double2 t2 = make_double2(acc.x - t0.x, acc.y - t0.y);
     a.out[v][k] = t2;
-    double2 qv = make_double2(a.a0 * t0.x, a.a0 * t0.y);
-    qv.x += a.c1r * t1.x - a.c1i * t1.y;
-    qv.y += a.c1r * t1.y + a.c1i * t1.x;
-    qv.x += a.cr * t2.x - a.ci * t2.y;
-    qv.y += a.cr * t2.y + a.ci * t2.x;
+    double2 qv = make_double2(a.a0[v] * t0.x, a.a0[v] * t0.y);
+    qv.x += a.c1r[v] * t1.x - a.c1i[v] * t1.y;
+    qv.y += a.c1r[v] * t1.y + a.c1i[v] * t1.x;
+    qv.x += a.cr[v] * t2.x - a.ci[v] * t2.y;
+    qv.y += a.cr[v] * t2.y + a.ci[v] * t2.x;
     a.psi[v][k] = qv;
   } else {
     const double2 tn = make_double2(acc.x - prev.x, acc.y - prev.y);
     a.out[v][k] = tn;
-    psiv.x += a.cr * tn.x - a.ci * tn.y;
-    psiv.y += a.cr * tn.y + a.ci * tn.x;
+    psiv.x += a.cr[v] * tn.x - a.ci[v] * tn.y;
+    psiv.y += a.cr[v] * tn.y + a.ci[v] * tn.x;
     a.psi[v][k] = psiv;
   }
 }
@@ -989,8 +991,10 @@ int xcd_remap(const spx_ham* h) {
 }
 
 // Dense-pair eligibility and its weight table (pair order (0,1),(0,2),...):
-// L = 14..18, no transverse field and no flip-both weight at this scale.
-// SPX_DENSE=0 (or a schedule with dense = 0) switches the kernel off.
+// no transverse field and no flip-both weight at this scale; L = 14..18 for
+// the all-pairs gather kernel (dense = 1), L = 14..20 for the Hadamard
+// sandwich (dense = 2). SPX_DENSE=0 (or a schedule with dense = 0) switches
+// both off.
 int dense_tile_bits(const spx_ham* h) {
   static const int env = env_int("SPX_DENSE_TB", 10);
   const int tb = h->sched.dense_tb > 0 ? h->sched.dense_tb : env;
@@ -1002,7 +1006,10 @@ static int prepare_dense(spx_ham* h, double scale) {
   const int enabled = h->sched.dense >= 0 ? h->sched.dense : env;
   const size_t L = h->nspins;
   h->dense_ok = false;
-  if (!enabled || L < 14 || L > 18) return SPX_OK;
+  h->wht_ok = false;
+  const bool pairs = enabled == 1 && L >= 14 && L <= 18;
+  const bool hadamard = enabled == 2 && hadamard_eligible(L);
+  if (!pairs && !hadamard) return SPX_OK;
   for (size_t s = 0; s < L; ++s)
     if (std::abs(scale * 0.5 * h->hx[s]) > kNegligible || std::abs(scale * 0.5 * h->hy[s]) > kNegligible)
       return SPX_OK;
@@ -1016,6 +1023,11 @@ static int prepare_dense(spx_ham* h, double scale) {
     const size_t lo = h->lo[p], hi = h->hi[p];
     const size_t q = lo * L - lo * (lo + 1) / 2 + (hi - lo - 1);
     coef[q] = std::abs(ff) > kNegligible ? ff : 0.0;
+  }
+  if (hadamard) {
+    SPX_TRY(hadamard_prepare(h, coef));
+    h->wht_ok = true;
+    return SPX_OK;
   }
   if (!h->d_coef) SPX_HIP(hipMalloc(&h->d_coef, 32 * 31 / 2 * sizeof(double)));
   SPX_HIP(hipMemcpyAsync(h->d_coef, coef.data(), np * sizeof(double), hipMemcpyHostToDevice,
@@ -1288,22 +1300,7 @@ static int launch_dense_kind(spx_ham* h, const KDArgs& a, unsigned nvec) {
   return SPX_OK;
 }
 
-static int launch_dense(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
-  KDArgs a{};
-  for (int i = 0; i < kMaxMany; ++i) {
-    a.in[i] = b.in[i];
-    a.out[i] = b.out[i];
-    a.psi[i] = b.psi[i];
-  }
-  a.diag = b.diag;
-  a.coef = h->d_coef;
-  a.xcd_remap = b.xcd_remap;
-  a.sdiag = b.sdiag;
-  a.a0 = b.a0;
-  a.c1r = b.c1r;
-  a.c1i = b.c1i;
-  a.cr = b.cr;
-  a.ci = b.ci;
+static int launch_dense_args(spx_ham* h, int kind, const KDArgs& a, unsigned nvec) {
   switch (kind) {
     case K_ORDER1: return launch_dense_kind<K_ORDER1>(h, a, nvec);
     case K_ORDER2: return launch_dense_kind<K_ORDER2>(h, a, nvec);
@@ -1313,7 +1310,42 @@ static int launch_dense(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
   }
 }
 
+// One Hamiltonian for every vector of the launch.
+static int launch_dense(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
+  KDArgs a{};
+  for (int i = 0; i < kMaxMany; ++i) {
+    a.in[i] = b.in[i];
+    a.out[i] = b.out[i];
+    a.psi[i] = b.psi[i];
+    a.diag[i] = b.diag;
+    a.coef[i] = h->d_coef;
+    a.sdiag[i] = b.sdiag;
+    a.a0[i] = b.a0;
+    a.c1r[i] = b.c1r;
+    a.c1i[i] = b.c1i;
+    a.cr[i] = b.cr;
+    a.ci[i] = b.ci;
+  }
+  a.xcd_remap = b.xcd_remap;
+  return launch_dense_args(h, kind, a, nvec);
+}
+
+static int launch_wht(spx_ham* h, int kind, const KArgs& b, unsigned nvec) {
+  HadamardLaunch a{};
+  for (int i = 0; i < kMaxMany; ++i) {
+    a.in[i] = b.in[i];
+    a.out[i] = b.out[i];
+    a.psi[i] = b.psi[i];
+    a.ham[i] = h;
+    a.slot[i] = i;
+    a.sdiag[i] = b.sdiag;
+    a.w[i] = OrderWeights{b.a0, b.c1r, b.c1i, b.cr, b.ci};
+  }
+  return launch_hadamard(kind, a, nvec);
+}
+
 static int launch_k(spx_ham* h, int kind, const KArgs& a, unsigned nvec) {
+  if (h->wht_ok) return launch_wht(h, kind, a, nvec);
   if (h->dense_ok) return launch_dense(h, kind, a, nvec);
   if (h->terms_variant == 3) return launch_k3(h, kind, a, nvec);
   spx_ctx* c = h->ctx;
@@ -1376,6 +1408,69 @@ int launch_order(spx_ham* h, int kind, const OrderLaunch& v, double a0, double c
   a.ci = ci;
   const int k = kind == 1 ? K_ORDER1 : kind == 2 ? K_ORDER2 : K_ORDERN;
   return launch_k(h, k, a, static_cast<unsigned>(v.nvec));
+}
+
+bool order_set_compatible(spx_ham* const* hs, size_t n) {
+  if (n == 0 || !hs[0] || !hs[0]->terms_valid) return false;
+  const spx_ham* h = hs[0];
+  for (size_t i = 1; i < n; ++i) {
+    const spx_ham* g = hs[i];
+    if (!g || !g->terms_valid || g->nspins != h->nspins || g->ctx != h->ctx) return false;
+    if (h->wht_ok) {
+      if (!g->wht_ok) return false;
+    } else if (h->dense_ok) {
+      if (!g->dense_ok || dense_tile_bits(g) != dense_tile_bits(h) || xcd_remap(g) != xcd_remap(h)) return false;
+    } else if (g != h) {
+      return false;
+    }
+  }
+  return true;
+}
+
+int launch_order_set(spx_ham* const* hs, int kind, const OrderLaunch& v, const OrderWeights* w) {
+  const unsigned nvec = static_cast<unsigned>(v.nvec);
+  if (nvec < 1 || nvec > static_cast<unsigned>(kMaxMany) || !order_set_compatible(hs, nvec))
+    return fail(SPX_EINVAL, "these Hamiltonians cannot share an order launch");
+  spx_ham* h = hs[0];
+  const int k = kind == 1 ? K_ORDER1 : kind == 2 ? K_ORDER2 : K_ORDERN;
+  if (h->wht_ok) {
+    HadamardLaunch a{};
+    for (unsigned i = 0; i < nvec; ++i) {
+      a.in[i] = v.in[i];
+      a.out[i] = v.out[i];
+      a.psi[i] = v.psi[i];
+      a.ham[i] = hs[i];
+      int slot = 0;
+      for (unsigned j = 0; j < i; ++j) slot += hs[j] == hs[i] ? 1 : 0;
+      a.slot[i] = slot;
+      a.sdiag[i] = hs[i]->terms_scale;
+      a.w[i] = w[i];
+    }
+    return launch_hadamard(k, a, nvec);
+  }
+  if (h->dense_ok) {
+    KDArgs a{};
+    for (unsigned i = 0; i < nvec; ++i) {
+      a.in[i] = v.in[i];
+      a.out[i] = v.out[i];
+      a.psi[i] = v.psi[i];
+      a.diag[i] = hs[i]->d_diag;
+      a.coef[i] = hs[i]->d_coef;
+      a.sdiag[i] = hs[i]->terms_scale;
+      a.a0[i] = w[i].a0;
+      a.c1r[i] = w[i].c1r;
+      a.c1i[i] = w[i].c1i;
+      a.cr[i] = w[i].cr;
+      a.ci[i] = w[i].ci;
+    }
+    a.xcd_remap = xcd_remap(h);
+    return launch_dense_args(h, k, a, nvec);
+  }
+  for (unsigned i = 1; i < nvec; ++i)
+    if (w[i].a0 != w[0].a0 || w[i].c1r != w[0].c1r || w[i].c1i != w[0].c1i || w[i].cr != w[0].cr ||
+        w[i].ci != w[0].ci)
+      return fail(SPX_EINVAL, "the table kernels take one set of weights per launch");
+  return launch_order(h, kind, v, w[0].a0, w[0].c1r, w[0].c1i, w[0].cr, w[0].ci);
 }
 
 int launch_fill(spx_ctx* c, double2* y, size_t n) {

@@ -1,14 +1,16 @@
 // Autotuner of the order-kernel schedule (no reference counterpart).
 //
 // For one Hamiltonian at one step bound it times every eligible schedule —
-// the all-pairs kernel at tile bits 10 and 9, the table kernels at tile bits
+// the Hadamard-sandwich path, the all-pairs kernel at tile bits 10 and 9,
+// the table kernels at tile bits
 // 6..10, each with and without the XCD-aware tile order, and the
 // per-amplitude-group kernel at two block sizes — on the general Chebyshev
 // order (T_n = 2Ĥ T_{n-1} - T_{n-2}, psi += c T_n) and keeps the fastest.
 // Small tiles (6, 7 bits) are tried up to L = 16, where few large tiles
 // would leave most compute units idle.
-// Winners are cached per problem shape (L, whether the all-pairs kernel is
-// eligible, number of flip terms), so the realisations of an ensemble pay
+// Winners are cached per problem shape (L, whether the flip terms are
+// flip-flop pairs only, number of flip terms, states per launch), so the
+// realisations of an ensemble pay
 // for one tuning. Schedules change the order of floating-point additions, not
 // the terms, so results agree to rounding across schedules.
 #include <algorithm>
@@ -24,14 +26,16 @@
 namespace spx {
 namespace {
 
-using Key = std::tuple<size_t, bool, size_t>;
+using Key = std::tuple<size_t, bool, size_t, int>;
 
 std::mutex g_mu;
 std::map<Key, Schedule> g_cache;
 int g_mode = 0;
 
-// The shape a schedule depends on, at the scale of one step.
-Key shape_of(const spx_ham* h, double scale) {
+// The shape a schedule depends on, at the scale of one step: L, whether the
+// flip terms are flip-flop pairs only (the all-pairs and Hadamard paths), the
+// flip-term count, and how many states share each launch.
+Key shape_of(const spx_ham* h, double scale, int nvec) {
   bool fields = false, flipboth = false;
   size_t terms = 0;
   for (size_t s = 0; s < h->nspins; ++s) {
@@ -45,11 +49,11 @@ Key shape_of(const spx_ham* h, double scale) {
     flipboth |= std::abs(fb) > kNegligible;
     terms += (std::abs(ff) > kNegligible || std::abs(fb) > kNegligible) ? 1 : 0;
   }
-  const bool dense = h->nspins >= 14 && h->nspins <= 18 && !fields && !flipboth;
-  return Key{h->nspins, dense, terms};
+  return Key{h->nspins, !fields && !flipboth, terms, nvec};
 }
 
 std::string describe(const Schedule& s, const spx_ham* h) {
+  if (h->wht_ok) return "hadamard";
   if (h->dense_ok) return "all-pairs tb=" + std::to_string(dense_tile_bits(h)) + " xcd=" + std::to_string(xcd_remap(h));
   const std::string kind = h->terms_variant == 3 ? "table" : "group";
   std::string d = kind + " tb=" + std::to_string(h->terms_tb) + " xcd=" + std::to_string(xcd_remap(h));
@@ -58,10 +62,15 @@ std::string describe(const Schedule& s, const spx_ham* h) {
   return d;
 }
 
-std::vector<Schedule> candidates(const spx_ham* h, bool dense) {
+std::vector<Schedule> candidates(const spx_ham* h, bool flipflop) {
   std::vector<Schedule> c;
   const int L = static_cast<int>(h->nspins);
-  if (dense)
+  if (flipflop && hadamard_eligible(h->nspins)) {
+    Schedule s;
+    s.dense = 2;
+    c.push_back(s);
+  }
+  if (flipflop && L >= 14 && L <= 18)
     for (int tb : {10, 9})
       for (int xcd : {0, 1}) {
         Schedule s;
@@ -96,23 +105,25 @@ std::vector<Schedule> candidates(const spx_ham* h, bool dense) {
 }
 
 struct Scratch {
-  double2* v[3] = {};
+  double2* v[3 * kMaxMany] = {};
   ~Scratch() {
     for (double2* p : v)
       if (p) (void)hipFree(p);
   }
 };
 
-int tune(spx_ham* h, double bound, std::string* report) {
+// nvec states share every timed launch (the width of a batched evolve).
+int tune(spx_ham* h, double bound, int nvec, std::string* report) {
   if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
+  if (nvec < 1 || nvec > kMaxMany) return fail(SPX_EINVAL, "states per launch must be in [1, 4]");
   const double scale = -2.0 / bound;
-  const Key key = shape_of(h, scale);
+  const Key key = shape_of(h, scale, nvec);
   spx_ctx* c = h->ctx;
   const size_t n = static_cast<size_t>(1) << h->nspins;
   Scratch sc;
-  for (double2*& p : sc.v) {
-    SPX_HIP(hipMalloc(&p, n * sizeof(double2)));
-    SPX_HIP(hipMemsetAsync(p, 0, n * sizeof(double2), c->stream));
+  for (int i = 0; i < 3 * nvec; ++i) {
+    SPX_HIP(hipMalloc(&sc.v[i], n * sizeof(double2)));
+    SPX_HIP(hipMemsetAsync(sc.v[i], 0, n * sizeof(double2), c->stream));
   }
   hipEvent_t e0, e1;
   SPX_HIP(hipEventCreate(&e0));
@@ -126,12 +137,14 @@ int tune(spx_ham* h, double bound, std::string* report) {
     h->sched = s;
     h->terms_valid = false;
     if ((rc = ham_prepare(h, scale)) != SPX_OK) break;
-    if (s.dense == 1 && !h->dense_ok) continue;
+    if ((s.dense == 1 && !h->dense_ok) || (s.dense == 2 && !h->wht_ok)) continue;
     OrderLaunch v{};
-    v.in[0] = sc.v[0];
-    v.out[0] = sc.v[1];
-    v.psi[0] = sc.v[2];
-    v.nvec = 1;
+    for (int i = 0; i < nvec; ++i) {
+      v.in[i] = sc.v[3 * i];
+      v.out[i] = sc.v[3 * i + 1];
+      v.psi[i] = sc.v[3 * i + 2];
+    }
+    v.nvec = nvec;
     for (int i = 0; i < 2 && rc == SPX_OK; ++i) rc = launch_order(h, 3, v, 0.0, 0.0, 0.0, 1e-3, 0.0);
     if (rc == SPX_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(SPX_EDEVICE, "hipEventRecord failed");
     for (int i = 0; i < reps && rc == SPX_OK; ++i) rc = launch_order(h, 3, v, 0.0, 0.0, 0.0, 1e-3, 0.0);
@@ -164,14 +177,14 @@ int tune(spx_ham* h, double bound, std::string* report) {
 
 }  // namespace
 
-int autotune_before_evolve(spx_ham* h, double bound) {
+int autotune_before_evolve(spx_ham* h, double bound, int nvec) {
   int mode;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     mode = g_mode;
   }
   if (mode == 0 || h->sched_set) return SPX_OK;
-  const Key key = shape_of(h, -2.0 / bound);
+  const Key key = shape_of(h, -2.0 / bound, nvec);
   {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_cache.find(key);
@@ -182,7 +195,7 @@ int autotune_before_evolve(spx_ham* h, double bound) {
       return SPX_OK;
     }
   }
-  return tune(h, bound, nullptr);
+  return tune(h, bound, nvec, nullptr);
 }
 
 }  // namespace spx
@@ -199,9 +212,13 @@ int spx_autotune_mode(int mode) {
 }
 
 int spx_ham_autotune(spx_ham* h, double bound, char* report, size_t len) {
+  return spx_ham_autotune_batch(h, bound, 1, report, len);
+}
+
+int spx_ham_autotune_batch(spx_ham* h, double bound, int nvec, char* report, size_t len) {
   if (!h) return fail(SPX_EINVAL, "null Hamiltonian");
   std::string r;
-  SPX_TRY(tune(h, bound, &r));
+  SPX_TRY(tune(h, bound, nvec, &r));
   if (report && len > 0) {
     const size_t k = std::min(len - 1, r.size());
     std::copy(r.begin(), r.begin() + static_cast<long>(k), report);
@@ -220,7 +237,8 @@ int spx_ham_set_schedule(spx_ham* h, int tile_bits, int block, int xcd, int dens
   s.tb = tile_bits;
   s.block = block;
   s.xcd = xcd < 0 ? -1 : (xcd ? 1 : 0);
-  s.dense = dense < 0 ? -1 : (dense ? 1 : 0);
+  if (dense > 2) return fail(SPX_EINVAL, "dense must be -1, 0, 1 (pair gathers) or 2 (Hadamard transforms)");
+  s.dense = dense < 0 ? -1 : dense;
   s.dense_tb = dense_tile_bits;
   h->sched = s;
   h->sched_set = true;

@@ -114,6 +114,11 @@ SIGNATURES = {
     "spx_prop_evolve": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_double, C.c_size_t, C.c_int]),
     "spx_prop_evolve_many": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), C.c_size_t, C.c_double,
                                        C.c_size_t, C.c_int]),
+    "spx_prop_evolve_set": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.c_size_t, C.c_double,
+                                      C.c_size_t, C.c_int]),
+    "spx_prop_steps_set": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.c_size_t, C.POINTER(_dp),
+                                     _szp, _dp, C.c_size_t, C.c_int]),
+    "spx_ham_autotune_batch": (C.c_int, [_vp, C.c_double, C.c_int, C.c_char_p, C.c_size_t]),
     "spx_prop_last_order": (C.c_int, [_vp, _szp]),
     "spx_prop_set_graphs": (C.c_int, [_vp, C.c_int]),
     "spx_prop_epsilon": (C.c_int, [_vp, _dp]),
@@ -371,11 +376,12 @@ class Hamiltonian:
         check(lib().spx_ham_spectral_bound(self.h, C.byref(v)))
         return v.value
 
-    def autotune(self, bound: float) -> str:
-        """Time the eligible order-kernel schedules at this step bound and
-        keep the fastest; returns the timing report."""
+    def autotune(self, bound: float, nvec: int = 1) -> str:
+        """Time the eligible order-kernel schedules at this step bound, with
+        `nvec` states sharing each launch, and keep the fastest; returns the
+        timing report."""
         buf = C.create_string_buffer(4096)
-        check(lib().spx_ham_autotune(self.h, bound, buf, 4096))
+        check(lib().spx_ham_autotune_batch(self.h, bound, nvec, buf, 4096))
         return buf.value.decode()
 
     def set_schedule(self, tile_bits=0, block=0, xcd=-1, dense=-1, dense_tile_bits=0):
@@ -430,6 +436,31 @@ class Propagator:
         arr = (_vp * len(states))(*[s.h for s in states])
         check(lib().spx_prop_evolve_many(self.h, ham.h, mode, arr, len(states), tau, nsteps,
                                          int(autonormalize)))
+
+    def evolve_set(self, hams: Sequence[Hamiltonian], mode: int, states: Sequence[StateVector], tau: float,
+                   nsteps: int = 1, autonormalize: bool = False):
+        """State i under hams[i], each with its own expansion; up to four
+        states share every order launch where their Hamiltonians allow."""
+        if len(hams) != len(states):
+            raise InvalidArgument(EINVAL, "one Hamiltonian per state")
+        ha = (_vp * len(hams))(*[h.h for h in hams])
+        sa = (_vp * len(states))(*[s.h for s in states])
+        check(lib().spx_prop_evolve_set(self.h, ha, mode, sa, len(states), tau, nsteps, int(autonormalize)))
+
+    def steps_set(self, hams: Sequence[Hamiltonian], mode: int, states: Sequence[StateVector],
+                  coeffs: Sequence[np.ndarray], bounds: Sequence[float], nsteps: int, autonormalize: bool = False):
+        """evolve_set over prebuilt expansions: state i takes coeffs[i] at bounds[i]."""
+        n = len(states)
+        if not (len(hams) == len(coeffs) == len(bounds) == n):
+            raise InvalidArgument(EINVAL, "one Hamiltonian, expansion and bound per state")
+        cs = [np.ascontiguousarray(c, dtype=float) for c in coeffs]
+        ha = (_vp * n)(*[h.h for h in hams])
+        sa = (_vp * n)(*[s.h for s in states])
+        ca = (_dp * n)(*[_d(c) for c in cs])
+        orders = np.array([c.size - 1 for c in cs], dtype=np.uintp)
+        b = np.ascontiguousarray(bounds, dtype=float)
+        check(lib().spx_prop_steps_set(self.h, ha, mode, sa, n, ca, orders.ctypes.data_as(_szp), _d(b), nsteps,
+                                       int(autonormalize)))
 
     def steps(self, ham: Hamiltonian, mode: int, psi: StateVector, coeffs: np.ndarray, bound: float,
               nsteps: int, autonormalize: bool = False):

@@ -1,0 +1,126 @@
+"""Propagator.evolve_set: realisations (one Hamiltonian each, as in
+LocalRunner::run, src/ensemble/EnsembleRunner.cpp:27-88) stepped together,
+up to four per order launch, each with its own spectral bound and
+expansion order. Every state must come out bitwise as when evolved alone."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from spinsim_amd import spx
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def realisation(L, index, scale=1.0, transverse=False):
+    c, hz, _ = O.dipolar_lattice(L, 20240726, index, min_sep=0.5, angular=True, disorder=0.5)
+    fields = np.zeros((L, 3))
+    fields[:, 2] = hz * scale
+    if transverse:
+        fields[:, 0] = 0.4
+    c.jx, c.jy, c.jz = c.jx * scale, c.jy * scale, c.jz * scale
+    return fields, c
+
+
+def device_ham(ctx, L, fields, c, dense):
+    h = spx.Hamiltonian(ctx, L)
+    h.set_fields(fields)
+    h.set_couplings(c.lo, c.hi, c.jx, c.jy, c.jz)
+    h.set_schedule(dense=dense)
+    return h
+
+
+def check_set(ctx, L, specs, dense, nsteps=3, tau=0.05):
+    """specs: (index, scale, transverse) per realisation."""
+    prop = spx.Propagator(ctx, L, 1e-12)
+    hams = [device_ham(ctx, L, *realisation(L, i, s, t), dense) for i, s, t in specs]
+    psis = [O.random_state(L, 300 + k) for k in range(len(specs))]
+    orders, singles = [], []
+    for h, p in zip(hams, psis):
+        s = spx.StateVector(ctx, L, p)
+        prop.evolve(h, spx.REAL, s, tau, nsteps)
+        orders.append(prop.last_order())
+        singles.append(s.download())
+    states = [spx.StateVector(ctx, L, p) for p in psis]
+    prop.evolve_set(hams, spx.REAL, states, tau, nsteps)
+    assert prop.last_order() == max(orders)
+    for k, (s, want) in enumerate(zip(states, singles)):
+        assert np.array_equal(s.download(), want), (k, orders[k])
+    return orders, [h.kernel_name() for h in hams]
+
+
+@pytest.mark.parametrize("dense", [2, 1])
+def test_uneven_orders_bitwise_equal_to_single_evolves(ctx, dense):
+    L = 16
+    banner(f"evolve_set L={L} dense={dense}")
+    # orders: x3 couplings (long), x1 (typical), x1e-6 (order 1), six states
+    specs = [(0, 1.0, False), (1, 3.0, False), (2, 1e-6, False), (3, 1.0, False), (4, 0.5, False), (5, 1.0, False)]
+    orders, names = check_set(ctx, L, specs, dense)
+    banner(f"orders {orders}")
+    assert 1 in orders and len(set(orders)) >= 3
+    prefix = "k_wht_rows_fwd" if dense == 2 else "k_dense"
+    assert all(n.startswith(prefix) for n in names), names
+
+
+def test_mixed_paths_fall_back_to_one_by_one(ctx):
+    L = 14
+    banner("evolve_set mixed paths L=14")
+    specs = [(0, 1.0, False), (1, 1.0, True), (2, 1.0, False), (3, 2.0, True), (4, 1.0, False)]
+    orders, names = check_set(ctx, L, specs, 2)
+    assert names[1].startswith("k_order3") or names[1].startswith("k_horder"), names
+
+
+def test_l18_hadamard_set_of_four(ctx):
+    L = 18
+    banner("evolve_set L=18 dense=2, four realisations")
+    check_set(ctx, L, [(i, 1.0, False) for i in range(4)], 2, nsteps=2)
+
+
+def test_same_hamiltonian_twice_and_argument_errors(ctx):
+    L = 14
+    fields, c = realisation(L, 9)
+    h = device_ham(ctx, L, fields, c, 2)
+    prop = spx.Propagator(ctx, L, 1e-12)
+    p0, p1 = O.random_state(L, 1), O.random_state(L, 2)
+    a, b = spx.StateVector(ctx, L, p0), spx.StateVector(ctx, L, p1)
+    prop.evolve_set([h, h], spx.REAL, [a, b], 0.05, 2)
+    for p, s in ((p0, a), (p1, b)):
+        one = spx.StateVector(ctx, L, p)
+        prop.evolve(h, spx.REAL, one, 0.05, 2)
+        assert np.array_equal(one.download(), s.download())
+    with pytest.raises(spx.InvalidArgument):
+        prop.evolve_set([h], spx.REAL, [a, b], 0.05, 1)
+    with pytest.raises(spx.InvalidArgument):
+        prop.evolve_set([h, h], spx.REAL, [a, a], 0.05, 1)
+    other = spx.StateVector(ctx, L + 1, O.random_state(L + 1, 3))
+    with pytest.raises(spx.InvalidArgument):
+        prop.evolve_set([h], spx.REAL, [other], 0.05, 1)
+
+
+def test_steps_set_over_prebuilt_expansions_matches_evolve_set(ctx):
+    L = 16
+    banner("steps_set vs evolve_set L=16")
+    hams = [device_ham(ctx, L, *realisation(L, i, s), 2) for i, s in ((0, 1.0), (1, 2.0), (2, 1.0))]
+    prop = spx.Propagator(ctx, L, 1e-12)
+    psis = [O.random_state(L, 400 + k) for k in range(3)]
+    a = [spx.StateVector(ctx, L, p) for p in psis]
+    prop.evolve_set(hams, spx.REAL, a, 0.05, 2)
+    built = [spx.cheb_build(spx.REAL, h.spectral_bound(), 0.05, 1e-12) for h in hams]
+    b = [spx.StateVector(ctx, L, p) for p in psis]
+    prop.steps_set(hams, spx.REAL, b, [c for c, _ in built], [e for _, e in built], 2)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.download(), y.download())
+
+
+def test_batched_autotune_reports_the_hadamard_path(ctx):
+    L = 16
+    banner("autotune at width 4, L=16")
+    h = device_ham(ctx, L, *realisation(L, 0), -1)
+    report = h.autotune(h.spectral_bound(), nvec=4)
+    banner(report[:300])
+    assert report.startswith("chosen ") and "hadamard: " in report
+    with pytest.raises(spx.InvalidArgument):
+        h.autotune(h.spectral_bound(), nvec=5)

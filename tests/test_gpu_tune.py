@@ -42,7 +42,7 @@ def test_every_schedule_matches_the_oracle_and_the_tuner_keeps_one(ctx, L):
     schedules = [dict(dense=0, tile_bits=10), dict(dense=0, tile_bits=9, xcd=1), dict(dense=0, tile_bits=8),
                  dict(dense=0, tile_bits=10, block=1024)]
     if L >= 14:
-        schedules += [dict(dense=1, dense_tile_bits=10), dict(dense=1, dense_tile_bits=9, xcd=1)]
+        schedules += [dict(dense=1, dense_tile_bits=10), dict(dense=1, dense_tile_bits=9, xcd=1), dict(dense=2)]
     names = set()
     for sch in schedules:
         h = device_ham(ctx, L, fields, c)
