@@ -458,6 +458,7 @@ int one_step(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t
     v.in[i] = states[i]->d;
     v.out[i] = B[i];
     v.psi[i] = states[i]->d;
+    v.uv_next[i] = order >= 2;   // each order hands its output's row transforms on
   }
   SPX_TRY(launch_order(h, 1, v, 0, 0, 0, 0, 0));
   const double c1r = 2.0 * a[1] * wre, c1i = 2.0 * a[1] * wim;
@@ -467,9 +468,11 @@ int one_step(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t
       SPX_TRY(launch_axpby(p->ctx, states[i]->d, a[0], B[i], c1r, c1i, states[i]->n));
     return SPX_OK;
   }
+  v.uv_ready = 1;
   for (size_t i = 0; i < nvec; ++i) {
     v.in[i] = B[i];
     v.out[i] = A[i];
+    v.uv_next[i] = order >= 3;
   }
   SPX_TRY(launch_order(h, 2, v, a[0], c1r, c1i, 2.0 * a[2] * wre, 2.0 * a[2] * wim));
   advance();
@@ -478,6 +481,7 @@ int one_step(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t
       const bool odd = (n % 2) == 1;
       v.in[i] = odd ? A[i] : B[i];    // T_{n-1}
       v.out[i] = odd ? B[i] : A[i];   // T_{n-2} -> T_n
+      v.uv_next[i] = n < order;
     }
     SPX_TRY(launch_order(h, 3, v, 0, 0, 0, 2.0 * a[n] * wre, 2.0 * a[n] * wim));
     advance();
@@ -587,6 +591,7 @@ int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* st
     v.in[i] = states[i]->d;
     v.out[i] = B[i];
     v.psi[i] = states[i]->d;
+    v.uv_next[i] = order_of(i) >= 2;   // each order hands its output's row transforms on
   }
   SPX_TRY(launch_order_set(hs, 1, v, w));
   double c1r[kMaxMany], c1i[kMaxMany];
@@ -602,9 +607,11 @@ int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* st
   }
   if (act == 0) return SPX_OK;
   v.nvec = static_cast<int>(act);
+  v.uv_ready = 1;
   for (size_t i = 0; i < act; ++i) {
     v.in[i] = B[i];
     v.out[i] = A[i];
+    v.uv_next[i] = order_of(i) >= 3;
     w[i] = OrderWeights{(*a[i])[0], c1r[i], c1i[i], 2.0 * (*a[i])[2] * wre, 2.0 * (*a[i])[2] * wim};
   }
   SPX_TRY(launch_order_set(hs, 2, v, w));
@@ -617,6 +624,7 @@ int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* st
       const bool odd = (n % 2) == 1;
       v.in[i] = odd ? A[i] : B[i];
       v.out[i] = odd ? B[i] : A[i];
+      v.uv_next[i] = order_of(i) > n;
       w[i] = OrderWeights{0.0, 0.0, 0.0, 2.0 * (*a[i])[n] * wre, 2.0 * (*a[i])[n] * wim};
     }
     SPX_TRY(launch_order_set(hs, 3, v, w));

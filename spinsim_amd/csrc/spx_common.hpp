@@ -109,7 +109,7 @@ struct spx_ham {
   bool dense_ok = false;         // specialised all-pairs kernel usable
   double* d_coef = nullptr;      // its weights
   bool wht_ok = false;           // Hadamard-sandwich flip-flop path selected
-  double* d_wht_e = nullptr;     // its Hadamard-basis diagonal E(k), 2^L
+  double* d_wht_e = nullptr;     // its Hadamard-basis diagonal: E_lo[2^10], E_hi[2^(L-10)], A[L-10][2^10]
   double2* d_wht = nullptr;      // its transform scratch, 2 * kMaxMany vectors of 2^L
   double terms_scale = 0.0;
   bool terms_valid = false;
@@ -143,6 +143,12 @@ struct OrderLaunch {
   double2* out[kMaxMany];        // result / T_n (in-place over T_{n-2})
   double2* psi[kMaxMany];        // accumulator
   int nvec;
+  // Hadamard path only (ignored elsewhere): the previous launch of these
+  // vectors already left the row transforms of `in` in the Hamiltonians'
+  // scratch (uv_ready), and vector i's launch leaves those of `out` for the
+  // next one (uv_next[i]); consecutive Chebyshev orders chain this way.
+  int uv_ready;
+  int uv_next[kMaxMany];
 };
 
 int ham_prepare(spx_ham* h, double scale);   // diagonal + flip table at scale
@@ -164,6 +170,8 @@ struct HadamardLaunch {
   int slot[kMaxMany];
   double sdiag[kMaxMany];
   OrderWeights w[kMaxMany];
+  int ready;             // see OrderLaunch::uv_ready
+  int next[kMaxMany];    // see OrderLaunch::uv_next
 };
 bool hadamard_eligible(size_t L);
 // coef: the L(L-1)/2 flip-flop weights at the apply scale, pair order (0,1),(0,2),...

@@ -25,6 +25,14 @@
 //   2. columns: X = W_high E W_high X for X in {U, V}         8 columns per workgroup
 //   3. rows:    H psi = sdiag diag psi + W_low U + P W_low V, fused with the
 //               same Chebyshev / accumulate epilogues as the gather kernels.
+// In a Chebyshev expansion the input of order n+1 is the output T_n of order
+// n, so pass 3 also runs pass 1 on T_n while it is in registers: an order
+// after the first costs two launches.
+// E is not stored per amplitude: with k = (x, c) (x the G column-pass bits,
+// c the TB row bits), E(k) = E_lo(c) + E_hi(x) + sum_j z_j(x) A_j(c), where
+// E_lo / E_hi collect the pairs inside c / inside x and A_j(c) = sum_i w_(i,
+// TB+j) z_i(c) / 2^(L+1) the pairs across; the tables hold 2^TB + 2^G +
+// G 2^TB values and stay in L2.
 // The transforms are exact sums and differences; results agree with the
 // gather kernels to rounding (a few ulps of the amplitude scale).
 #include <cmath>
@@ -48,9 +56,10 @@ struct WhtArgs {
   double2* u[kMaxMany];         // scratch: W E W psi
   double2* w[kMaxMany];         // scratch: W E W P psi
   const double* diag[kMaxMany];
-  const double* e[kMaxMany];    // E(k), length 2^L
+  const double* e[kMaxMany];    // E_lo[2^TB], E_hi[2^G], A[G][2^TB]
   double sdiag[kMaxMany];
   double a0[kMaxMany], c1r[kMaxMany], c1i[kMaxMany], cr[kMaxMany], ci[kMaxMany];
+  int next[kMaxMany];           // pass 3 also leaves the row transforms of its output
 };
 
 enum { K_APPLY = 0, K_ACCUM = 1, K_ORDER1 = 2, K_ORDER2 = 3, K_ORDERN = 4 };
@@ -152,27 +161,40 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_fwd(WhtArgs a) {
 // multiply by E and transform again. Each row segment is 128 contiguous bytes.
 template <int G>
 __global__ __launch_bounds__(1 << G) void k_wht_cols(WhtArgs a) {
-  constexpr uint32_t NT = 1u << G, NV = kCols << G, M = NV / NT;
+  constexpr uint32_t T = 1u << kTB, NT = 1u << G, NV = kCols << G, M = NV / NT;
   extern __shared__ double2 smem[];
   const uint32_t t = threadIdx.x;
   const int v = static_cast<int>(blockIdx.y >> 1);
   double2* __restrict__ X = (blockIdx.y & 1u) ? a.w[v] : a.u[v];
   const uint32_t col0 = blockIdx.x * kCols;
-  const double* __restrict__ e = a.e[v];
   double2 x[M];
-  double s[M];
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
     const uint32_t f = t + m * NT;
     x[m] = X[(static_cast<size_t>(f >> 3) << kTB) | (col0 + (f & 7u))];
   }
+  // E at this thread's elements: column c = t mod 8 (NT is a multiple of 8),
+  // rows (t >> 3) + m NT / 8
+  const double* __restrict__ tab = a.e[v];
+  const uint32_t c = col0 + (t & 7u);
+  double aj[G];
+  const double elo = tab[c];
 #pragma unroll
-  for (uint32_t m = 0; m < M; ++m) {
-    const uint32_t f = t + m * NT;
-    s[m] = e[(static_cast<size_t>(f >> 3) << kTB) | (col0 + (f & 7u))];
-  }
+  for (int j = 0; j < G; ++j) aj[j] = tab[T + NT + j * T + c];
+  double ehi[M];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) ehi[m] = tab[T + (t >> 3) + m * (NT >> 3)];
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) smem[t + m * NT] = x[m];
+  double s[M];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint32_t row = (t >> 3) + m * (NT >> 3);
+    double e = elo + ehi[m];
+#pragma unroll
+    for (int j = 0; j < G; ++j) e += ((row >> j) & 1u) ? -aj[j] : aj[j];
+    s[m] = e;
+  }
   __syncthreads();
   wht_rounds<G, 3>(smem, t, ColSlot{});
 #pragma unroll
@@ -202,8 +224,9 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   const uint32_t t = threadIdx.x;
   const int v = blockIdx.y;
   const size_t base = static_cast<size_t>(blockIdx.x) << kTB;
-  const double2* __restrict__ U = a.u[v];
-  const double2* __restrict__ V = a.w[v];
+  // read here, rewritten in place at the end when the next order is chained
+  const double2* U = a.u[v];
+  const double2* V = a.w[v];
   const double2* __restrict__ in = a.in[v];
   double2* __restrict__ out = a.out[v];
   double2* __restrict__ psi = a.psi[v];
@@ -237,6 +260,7 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
     xu[m] = smem[rslot(t + m * NT)];
     xw[m] = smem[T + rslot(t + m * NT)];
   }
+  double2 o[M];
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
     const size_t k = base + t + m * NT;
@@ -246,45 +270,64 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
     acc.x = fma(f, xv[m].x, xu[m].x + p * xw[m].x);
     acc.y = fma(f, xv[m].y, xu[m].y + p * xw[m].y);
     if constexpr (KIND == K_APPLY) {
-      out[k] = acc;
+      o[m] = acc;
     } else if constexpr (KIND == K_ACCUM) {
-      out[k] = make_double2(prev[m].x + acc.x, prev[m].y + acc.y);
+      o[m] = make_double2(prev[m].x + acc.x, prev[m].y + acc.y);
     } else if constexpr (KIND == K_ORDER1) {
-      out[k] = make_double2(0.5 * acc.x, 0.5 * acc.y);
+      o[m] = make_double2(0.5 * acc.x, 0.5 * acc.y);
     } else if constexpr (KIND == K_ORDER2) {
       const double2 t0 = pv[m], t1 = xv[m];
       const double2 t2 = make_double2(acc.x - t0.x, acc.y - t0.y);
-      out[k] = t2;
+      o[m] = t2;
       double2 q = make_double2(a0 * t0.x, a0 * t0.y);
       q.x += c1r * t1.x - c1i * t1.y;
       q.y += c1r * t1.y + c1i * t1.x;
       q.x += cr * t2.x - ci * t2.y;
       q.y += cr * t2.y + ci * t2.x;
-      psi[k] = q;
+      pv[m] = q;
     } else {
       const double2 tn = make_double2(acc.x - prev[m].x, acc.y - prev[m].y);
-      out[k] = tn;
+      o[m] = tn;
       double2 q = pv[m];
       q.x += cr * tn.x - ci * tn.y;
       q.y += cr * tn.y + ci * tn.x;
-      psi[k] = q;
+      pv[m] = q;
     }
   }
-}
-
-// E(k) = sum_q c_q z_i(k) z_j(k), c_q already divided by 2^(L+1); the
-// table holds (mask, c) pairs with the mask as a double.
-__global__ __launch_bounds__(256) void k_wht_energy(double* __restrict__ e, size_t n,
-                                                    const double* __restrict__ tab, int np) {
-  const size_t k = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (k >= n) return;
-  double s = 0.0;
-  for (int i = 0; i < np; ++i) {
-    const uint32_t m = static_cast<uint32_t>(tab[2 * i]);
-    const double c = tab[2 * i + 1];
-    s += (__popc(static_cast<uint32_t>(k) & m) != 1) ? c : -c;
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const size_t k = base + t + m * NT;
+    out[k] = o[m];
+    if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN) psi[k] = pv[m];
   }
-  e[k] = s;
+  if constexpr (KIND == K_ORDER1 || KIND == K_ORDER2 || KIND == K_ORDERN) {
+    if (a.next[v]) {  // uniform over the workgroup
+      // pass 1 of the next order on T_n: every thread has read its transform
+      // values out of LDS before the barrier
+      __syncthreads();
+#pragma unroll
+      for (uint32_t m = 0; m < M; ++m) {
+        const uint32_t i = t + m * NT;
+        const double p = psign(static_cast<uint32_t>(base + i));
+        smem[rslot(i)] = o[m];
+        smem[T + rslot(i)] = make_double2(p * o[m].x, p * o[m].y);
+      }
+      __syncthreads();
+      wht_rounds<kTB, 0>(smem + half * T, t - half * H, RowSlot{});
+#pragma unroll
+      for (uint32_t m = 0; m < M; ++m) {
+        xu[m] = smem[rslot(t + m * NT)];
+        xw[m] = smem[T + rslot(t + m * NT)];
+      }
+      double2* __restrict__ Un = a.u[v];
+      double2* __restrict__ Vn = a.w[v];
+#pragma unroll
+      for (uint32_t m = 0; m < M; ++m) {
+        Un[base + t + m * NT] = xu[m];
+        Vn[base + t + m * NT] = xw[m];
+      }
+    }
+  }
 }
 
 template <int G>
@@ -302,31 +345,40 @@ void launch_bwd(hipStream_t st, const WhtArgs& a, dim3 grid) {
 
 bool hadamard_eligible(size_t L) { return L >= kHadamardMinSpins && L <= kHadamardMaxSpins; }
 
+// The factors of E (see the file comment), from the pair weights in pair
+// order (0,1),(0,2),...; sums over unscaled weights, then the exact scale
+// 2^-(L+1).
 int hadamard_prepare(spx_ham* h, const std::vector<double>& coef) {
   const size_t L = h->nspins, n = static_cast<size_t>(1) << L;
   if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard path needs 14 <= L <= 20");
-  std::vector<double> tab;
-  size_t q = 0;
-  for (size_t lo = 0; lo < L; ++lo)
-    for (size_t hi = lo + 1; hi < L; ++hi, ++q) {
-      if (coef[q] == 0.0) continue;
-      tab.push_back(static_cast<double>((1u << lo) | (1u << hi)));
-      tab.push_back(std::ldexp(coef[q], -static_cast<int>(L + 1)));
+  const size_t T = size_t(1) << kTB, G = L - kTB, R = size_t(1) << G;
+  const int sc = -static_cast<int>(L + 1);
+  auto weight = [&](size_t lo, size_t hi) { return coef[lo * L - lo * (lo + 1) / 2 + (hi - lo - 1)]; };
+  auto z = [](size_t k, size_t bit) { return ((k >> bit) & 1u) ? -1.0 : 1.0; };
+  std::vector<double> tab(T + R + G * T, 0.0);
+  for (size_t c = 0; c < T; ++c) {
+    double e = 0.0;
+    for (size_t i = 0; i < kTB; ++i)
+      for (size_t k = i + 1; k < static_cast<size_t>(kTB); ++k) e += weight(i, k) * z(c, i) * z(c, k);
+    tab[c] = std::ldexp(e, sc);
+  }
+  for (size_t x = 0; x < R; ++x) {
+    double e = 0.0;
+    for (size_t i = 0; i < G; ++i)
+      for (size_t k = i + 1; k < G; ++k) e += weight(kTB + i, kTB + k) * z(x, i) * z(x, k);
+    tab[T + x] = std::ldexp(e, sc);
+  }
+  for (size_t j = 0; j < G; ++j)
+    for (size_t c = 0; c < T; ++c) {
+      double e = 0.0;
+      for (size_t i = 0; i < static_cast<size_t>(kTB); ++i) e += weight(i, kTB + j) * z(c, i);
+      tab[T + R + j * T + c] = std::ldexp(e, sc);
     }
   spx_ctx* c = h->ctx;
-  if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, n * sizeof(double)));
+  if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, (T + (size_t(1) << (kHadamardMaxSpins - kTB)) +
+                                                   (kHadamardMaxSpins - kTB) * T) * sizeof(double)));
   if (!h->d_wht) SPX_HIP(hipMalloc(&h->d_wht, 2 * kMaxMany * n * sizeof(double2)));
-  const size_t tbytes = (tab.size() + 2) * sizeof(double);
-  if (tbytes > h->tab_cap) {
-    if (h->d_tab) SPX_HIP(hipFree(h->d_tab));
-    SPX_HIP(hipMalloc(&h->d_tab, tbytes));
-    h->tab_cap = tbytes;
-  }
-  if (!tab.empty())
-    SPX_HIP(hipMemcpyAsync(h->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  k_wht_energy<<<static_cast<unsigned>((n + 255) / 256), 256, 0, c->stream>>>(h->d_wht_e, n, h->d_tab,
-                                                                               static_cast<int>(tab.size() / 2));
-  SPX_HIP(hipGetLastError());
+  SPX_HIP(hipMemcpyAsync(h->d_wht_e, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   SPX_HIP(hipStreamSynchronize(c->stream));  // `tab` is pageable and local
   return SPX_OK;
 }
@@ -354,10 +406,11 @@ int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
     a.c1i[i] = b.w[i].c1i;
     a.cr[i] = b.w[i].cr;
     a.ci[i] = b.w[i].ci;
+    a.next[i] = b.next[i] ? 1 : 0;
   }
   hipStream_t st = h0->ctx->stream;
   const dim3 rows(static_cast<unsigned>(n >> kTB), nvec);
-  k_wht_rows_fwd<<<rows, kRowThreads, sizeof(double2) * (2u << kTB), st>>>(a);
+  if (!b.ready) k_wht_rows_fwd<<<rows, kRowThreads, sizeof(double2) * (2u << kTB), st>>>(a);
   switch (L - kTB) {
     case 4: launch_cols<4>(st, a, nvec); break;
     case 5: launch_cols<5>(st, a, nvec); break;

@@ -1407,6 +1407,21 @@ int launch_order(spx_ham* h, int kind, const OrderLaunch& v, double a0, double c
   a.cr = cr;
   a.ci = ci;
   const int k = kind == 1 ? K_ORDER1 : kind == 2 ? K_ORDER2 : K_ORDERN;
+  if (h->wht_ok) {
+    HadamardLaunch b{};
+    for (int i = 0; i < v.nvec; ++i) {
+      b.in[i] = v.in[i];
+      b.out[i] = v.out[i];
+      b.psi[i] = v.psi[i];
+      b.ham[i] = h;
+      b.slot[i] = i;
+      b.sdiag[i] = h->terms_scale;
+      b.w[i] = OrderWeights{a0, c1r, c1i, cr, ci};
+      b.next[i] = v.uv_next[i];
+    }
+    b.ready = v.uv_ready;
+    return launch_hadamard(k, b, static_cast<unsigned>(v.nvec));
+  }
   return launch_k(h, k, a, static_cast<unsigned>(v.nvec));
 }
 
@@ -1445,7 +1460,9 @@ int launch_order_set(spx_ham* const* hs, int kind, const OrderLaunch& v, const O
       a.slot[i] = slot;
       a.sdiag[i] = hs[i]->terms_scale;
       a.w[i] = w[i];
+      a.next[i] = v.uv_next[i];
     }
+    a.ready = v.uv_ready;
     return launch_hadamard(k, a, nvec);
   }
   if (h->dense_ok) {

@@ -145,7 +145,13 @@ int tune(spx_ham* h, double bound, int nvec, std::string* report) {
       v.psi[i] = sc.v[3 * i + 2];
     }
     v.nvec = nvec;
-    for (int i = 0; i < 2 && rc == SPX_OK; ++i) rc = launch_order(h, 3, v, 0.0, 0.0, 0.0, 1e-3, 0.0);
+    // the steady state of an expansion: each order hands its row transforms
+    // to the next (Hadamard path; the other kernels ignore the flags)
+    for (int i = 0; i < nvec; ++i) v.uv_next[i] = 1;
+    for (int i = 0; i < 2 && rc == SPX_OK; ++i) {
+      rc = launch_order(h, 3, v, 0.0, 0.0, 0.0, 1e-3, 0.0);
+      v.uv_ready = 1;
+    }
     if (rc == SPX_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(SPX_EDEVICE, "hipEventRecord failed");
     for (int i = 0; i < reps && rc == SPX_OK; ++i) rc = launch_order(h, 3, v, 0.0, 0.0, 0.0, 1e-3, 0.0);
     if (rc == SPX_OK && hipEventRecord(e1, c->stream) != hipSuccess) rc = fail(SPX_EDEVICE, "hipEventRecord failed");
