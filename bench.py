@@ -2,12 +2,12 @@
 """Headline benchmark: the L=18 FP64 Chebyshev propagator on one MI355X.
 
 Workload (the single-large-state configuration, whose 64 realisations run
-four at a time): 2^18 complex FP64 amplitudes per realisation, all-to-all
+eight at a time): 2^18 complex FP64 amplitudes per realisation, all-to-all
 secular dipolar XXZ couplings (153 pairs) from random 2D positions at unit
 density (min spacing 0.5, (1 - 3cos^2 theta)/r^3) plus Gaussian Hz disorder
 (sigma 0.5), real-time steps of dt = 0.05 at epsilon = 1e-12,
 autonormalisation off. One bench "step" is one Chebyshev time step of each of
-the `--set` realisations (default 4), stepped together: every order is one
+the `--set` realisations (default 8), stepped together: every order is one
 launch sequence for the set, each realisation with its own bound and order.
 
 Prints ONE JSON line. `value` is algorithmic GB/s of the whole timed region
@@ -304,8 +304,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule")
-    ap.add_argument("--set", type=int, default=4, metavar="S",
-                    help="realisations stepped together, one launch per order (1..4; 1 = a single state)")
+    ap.add_argument("--set", type=int, default=8, metavar="S",
+                    help="realisations stepped together, one launch per order (1..8; 1 = a single state)")
     ap.add_argument("--graphs", action="store_true", help="replay each step as a captured HIP graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="L",
                     help="only time the in-place scale kernel on 2^L amplitudes")
@@ -323,8 +323,8 @@ def main():
     L = args.nspins
     n = 1 << L
     S = args.set
-    if not 1 <= S <= 4:
-        raise SystemExit("--set takes 1..4 realisations per launch")
+    if not 1 <= S <= 8:
+        raise SystemExit("--set takes 1..8 realisations per launch")
     systems = [synthetic_system(L, index=i) for i in range(S)]
     fields, lo, hi, jx, jy, jz, _ = systems[0]
     log(f"[bench] L={L} pairs={lo.size} realisations={S} building device objects")

@@ -431,62 +431,18 @@ int ensure_scratch(spx_prop* p, size_t nvec) {
 // T_{n-2} in place and psi += c_n T_n in the same pass.
 constexpr size_t kGraphMinSteps = 4;
 
+int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* states, size_t nvec,
+                 const std::vector<double>* const* a);
+
 int one_step(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t nvec,
              const std::vector<double>& a) {
-  const size_t order = a.size() - 1;
-  double2* A[kMaxMany];
-  double2* B[kMaxMany];
+  spx_ham* hs[kMaxMany];
+  const std::vector<double>* cs[kMaxMany];
   for (size_t i = 0; i < nvec; ++i) {
-    A[i] = p->d_scratch[2 * i];
-    B[i] = p->d_scratch[2 * i + 1];
+    hs[i] = h;
+    cs[i] = &a;
   }
-  // TermWeight (src/core/Propagator.cpp:14-36): w_1 = -i (real) or -1 (imag)
-  double wre = mode == 0 ? 0.0 : -1.0, wim = mode == 0 ? -1.0 : 0.0;
-  auto advance = [&] {
-    if (mode == 0) {
-      const double pr = wre;
-      wre = wim;
-      wim = -pr;
-    } else {
-      wre = -wre;
-      wim = -wim;
-    }
-  };
-  OrderLaunch v{};
-  v.nvec = static_cast<int>(nvec);
-  for (size_t i = 0; i < nvec; ++i) {
-    v.in[i] = states[i]->d;
-    v.out[i] = B[i];
-    v.psi[i] = states[i]->d;
-    v.uv_next[i] = order >= 2;   // each order hands its output's row transforms on
-  }
-  SPX_TRY(launch_order(h, 1, v, 0, 0, 0, 0, 0));
-  const double c1r = 2.0 * a[1] * wre, c1i = 2.0 * a[1] * wim;
-  advance();
-  if (order == 1) {
-    for (size_t i = 0; i < nvec; ++i)
-      SPX_TRY(launch_axpby(p->ctx, states[i]->d, a[0], B[i], c1r, c1i, states[i]->n));
-    return SPX_OK;
-  }
-  v.uv_ready = 1;
-  for (size_t i = 0; i < nvec; ++i) {
-    v.in[i] = B[i];
-    v.out[i] = A[i];
-    v.uv_next[i] = order >= 3;
-  }
-  SPX_TRY(launch_order(h, 2, v, a[0], c1r, c1i, 2.0 * a[2] * wre, 2.0 * a[2] * wim));
-  advance();
-  for (size_t n = 3; n <= order; ++n) {
-    for (size_t i = 0; i < nvec; ++i) {
-      const bool odd = (n % 2) == 1;
-      v.in[i] = odd ? A[i] : B[i];    // T_{n-1}
-      v.out[i] = odd ? B[i] : A[i];   // T_{n-2} -> T_n
-      v.uv_next[i] = n < order;
-    }
-    SPX_TRY(launch_order(h, 3, v, 0, 0, 0, 2.0 * a[n] * wre, 2.0 * a[n] * wim));
-    advance();
-  }
-  return SPX_OK;
+  return one_step_set(p, hs, mode, states, nvec, cs);
 }
 
 int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t nstates,
@@ -524,6 +480,7 @@ int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, siz
   SPX_TRY(ensure_scratch(p, nstates < kMaxMany ? nstates : kMaxMany));
   SPX_TRY(autotune_before_evolve(h, bound, static_cast<int>(nstates < kMaxMany ? nstates : kMaxMany)));
   SPX_TRY(ham_prepare(h, -2.0 / bound));
+  if (h->wht_ok) SPX_TRY(hadamard_reserve(h, static_cast<int>(nstates < kMaxMany ? nstates : kMaxMany)));
   // Without autonormalisation every step is the same sequence of launches
   // (same expansion, same buffers): record it once as a HIP graph and replay
   // it, so the host issues one launch per step instead of one per order.
@@ -558,6 +515,15 @@ int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, siz
       for (size_t i = 0; i < nstates; ++i) SPX_TRY(spx_state_normalize(states[i], nullptr));
   }
   return SPX_OK;
+}
+
+// Accumulator batching (OrderWeights::defer) for general order n of an
+// expansion of order N on the Hadamard path: the terms of orders 3, 4, 5 are
+// added at order 5, those of 6, 7, 8 at 8, ..., and whatever waits at N.
+int defer_code(size_t n, size_t N) {
+  const size_t k = (n - 3) % 3;
+  if (n == N) return k == 0 ? 0 : static_cast<int>(k) + 1;
+  return k == 2 ? 3 : 1;
 }
 
 // One step of a set of states, state i under hs[i] with its own expansion
@@ -615,6 +581,13 @@ int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* st
     w[i] = OrderWeights{(*a[i])[0], c1r[i], c1i[i], 2.0 * (*a[i])[2] * wre, 2.0 * (*a[i])[2] * wim};
   }
   SPX_TRY(launch_order_set(hs, 2, v, w));
+  // weights of orders n-1 and n-2 for batched accumulator updates
+  double p1r[kMaxMany], p1i[kMaxMany], p2r[kMaxMany] = {}, p2i[kMaxMany] = {};
+  for (size_t i = 0; i < act; ++i) {
+    p1r[i] = w[i].cr;
+    p1i[i] = w[i].ci;
+  }
+  const bool batch_psi = hs[0]->wht_ok;   // every Hamiltonian of the set then is
   advance();
   const size_t top = order_of(0);
   for (size_t n = 3; n <= top; ++n) {
@@ -626,6 +599,15 @@ int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* st
       v.out[i] = odd ? B[i] : A[i];
       v.uv_next[i] = order_of(i) > n;
       w[i] = OrderWeights{0.0, 0.0, 0.0, 2.0 * (*a[i])[n] * wre, 2.0 * (*a[i])[n] * wim};
+      w[i].pr = p1r[i];
+      w[i].pi = p1i[i];
+      w[i].qr = p2r[i];
+      w[i].qi = p2i[i];
+      w[i].defer = batch_psi ? defer_code(n, order_of(i)) : 0;
+      p2r[i] = p1r[i];
+      p2i[i] = p1i[i];
+      p1r[i] = w[i].cr;
+      p1i[i] = w[i].ci;
     }
     SPX_TRY(launch_order_set(hs, 3, v, w));
     advance();
@@ -671,6 +653,12 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
     SPX_TRY(autotune_before_evolve(hams[i], bounds[i], width));
     SPX_TRY(ham_prepare(hams[i], -2.0 / bounds[i]));
   }
+  for (size_t i : idx)
+    if (hams[i]->wht_ok) {
+      int copies = 0;   // one scratch slot per appearance, at most one launch's worth
+      for (size_t j : idx) copies += hams[j] == hams[i] ? 1 : 0;
+      SPX_TRY(hadamard_reserve(hams[i], copies < kMaxMany ? copies : kMaxMany));
+    }
   // groups of up to kMaxMany consecutive states whose Hamiltonians can share launches
   std::vector<std::pair<size_t, size_t>> groups;
   for (size_t g = 0; g < idx.size();) {

@@ -17,7 +17,7 @@ namespace spx {
 constexpr double kNegligible = 1.0e-14;
 constexpr size_t kMaxSpins = 30;          // include/spinsim/core/Bits.hpp:13
 constexpr int kMaxEvents = 16;
-constexpr int kMaxMany = 4;               // states per batched evolve launch
+constexpr int kMaxMany = 8;               // states per batched evolve launch
 constexpr int kReduceBlocks = 1024;       // fixed reduction fan-in (determinism)
 constexpr int kReduceThreads = 256;
 
@@ -110,7 +110,8 @@ struct spx_ham {
   double* d_coef = nullptr;      // its weights
   bool wht_ok = false;           // Hadamard-sandwich flip-flop path selected
   double* d_wht_e = nullptr;     // its Hadamard-basis diagonal: E_lo[2^10], E_hi[2^(L-10)], A[L-10][2^10]
-  double2* d_wht = nullptr;      // its transform scratch, 2 * kMaxMany vectors of 2^L
+  double2* d_wht = nullptr;      // its transform scratch: wht_slots pairs of 2^L vectors
+  int wht_slots = 0;
   double terms_scale = 0.0;
   bool terms_valid = false;
   std::vector<spx::FlipTerm> h_terms;
@@ -157,8 +158,14 @@ int ham_prepare(spx_ham* h, double scale);   // diagonal + flip table at scale
 // (spx_hadamard.hip): three passes per application, any L in range.
 constexpr size_t kHadamardMinSpins = 14, kHadamardMaxSpins = 20;
 // Chebyshev weights of one vector of an order launch (see launch_order).
+// `defer` (general orders on the Hadamard path only, 0 elsewhere) batches
+// the accumulator updates: 0 psi += c T_n; 1 leave psi (the term waits);
+// 2 psi += p T_{n-1} + c T_n; 3 psi += q T_{n-2} + p T_{n-1} + c T_n, in
+// that order, so psi is read and written once per three orders.
 struct OrderWeights {
   double a0, c1r, c1i, cr, ci;
+  double pr = 0.0, pi = 0.0, qr = 0.0, qi = 0.0;
+  int defer = 0;
 };
 // Vector i of a launch is propagated under ham[i] (prepared at its own
 // scale sdiag[i]) and uses scratch pair slot[i] of that Hamiltonian.
@@ -177,6 +184,9 @@ bool hadamard_eligible(size_t L);
 // coef: the L(L-1)/2 flip-flop weights at the apply scale, pair order (0,1),(0,2),...
 int hadamard_prepare(spx_ham* h, const std::vector<double>& coef);
 int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec);
+// Scratch for `slots` vectors of one launch under h (call before any launch
+// or graph capture that needs them; launches check and fail otherwise).
+int hadamard_reserve(spx_ham* h, int slots);
 int launch_apply(spx_ham* h, double scale, const double2* in, double2* out, bool accumulate);
 // kind: 1 = first order (out = 0.5 acc), 2 = second order (fused psi
 // a0 T0 + c1 T1 + c2 T2), 3 = general order (T = acc - out; psi += c T)

@@ -59,8 +59,16 @@ struct WhtArgs {
   const double* e[kMaxMany];    // E_lo[2^TB], E_hi[2^G], A[G][2^TB]
   double sdiag[kMaxMany];
   double a0[kMaxMany], c1r[kMaxMany], c1i[kMaxMany], cr[kMaxMany], ci[kMaxMany];
+  double pr[kMaxMany], pi[kMaxMany], qr[kMaxMany], qi[kMaxMany];
+  int defer[kMaxMany];          // accumulator batching (OrderWeights::defer)
   int next[kMaxMany];           // pass 3 also leaves the row transforms of its output
 };
+
+// q += (cr + i ci) t, the one form every accumulator update takes
+__device__ __forceinline__ void caxpy(double2& q, double cr, double ci, double2 t) {
+  q.x = fma(-ci, t.y, fma(cr, t.x, q.x));
+  q.y = fma(ci, t.x, fma(cr, t.y, q.y));
+}
 
 enum { K_APPLY = 0, K_ACCUM = 1, K_ORDER1 = 2, K_ORDER2 = 3, K_ORDERN = 4 };
 
@@ -232,6 +240,7 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   double2* __restrict__ psi = a.psi[v];
   const double* __restrict__ diag = a.diag[v];
   const double sdiag = a.sdiag[v], a0 = a.a0[v], c1r = a.c1r[v], c1i = a.c1i[v], cr = a.cr[v], ci = a.ci[v];
+  const int defer = KIND == K_ORDERN ? a.defer[v] : 0;   // uniform over the workgroup
   double2 xu[M], xw[M], xv[M], prev[M], pv[M];
   double dg[M];
 #pragma unroll
@@ -245,7 +254,8 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
     xv[m] = in[k];
     dg[m] = diag[k];
     if constexpr (KIND == K_ACCUM || KIND == K_ORDERN) prev[m] = out[k];
-    if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN) pv[m] = psi[k];
+    if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN)
+      if (defer != 1) pv[m] = psi[k];
   }
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
@@ -280,25 +290,23 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
       const double2 t2 = make_double2(acc.x - t0.x, acc.y - t0.y);
       o[m] = t2;
       double2 q = make_double2(a0 * t0.x, a0 * t0.y);
-      q.x += c1r * t1.x - c1i * t1.y;
-      q.y += c1r * t1.y + c1i * t1.x;
-      q.x += cr * t2.x - ci * t2.y;
-      q.y += cr * t2.y + ci * t2.x;
+      caxpy(q, c1r, c1i, t1);
+      caxpy(q, cr, ci, t2);
       pv[m] = q;
     } else {
       const double2 tn = make_double2(acc.x - prev[m].x, acc.y - prev[m].y);
       o[m] = tn;
-      double2 q = pv[m];
-      q.x += cr * tn.x - ci * tn.y;
-      q.y += cr * tn.y + ci * tn.x;
-      pv[m] = q;
+      if (defer == 3) caxpy(pv[m], a.qr[v], a.qi[v], prev[m]);
+      if (defer >= 2) caxpy(pv[m], a.pr[v], a.pi[v], xv[m]);
+      if (defer != 1) caxpy(pv[m], cr, ci, tn);
     }
   }
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
     const size_t k = base + t + m * NT;
     out[k] = o[m];
-    if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN) psi[k] = pv[m];
+    if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN)
+      if (defer != 1) psi[k] = pv[m];
   }
   if constexpr (KIND == K_ORDER1 || KIND == K_ORDER2 || KIND == K_ORDERN) {
     if (a.next[v]) {  // uniform over the workgroup
@@ -377,9 +385,24 @@ int hadamard_prepare(spx_ham* h, const std::vector<double>& coef) {
   spx_ctx* c = h->ctx;
   if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, (T + (size_t(1) << (kHadamardMaxSpins - kTB)) +
                                                    (kHadamardMaxSpins - kTB) * T) * sizeof(double)));
-  if (!h->d_wht) SPX_HIP(hipMalloc(&h->d_wht, 2 * kMaxMany * n * sizeof(double2)));
   SPX_HIP(hipMemcpyAsync(h->d_wht_e, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   SPX_HIP(hipStreamSynchronize(c->stream));  // `tab` is pageable and local
+  (void)n;
+  return hadamard_reserve(h, 1);
+}
+
+int hadamard_reserve(spx_ham* h, int slots) {
+  if (slots < 1 || slots > kMaxMany) return fail(SPX_EINVAL, "scratch slots out of range");
+  if (slots <= h->wht_slots) return SPX_OK;
+  const size_t n = static_cast<size_t>(1) << h->nspins;
+  if (h->d_wht) {
+    SPX_HIP(hipStreamSynchronize(h->ctx->stream));  // launches in flight may still use it
+    SPX_HIP(hipFree(h->d_wht));
+    h->d_wht = nullptr;
+    h->wht_slots = 0;
+  }
+  SPX_HIP(hipMalloc(&h->d_wht, 2 * static_cast<size_t>(slots) * n * sizeof(double2)));
+  h->wht_slots = slots;
   return SPX_OK;
 }
 
@@ -391,8 +414,8 @@ int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
   WhtArgs a{};
   for (unsigned i = 0; i < nvec; ++i) {
     const spx_ham* h = b.ham[i];
-    if (!h || h->nspins != L || !h->wht_ok || !h->d_wht || b.slot[i] < 0 || b.slot[i] >= kMaxMany)
-      return fail(SPX_EINVAL, "Hadamard launch mixes sizes or unprepared Hamiltonians");
+    if (!h || h->nspins != L || !h->wht_ok || !h->d_wht || b.slot[i] < 0 || b.slot[i] >= h->wht_slots)
+      return fail(SPX_EINVAL, "Hadamard launch mixes sizes, or its Hamiltonians lack prepared scratch");
     a.in[i] = b.in[i];
     a.out[i] = b.out[i];
     a.psi[i] = b.psi[i];
@@ -406,6 +429,12 @@ int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
     a.c1i[i] = b.w[i].c1i;
     a.cr[i] = b.w[i].cr;
     a.ci[i] = b.w[i].ci;
+    a.pr[i] = b.w[i].pr;
+    a.pi[i] = b.w[i].pi;
+    a.qr[i] = b.w[i].qr;
+    a.qi[i] = b.w[i].qi;
+    a.defer[i] = b.w[i].defer;
+    if (a.defer[i] < 0 || a.defer[i] > 3) return fail(SPX_EINVAL, "accumulator batching code out of range");
     a.next[i] = b.next[i] ? 1 : 0;
   }
   hipStream_t st = h0->ctx->stream;

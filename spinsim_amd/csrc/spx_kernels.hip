@@ -1468,6 +1468,7 @@ int launch_order_set(spx_ham* const* hs, int kind, const OrderLaunch& v, const O
   if (h->dense_ok) {
     KDArgs a{};
     for (unsigned i = 0; i < nvec; ++i) {
+      if (w[i].defer != 0) return fail(SPX_EINVAL, "batched accumulator updates need the Hadamard path");
       a.in[i] = v.in[i];
       a.out[i] = v.out[i];
       a.psi[i] = v.psi[i];
@@ -1483,9 +1484,9 @@ int launch_order_set(spx_ham* const* hs, int kind, const OrderLaunch& v, const O
     a.xcd_remap = xcd_remap(h);
     return launch_dense_args(h, k, a, nvec);
   }
-  for (unsigned i = 1; i < nvec; ++i)
+  for (unsigned i = 0; i < nvec; ++i)
     if (w[i].a0 != w[0].a0 || w[i].c1r != w[0].c1r || w[i].c1i != w[0].c1i || w[i].cr != w[0].cr ||
-        w[i].ci != w[0].ci)
+        w[i].ci != w[0].ci || w[i].defer != 0)
       return fail(SPX_EINVAL, "the table kernels take one set of weights per launch");
   return launch_order(h, kind, v, w[0].a0, w[0].c1r, w[0].c1i, w[0].cr, w[0].ci);
 }

@@ -115,7 +115,7 @@ struct Scratch {
 // nvec states share every timed launch (the width of a batched evolve).
 int tune(spx_ham* h, double bound, int nvec, std::string* report) {
   if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
-  if (nvec < 1 || nvec > kMaxMany) return fail(SPX_EINVAL, "states per launch must be in [1, 4]");
+  if (nvec < 1 || nvec > kMaxMany) return fail(SPX_EINVAL, "states per launch must be in [1, 8]");
   const double scale = -2.0 / bound;
   const Key key = shape_of(h, scale, nvec);
   spx_ctx* c = h->ctx;
@@ -138,6 +138,7 @@ int tune(spx_ham* h, double bound, int nvec, std::string* report) {
     h->terms_valid = false;
     if ((rc = ham_prepare(h, scale)) != SPX_OK) break;
     if ((s.dense == 1 && !h->dense_ok) || (s.dense == 2 && !h->wht_ok)) continue;
+    if (h->wht_ok && (rc = hadamard_reserve(h, nvec)) != SPX_OK) break;
     OrderLaunch v{};
     for (int i = 0; i < nvec; ++i) {
       v.in[i] = sc.v[3 * i];

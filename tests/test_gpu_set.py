@@ -1,6 +1,6 @@
 """Propagator.evolve_set: realisations (one Hamiltonian each, as in
 LocalRunner::run, src/ensemble/EnsembleRunner.cpp:27-88) stepped together,
-up to four per order launch, each with its own spectral bound and
+up to eight per order launch, each with its own spectral bound and
 expansion order. Every state must come out bitwise as when evolved alone."""
 import numpy as np
 import pytest
@@ -65,6 +65,13 @@ def test_uneven_orders_bitwise_equal_to_single_evolves(ctx, dense):
     assert all(n.startswith(prefix) for n in names), names
 
 
+def test_more_states_than_one_launch_takes(ctx):
+    L = 14
+    banner("evolve_set L=14, ten realisations (two launch groups)")
+    specs = [(i, 1.0 + 0.25 * (i % 3), False) for i in range(10)]
+    check_set(ctx, L, specs, 2, nsteps=2)
+
+
 def test_mixed_paths_fall_back_to_one_by_one(ctx):
     L = 14
     banner("evolve_set mixed paths L=14")
@@ -123,4 +130,4 @@ def test_batched_autotune_reports_the_hadamard_path(ctx):
     banner(report[:300])
     assert report.startswith("chosen ") and "hadamard: " in report
     with pytest.raises(spx.InvalidArgument):
-        h.autotune(h.spectral_bound(), nvec=5)
+        h.autotune(h.spectral_bound(), nvec=9)
