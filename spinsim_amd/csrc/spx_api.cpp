@@ -478,7 +478,9 @@ int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, siz
   if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
   SPX_TRY(set_device(p->ctx));
   SPX_TRY(ensure_scratch(p, nstates < kMaxMany ? nstates : kMaxMany));
-  SPX_TRY(autotune_before_evolve(h, bound, static_cast<int>(nstates < kMaxMany ? nstates : kMaxMany)));
+  const int width = static_cast<int>(nstates < kMaxMany ? nstates : kMaxMany);
+  SPX_TRY(autotune_before_evolve(h, bound, width));
+  set_launch_width(h, width);
   SPX_TRY(ham_prepare(h, -2.0 / bound));
   if (h->wht_ok) SPX_TRY(hadamard_reserve(h, static_cast<int>(nstates < kMaxMany ? nstates : kMaxMany)));
   // Without autonormalisation every step is the same sequence of launches
@@ -651,6 +653,7 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
   const int width = static_cast<int>(idx.size() < static_cast<size_t>(kMaxMany) ? idx.size() : kMaxMany);
   for (size_t i : idx) {
     SPX_TRY(autotune_before_evolve(hams[i], bounds[i], width));
+    set_launch_width(hams[i], width);
     SPX_TRY(ham_prepare(hams[i], -2.0 / bounds[i]));
   }
   for (size_t i : idx)

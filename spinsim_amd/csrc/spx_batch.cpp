@@ -38,67 +38,94 @@ size_t obs_width(const std::vector<int>& codes) {
   return w;
 }
 
-struct Walker {
-  spx_batch* b;
+// Realisations that share a program walk it in lockstep as a group: every
+// evolve of the program is one evolve_set over each member's state and
+// reference (under that member's Hamiltonian), so their Chebyshev orders
+// share launches; pulses and measurements are per member. Each member sees
+// exactly the arithmetic of walking it alone.
+struct Member {
   const Realisation* r;
   spx_ham* h;
   spx_state* state;
   spx_state* ref;
-  spx_prop* prop;
-  double time = 0.0;
-  double* times;
+  double* times;   // null unless this member records the time column
   double* values;
   size_t row = 0;
+};
 
-  int measure() {
-    double* out = values + row * b->width;
+struct Walker {
+  spx_batch* b;
+  std::vector<Member>& m;
+  spx_prop* prop;
+  double time = 0.0;
+
+  int measure(Member& x) {
+    double* out = x.values + x.row * b->width;
     size_t at = 0;
     for (size_t o = 0; o < b->obs_codes.size(); ++o) {
       const int code = b->obs_codes[o];
       if (code <= 2) {
-        SPX_TRY(spx_measure_corr(code, state, ref, &out[at], &out[at + 1]));
+        SPX_TRY(spx_measure_corr(code, x.state, x.ref, &out[at], &out[at + 1]));
         at += 2;
       } else if (code <= 5) {
-        SPX_TRY(spx_measure_mag(code - 3, static_cast<size_t>(b->obs_spins[o]), state, &out[at]));
+        SPX_TRY(spx_measure_mag(code - 3, static_cast<size_t>(b->obs_spins[o]), x.state, &out[at]));
         at += 1;
       } else if (code == 6) {
-        SPX_TRY(spx_state_norm(state, &out[at]));
-        SPX_TRY(spx_state_norm(ref, &out[at + 1]));
+        SPX_TRY(spx_state_norm(x.state, &out[at]));
+        SPX_TRY(spx_state_norm(x.ref, &out[at + 1]));
         at += 2;
       } else {
-        SPX_TRY(spx_measure_energy(h, state, &out[at]));
+        SPX_TRY(spx_measure_energy(x.h, x.state, &out[at]));
         at += 1;
       }
     }
-    if (times) times[row] = time;
-    ++row;
+    if (x.times) x.times[x.row] = time;
+    ++x.row;
+    return SPX_OK;
+  }
+
+  int evolve(const ProgOp& op) {
+    const size_t g = m.size();
+    std::vector<std::vector<double>> saved(g);
+    int rc = SPX_OK;
+    for (size_t k = 0; k < g && rc == SPX_OK && (!op.fields.empty() || op.uniform); ++k) {
+      const size_t L = m[k].r->nspins;
+      std::vector<double> tmp;
+      const std::vector<double>& f = step_fields(op, m[k].r->fields, L, tmp);
+      if (f.size() != 3 * L) {
+        rc = fail(SPX_EINVAL, "step field table does not match the number of spins");
+        break;
+      }
+      saved[k].resize(3 * L);
+      for (size_t s = 0; s < L && rc == SPX_OK; ++s) rc = spx_ham_get_field(m[k].h, s, &saved[k][3 * s]);
+      for (size_t s = 0; s < L && rc == SPX_OK; ++s) rc = spx_ham_set_field(m[k].h, s, f[3 * s], f[3 * s + 1], f[3 * s + 2]);
+    }
+    if (rc == SPX_OK) {
+      std::vector<spx_ham*> hs;
+      std::vector<spx_state*> ss;
+      for (Member& x : m) {
+        hs.push_back(x.h);
+        hs.push_back(x.h);
+        ss.push_back(x.state);
+        ss.push_back(x.ref);
+      }
+      rc = spx_prop_evolve_set(prop, hs.data(), op.mode, ss.data(), ss.size(), op.tau, op.nsteps, op.autonorm);
+    }
+    for (size_t k = 0; k < g; ++k)
+      for (size_t s = 0; 3 * s < saved[k].size(); ++s)
+        spx_ham_set_field(m[k].h, s, saved[k][3 * s], saved[k][3 * s + 1], saved[k][3 * s + 2]);
+    SPX_TRY(rc);
+    time += op.tau * static_cast<double>(op.nsteps);
     return SPX_OK;
   }
 
   int run(size_t beg, size_t end) {
     const std::vector<ProgOp>& ops = b->program.ops;
-    const size_t L = r->nspins;
+    const size_t L = m.front().r->nspins;
     for (size_t i = beg; i < end; ++i) {
       const ProgOp& op = ops[i];
       if (op.kind == OP_EVOLVE) {
-        std::vector<double> saved, tmp;
-        if (!op.fields.empty() || op.uniform) {
-          const std::vector<double>& f = step_fields(op, r->fields, L, tmp);
-          if (f.size() != 3 * L)
-            return fail(SPX_EINVAL, "step field table does not match the number of spins");
-          saved.resize(3 * L);
-          for (size_t s = 0; s < L; ++s) {
-            SPX_TRY(spx_ham_get_field(h, s, &saved[3 * s]));
-            SPX_TRY(spx_ham_set_field(h, s, f[3 * s], f[3 * s + 1], f[3 * s + 2]));
-          }
-        }
-        spx_state* both[2] = {state, ref};
-        int rc = spx_prop_evolve_many(prop, h, op.mode, both, 2, op.tau, op.nsteps, op.autonorm);
-        if (!saved.empty())
-          for (size_t s = 0; s < L; ++s)
-            spx_ham_set_field(h, s, saved[3 * s], saved[3 * s + 1], saved[3 * s + 2]);
-        SPX_TRY(rc);
-        time += op.tau * static_cast<double>(op.nsteps);
+        SPX_TRY(evolve(op));
       } else if (op.kind == OP_PULSE) {
         std::vector<uint32_t> all;
         const uint32_t* t = op.targets.data();
@@ -108,10 +135,12 @@ struct Walker {
           t = all.data();
           nt = all.size();
         }
-        SPX_TRY(spx_rotate(state, op.axis[0], op.axis[1], op.axis[2], op.angle, t, nt));
-        SPX_TRY(spx_rotate(ref, op.axis[0], op.axis[1], op.axis[2], op.angle, t, nt));
+        for (Member& x : m) {
+          SPX_TRY(spx_rotate(x.state, op.axis[0], op.axis[1], op.axis[2], op.angle, t, nt));
+          SPX_TRY(spx_rotate(x.ref, op.axis[0], op.axis[1], op.axis[2], op.angle, t, nt));
+        }
       } else if (op.kind == OP_MEASURE) {
-        SPX_TRY(measure());
+        for (Member& x : m) SPX_TRY(measure(x));
       } else if (op.kind == OP_REPEAT) {
         for (size_t k = 0; k < op.count; ++k) SPX_TRY(run(i + 1, op.match));
         i = op.match;
@@ -123,88 +152,118 @@ struct Walker {
 
 }  // namespace
 
-// Per-realisation device objects, one kernel sequence per program step.
-// Device objects of one worker, reused across its realisations (allocation
-// and release synchronise the device, which would serialise the workers);
+// Realisations per lockstep group: a state and a reference each, so one
+// group fills an order launch.
+constexpr size_t kGroup = kMaxMany / 2;
+
+// Device objects of one worker, reused across its groups (allocation and
+// release synchronise the device, which would serialise the workers);
 // rebuilt only when L changes.
 struct WorkerObjects {
   spx_ctx* c = nullptr;
   size_t L = 0;
-  spx_ham* h = nullptr;
-  spx_state* state = nullptr;
-  spx_state* ref = nullptr;
+  spx_ham* h[kGroup] = {};
+  spx_state* state[kGroup] = {};
+  spx_state* ref[kGroup] = {};
   spx_prop* prop = nullptr;
 
   void release() {
     spx_prop_destroy(prop);
-    spx_state_destroy(ref);
-    spx_state_destroy(state);
-    spx_ham_destroy(h);
     prop = nullptr;
-    ref = state = nullptr;
-    h = nullptr;
+    for (size_t k = 0; k < kGroup; ++k) {
+      spx_state_destroy(ref[k]);
+      spx_state_destroy(state[k]);
+      spx_ham_destroy(h[k]);
+      ref[k] = state[k] = nullptr;
+      h[k] = nullptr;
+    }
     L = 0;
   }
   ~WorkerObjects() { release(); }
 
   int prepare(size_t nspins, double epsilon) {
     if (L == nspins) {
-      // same size: clear the Hamiltonian's fields and couplings (its device
+      // same size: clear the Hamiltonians' fields and couplings (their device
       // caches are keyed on them and rebuild on first use)
-      std::fill(h->hx.begin(), h->hx.end(), 0.0);
-      std::fill(h->hy.begin(), h->hy.end(), 0.0);
-      std::fill(h->hz.begin(), h->hz.end(), 0.0);
-      h->lo.clear();
-      h->hi.clear();
-      h->jx.clear();
-      h->jy.clear();
-      h->jz.clear();
-      h->terms_valid = false;
-      h->sched_set = false;
+      for (spx_ham* x : h) {
+        std::fill(x->hx.begin(), x->hx.end(), 0.0);
+        std::fill(x->hy.begin(), x->hy.end(), 0.0);
+        std::fill(x->hz.begin(), x->hz.end(), 0.0);
+        x->lo.clear();
+        x->hi.clear();
+        x->jx.clear();
+        x->jy.clear();
+        x->jz.clear();
+        x->terms_valid = false;
+        x->sched_set = false;
+      }
       return SPX_OK;
     }
     release();
-    SPX_TRY(spx_ham_create(c, nspins, &h));
-    SPX_TRY(spx_state_create(c, nspins, &state));
-    SPX_TRY(spx_state_create(c, nspins, &ref));
+    for (size_t k = 0; k < kGroup; ++k) {
+      SPX_TRY(spx_ham_create(c, nspins, &h[k]));
+      SPX_TRY(spx_state_create(c, nspins, &state[k]));
+      SPX_TRY(spx_state_create(c, nspins, &ref[k]));
+    }
     SPX_TRY(spx_prop_create(c, nspins, epsilon, &prop));
     L = nspins;
     return SPX_OK;
   }
 };
 
-// One realisation on a worker's objects, rows written at `values`.
-static int run_one(spx_batch* b, WorkerObjects& o, const Realisation& r, double* times, double* values) {
-  SPX_TRY(o.prepare(r.nspins, b->epsilon));
-  for (size_t s = 0; s < r.nspins; ++s)
-    SPX_TRY(spx_ham_set_field(o.h, s, r.fields[3 * s], r.fields[3 * s + 1], r.fields[3 * s + 2]));
-  for (size_t p = 0; p < r.lo.size(); ++p)
-    SPX_TRY(spx_ham_set_coupling(o.h, r.lo[p], r.hi[p], r.jx[p], r.jy[p], r.jz[p]));
-  SPX_TRY(spx_state_upload(o.state, r.psi0.data()));
-  SPX_TRY(spx_apply_total_spin(b->reference_axis, o.state, o.ref));
-  Walker w{b, &r, o.h, o.state, o.ref, o.prop, 0.0, times, values};
+// One group (realisations idx[0..n), all of one size) on a worker's
+// objects; realisation i writes its rows at values_of(i), and the one whose
+// index is `timed` also the time column.
+template <typename Rows>
+static int run_group(spx_batch* b, WorkerObjects& o, const size_t* idx, size_t n, size_t timed, double* times,
+                     Rows values_of) {
+  SPX_TRY(o.prepare(b->items[idx[0]].nspins, b->epsilon));
+  std::vector<Member> m;
+  for (size_t k = 0; k < n; ++k) {
+    const Realisation& r = b->items[idx[k]];
+    for (size_t s = 0; s < r.nspins; ++s)
+      SPX_TRY(spx_ham_set_field(o.h[k], s, r.fields[3 * s], r.fields[3 * s + 1], r.fields[3 * s + 2]));
+    for (size_t p = 0; p < r.lo.size(); ++p)
+      SPX_TRY(spx_ham_set_coupling(o.h[k], r.lo[p], r.hi[p], r.jx[p], r.jy[p], r.jz[p]));
+    SPX_TRY(spx_state_upload(o.state[k], r.psi0.data()));
+    SPX_TRY(spx_apply_total_spin(b->reference_axis, o.state[k], o.ref[k]));
+    m.push_back(Member{&r, o.h[k], o.state[k], o.ref[k], idx[k] == timed ? times : nullptr, values_of(idx[k])});
+  }
+  Walker w{b, m, o.prop};
   SPX_TRY(w.run(0, b->program.ops.size()));
   return spx_ctx_synchronize(o.c);
 }
 
-// Realisations one after another on the batch's context, or, with several
-// workers, realisation i on worker i mod W, each worker a host thread with
-// its own context (stream), as LocalRunner's pool runs jobs concurrently;
-// the kernels of different realisations then overlap on the device. Every
-// realisation runs the same kernels on its own rows, so the output does not
-// depend on the number of workers.
+// Realisations in lockstep groups of up to kGroup of one size (by size, in
+// index order within a size), the groups one after another on the batch's
+// context or, with several workers, group g on worker g mod W, each worker
+// a host thread with its own context (stream), as LocalRunner's pool runs
+// jobs concurrently. Every realisation sees the same arithmetic on its own
+// rows, so the output depends neither on the grouping nor on the workers.
 int batch_run_streaming(spx_batch* b, size_t first, size_t count, double* times, double* values) {
   const size_t points = program_measurements(b->program, 0, b->program.ops.size());
   static const int env_workers = [] {
     const char* e = std::getenv("SPX_BATCH_WORKERS");
     return e ? std::atoi(e) : 4;
   }();
-  const size_t workers = std::min<size_t>(count, static_cast<size_t>(std::max(1, std::min(env_workers, 8))));
+  std::vector<size_t> idx(count);
+  for (size_t i = 0; i < count; ++i) idx[i] = first + i;
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](size_t x, size_t y) { return b->items[x].nspins < b->items[y].nspins; });
+  std::vector<std::pair<size_t, size_t>> groups;   // [begin, end) into idx
+  for (size_t g = 0; g < count;) {
+    size_t e = g + 1;
+    while (e < count && e - g < kGroup && b->items[idx[e]].nspins == b->items[idx[g]].nspins) ++e;
+    groups.emplace_back(g, e);
+    g = e;
+  }
+  auto values_of = [&](size_t i) { return values + (i - first) * points * b->width; };
+  const size_t workers =
+      std::min<size_t>(groups.size(), static_cast<size_t>(std::max(1, std::min(env_workers, 8))));
   if (workers <= 1) {
     WorkerObjects o;
     o.c = b->ctx;
-    for (size_t i = first; i < first + count; ++i)
-      SPX_TRY(run_one(b, o, b->items[i], (i == first) ? times : nullptr, values + (i - first) * points * b->width));
+    for (const auto& [g, e] : groups) SPX_TRY(run_group(b, o, &idx[g], e - g, first, times, values_of));
     return SPX_OK;
   }
   std::vector<int> rcs(workers, SPX_OK);
@@ -217,8 +276,9 @@ int batch_run_streaming(spx_batch* b, size_t first, size_t count, double* times,
       {
         WorkerObjects o;
         o.c = c;
-        for (size_t i = first + w; rc == SPX_OK && i < first + count; i += workers)
-          rc = run_one(b, o, b->items[i], (i == first) ? times : nullptr, values + (i - first) * points * b->width);
+        for (size_t gi = w; rc == SPX_OK && gi < groups.size(); gi += workers)
+          rc = run_group(b, o, &idx[groups[gi].first], groups[gi].second - groups[gi].first, first, times,
+                         values_of);
         if (rc != SPX_OK) msgs[w] = spx_last_error();
       }
       if (c) spx_ctx_destroy(c);

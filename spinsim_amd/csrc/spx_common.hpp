@@ -118,6 +118,7 @@ struct spx_ham {
   double* d_tab = nullptr;   // diagonal build table
   double2* d_scratch = nullptr;  // H psi for the energy observable
   size_t tab_cap = 0;
+  int width = 1;                 // states sharing each order launch of the latest evolve
   spx::Schedule sched;           // launch schedule (autotuned or default)
   bool sched_set = false;        // chosen by the autotuner (or its cache)
 };
@@ -224,6 +225,12 @@ int tile_bits_for(const spx_ham* h);
 int block_threads(const spx_ham* h);
 int xcd_remap(const spx_ham* h);
 int dense_tile_bits(const spx_ham* h);
+// The flip-flop-only path used when neither the schedule nor SPX_DENSE names
+// one (1 all-pairs gathers, 2 Hadamard transforms), by L and launch width;
+// set_launch_width records an evolve's width and invalidates the prepared
+// tables when that changes the choice.
+int default_dense_path(size_t L, int width);
+void set_launch_width(spx_ham* h, int width);
 // the autotuner's process-wide mode and its hook in the propagator
 int autotune_before_evolve(spx_ham* h, double bound, int nvec);
 

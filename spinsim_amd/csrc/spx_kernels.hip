@@ -994,17 +994,34 @@ int xcd_remap(const spx_ham* h) {
 // no transverse field and no flip-both weight at this scale; L = 14..18 for
 // the all-pairs gather kernel (dense = 1), L = 14..20 for the Hadamard
 // sandwich (dense = 2). SPX_DENSE=0 (or a schedule with dense = 0) switches
-// both off.
+// both off; without either, default_dense_path chooses.
 int dense_tile_bits(const spx_ham* h) {
   static const int env = env_int("SPX_DENSE_TB", 10);
   const int tb = h->sched.dense_tb > 0 ? h->sched.dense_tb : env;
   return tb == 9 ? 9 : 10;
 }
 
+// Measured per state and order (dipolar couplings, µs): L = 16, one state
+// 13.6 gathers / 13.1 Hadamard, four per launch 4.3 / 5.5; L = 17, four
+// 9.4 / 7.6; L = 18, one 17.4 / 23.1, eight 16.7 / 8.5; from L = 19 only the
+// Hadamard path exists.
+int default_dense_path(size_t L, int width) {
+  if (L >= 19) return 2;
+  if (L >= 17 && width >= 2) return 2;
+  return 1;
+}
+
+void set_launch_width(spx_ham* h, int width) {
+  if (h->width == width) return;
+  const bool changes = default_dense_path(h->nspins, h->width) != default_dense_path(h->nspins, width);
+  h->width = width;
+  if (changes) h->terms_valid = false;
+}
+
 static int prepare_dense(spx_ham* h, double scale) {
-  static const int env = env_int("SPX_DENSE", 1);
-  const int enabled = h->sched.dense >= 0 ? h->sched.dense : env;
+  static const int env = env_int("SPX_DENSE", -1);
   const size_t L = h->nspins;
+  const int enabled = h->sched.dense >= 0 ? h->sched.dense : env >= 0 ? env : default_dense_path(L, h->width);
   h->dense_ok = false;
   h->wht_ok = false;
   const bool pairs = enabled == 1 && L >= 14 && L <= 18;
