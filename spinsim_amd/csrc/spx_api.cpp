@@ -362,7 +362,7 @@ int spx_ham_schedule(const spx_ham* h, char* buf, size_t len) {
   std::string name;
   if (!h->terms_valid) name = "unprepared";
   else if (h->wht_ok)
-    name = "k_wht_rows_fwd + k_wht_cols<" + std::to_string(h->nspins - 10) + "> + k_wht_rows_bwd<K_ORDERN>";
+    name = "k_wht_cols<" + std::to_string(h->nspins - 10) + "> + k_wht_rows_bwd<K_ORDERN>";
   else if (h->dense_ok)
     name = "k_dense<K_ORDERN, " + std::to_string(h->nspins) + ", " + std::to_string(dense_tile_bits(h)) + ">";
   else if (h->terms_variant == 3)

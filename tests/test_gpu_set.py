@@ -61,7 +61,7 @@ def test_uneven_orders_bitwise_equal_to_single_evolves(ctx, dense):
     orders, names = check_set(ctx, L, specs, dense)
     banner(f"orders {orders}")
     assert 1 in orders and len(set(orders)) >= 3
-    prefix = "k_wht_rows_fwd" if dense == 2 else "k_dense"
+    prefix = "k_wht_" if dense == 2 else "k_dense"
     assert all(n.startswith(prefix) for n in names), names
 
 
@@ -131,3 +131,23 @@ def test_batched_autotune_reports_the_hadamard_path(ctx):
     assert report.startswith("chosen ") and "hadamard: " in report
     with pytest.raises(spx.InvalidArgument):
         h.autotune(h.spectral_bound(), nvec=9)
+
+
+def test_default_path_follows_size_and_launch_width(ctx):
+    """Without a schedule: all-pairs gathers for one state per launch at
+    L=18, the Hadamard path when states share launches, and back."""
+    L = 18
+    banner("default flip-flop path by launch width, L=18")
+    fields, c = realisation(L, 0)
+    h = spx.Hamiltonian(ctx, L)
+    h.set_fields(fields)
+    h.set_couplings(c.lo, c.hi, c.jx, c.jy, c.jz)
+    prop = spx.Propagator(ctx, L, 1e-12)
+    s = spx.StateVector(ctx, L, O.random_state(L, 1))
+    t = spx.StateVector(ctx, L, O.random_state(L, 2))
+    prop.evolve(h, spx.REAL, s, 0.05, 1)
+    assert h.kernel_name().startswith("k_dense"), h.kernel_name()
+    prop.evolve_many(h, spx.REAL, [s, t], 0.05, 1)
+    assert h.kernel_name().startswith("k_wht_"), h.kernel_name()
+    prop.evolve(h, spx.REAL, s, 0.05, 1)
+    assert h.kernel_name().startswith("k_dense"), h.kernel_name()
