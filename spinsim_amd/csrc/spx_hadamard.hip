@@ -164,61 +164,120 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_fwd(WhtArgs a) {
   }
 }
 
+// Butterflies of one round on the thread's eight values v[r] (r = its three
+// register bits): stages s in [S0, S1).
+template <int S0, int S1>
+__device__ __forceinline__ void butterflies(double2 (&v)[8]) {
+#pragma unroll
+  for (int s = S0; s < S1; ++s) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (r & (1 << s)) continue;
+      const double2 a = v[r], b = v[r | (1 << s)];
+      v[r] = make_double2(a.x + b.x, a.y + b.y);
+      v[r | (1 << s)] = make_double2(a.x - b.x, a.y - b.y);
+    }
+  }
+}
+
+// Index of value r of a thread with free bits u in a round whose register
+// bits are [LO, LO + 3).
+template <int LO>
+__device__ __forceinline__ uint32_t round_index(uint32_t u, int r) {
+  return ((u >> LO) << (LO + 3)) | (u & ((1u << LO) - 1u)) | (static_cast<uint32_t>(r) << LO);
+}
+
+// Column-pass round k, register bits [lo, lo + 3): the forward transform
+// takes bits 0, 3, 6, ... (the last round aligned to the top, its lower
+// stages already done); the inverse starts in the forward's last layout with
+// all three stages and walks down to bit 0.
+constexpr int fwd_lo(int G, int k) { return 3 * k + 3 <= G ? 3 * k : G - 3; }
+constexpr int fwd_s0(int G, int k) { return 3 * k - fwd_lo(G, k); }
+constexpr int inv_rounds(int G) { return 1 + (G - 3 + 2) / 3; }
+constexpr int inv_top(int G, int k) { return k == 0 ? G : G - 3 - 3 * (k - 1); }   // bits [lo, top)
+constexpr int inv_lo(int G, int k) { return k == 0 ? G - 3 : (inv_top(G, k) - 3 > 0 ? inv_top(G, k) - 3 : 0); }
+
+template <int LO>
+__device__ __forceinline__ void col_store(double2* S, uint32_t u, uint32_t c, const double2 (&v)[8]) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) S[(round_index<LO>(u, r) << 3) | c] = v[r];
+}
+template <int LO>
+__device__ __forceinline__ void col_load(const double2* S, uint32_t u, uint32_t c, double2 (&v)[8]) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = S[(round_index<LO>(u, r) << 3) | c];
+}
+
+// forward rounds 1 .. NF-1 (round 0 runs on the values loaded from memory)
+template <int G, int K>
+__device__ __forceinline__ void col_forward(double2* S, uint32_t u, uint32_t c, double2 (&v)[8]) {
+  constexpr int NF = (G + 2) / 3;
+  if constexpr (K < NF) {
+    col_store<fwd_lo(G, K - 1)>(S, u, c, v);
+    __syncthreads();
+    col_load<fwd_lo(G, K)>(S, u, c, v);
+    butterflies<fwd_s0(G, K), 3>(v);
+    col_forward<G, K + 1>(S, u, c, v);
+  }
+}
+// inverse rounds 1 .. NI-1 (round 0 runs in registers after the E product)
+template <int G, int K>
+__device__ __forceinline__ void col_inverse(double2* S, uint32_t u, uint32_t c, double2 (&v)[8]) {
+  if constexpr (K < inv_rounds(G)) {
+    col_store<inv_lo(G, K - 1)>(S, u, c, v);
+    __syncthreads();
+    col_load<inv_lo(G, K)>(S, u, c, v);
+    butterflies<0, inv_top(G, K) - inv_lo(G, K)>(v);
+    col_inverse<G, K + 1>(S, u, c, v);
+  }
+}
+
 // Pass 2: one workgroup per group of 8 columns (low indices) and per scratch
-// vector; 2^G threads hold 8 columns x 2^G rows in LDS, transform the rows,
-// multiply by E and transform again. Each row segment is 128 contiguous bytes.
+// vector; 2^G threads (column c = t mod 8, free row bits u = t / 8) hold 8
+// columns x 2^G rows and transform the rows in place: the first round on the
+// values as loaded (register bits 0-2: 128-byte row segments), then through
+// LDS; the product with E and the first inverse round in registers; the last
+// inverse round (register bits 0-2 again) straight back to memory.
 template <int G>
 __global__ __launch_bounds__(1 << G) void k_wht_cols(WhtArgs a) {
-  constexpr uint32_t T = 1u << kTB, NT = 1u << G, NV = kCols << G, M = NV / NT;
+  static_assert(inv_lo(G, inv_rounds(G) - 1) == 0, "the inverse ends at bit 0");
+  constexpr uint32_t T = 1u << kTB, NT = 1u << G;
+  constexpr int LT = G - 3;   // register bits of the forward's last round
   extern __shared__ double2 smem[];
-  const uint32_t t = threadIdx.x;
+  const uint32_t t = threadIdx.x, c = t & 7u, u = t >> 3;
   const int v = static_cast<int>(blockIdx.y >> 1);
   double2* __restrict__ X = (blockIdx.y & 1u) ? a.w[v] : a.u[v];
-  const uint32_t col0 = blockIdx.x * kCols;
-  double2 x[M];
+  const uint32_t col = blockIdx.x * kCols + c;
+  double2 x[8];
 #pragma unroll
-  for (uint32_t m = 0; m < M; ++m) {
-    const uint32_t f = t + m * NT;
-    x[m] = X[(static_cast<size_t>(f >> 3) << kTB) | (col0 + (f & 7u))];
-  }
-  // E at this thread's elements: column c = t mod 8 (NT is a multiple of 8),
-  // rows (t >> 3) + m NT / 8
+  for (int r = 0; r < 8; ++r) x[r] = X[(static_cast<size_t>(round_index<0>(u, r)) << kTB) | col];
+  // E at the elements of the forward's last layout: rows u | r << LT
   const double* __restrict__ tab = a.e[v];
-  const uint32_t c = col0 + (t & 7u);
-  double aj[G];
-  const double elo = tab[c];
+  double e[8];
+  {
+    double part = tab[col];
 #pragma unroll
-  for (int j = 0; j < G; ++j) aj[j] = tab[T + NT + j * T + c];
-  double ehi[M];
+    for (int j = 0; j < LT; ++j) part += ((u >> j) & 1u) ? -tab[T + NT + j * T + col] : tab[T + NT + j * T + col];
+    double top[3];
 #pragma unroll
-  for (uint32_t m = 0; m < M; ++m) ehi[m] = tab[T + (t >> 3) + m * (NT >> 3)];
+    for (int j = 0; j < 3; ++j) top[j] = tab[T + NT + (LT + j) * T + col];
 #pragma unroll
-  for (uint32_t m = 0; m < M; ++m) smem[t + m * NT] = x[m];
-  double s[M];
+    for (int r = 0; r < 8; ++r) {
+      double s = part + tab[T + round_index<LT>(u, r)];
 #pragma unroll
-  for (uint32_t m = 0; m < M; ++m) {
-    const uint32_t row = (t >> 3) + m * (NT >> 3);
-    double e = elo + ehi[m];
-#pragma unroll
-    for (int j = 0; j < G; ++j) e += ((row >> j) & 1u) ? -aj[j] : aj[j];
-    s[m] = e;
+      for (int j = 0; j < 3; ++j) s += ((r >> j) & 1) ? -top[j] : top[j];
+      e[r] = s;
+    }
   }
-  __syncthreads();
-  wht_rounds<G, 3>(smem, t, ColSlot{});
+  butterflies<0, 3>(x);
+  col_forward<G, 1>(smem, u, c, x);
 #pragma unroll
-  for (uint32_t m = 0; m < M; ++m) {
-    const double2 y = smem[t + m * NT];
-    smem[t + m * NT] = make_double2(s[m] * y.x, s[m] * y.y);
-  }
-  __syncthreads();
-  wht_rounds<G, 3>(smem, t, ColSlot{});
+  for (int r = 0; r < 8; ++r) x[r] = make_double2(e[r] * x[r].x, e[r] * x[r].y);
+  butterflies<0, 3>(x);
+  col_inverse<G, 1>(smem, u, c, x);
 #pragma unroll
-  for (uint32_t m = 0; m < M; ++m) x[m] = smem[t + m * NT];
-#pragma unroll
-  for (uint32_t m = 0; m < M; ++m) {
-    const uint32_t f = t + m * NT;
-    X[(static_cast<size_t>(f >> 3) << kTB) | (col0 + (f & 7u))] = x[m];
-  }
+  for (int r = 0; r < 8; ++r) X[(static_cast<size_t>(round_index<0>(u, r)) << kTB) | col] = x[r];
+  (void)NT;
 }
 
 // Pass 3: the row transforms back, the diagonal term and the epilogue of
