@@ -141,3 +141,19 @@ def test_falls_back_when_not_eligible(ctx, L, transverse, aniso):
     s, o = spx.StateVector(ctx, L, O.random_state(L, 9)), spx.StateVector(ctx, L)
     h.apply_physical(s, o)
     assert not h.kernel_name().startswith("k_wht"), h.kernel_name()
+
+
+def test_imaginary_time_matches_the_table_kernels(ctx):
+    L = 15
+    banner("hadamard imaginary time vs table kernels L=15")
+    fields, c = dipolar_system(L, 6)
+    psi0 = O.random_state(L, 8)
+    prop = spx.Propagator(ctx, L, 1e-12)
+    got = {}
+    for dense in (2, 0):
+        h = device_ham(ctx, L, fields, c, dense)
+        s = spx.StateVector(ctx, L, psi0)
+        prop.evolve(h, spx.IMAGINARY, s, 0.05, 3, autonormalize=True)
+        got[dense] = (s.download(), h.kernel_name())
+    assert got[2][1].startswith("k_wht_") and not got[0][1].startswith("k_wht")
+    assert float(np.max(np.abs(got[2][0] - got[0][0]))) < 1e-12 * max(1.0, float(np.max(np.abs(got[0][0]))))

@@ -428,12 +428,13 @@ def test_imaginary_time_relaxes_to_the_ground_state(ctx):
     assert overlap > 1.0 - 1e-9
 
 
-@pytest.mark.parametrize("L", [9, 14, 16])
+@pytest.mark.parametrize("L", [9, 14, 16, 18])
 def test_step_graph_replays_equal_eager_launches_bitwise(ctx, L):
     """Evolve calls that record one step as a HIP graph and replay it agree
     bitwise with the same calls launched order by order, over several calls
     with eager work (gates, measurements, host syncs) between them; also for
-    several states at once and in imaginary time."""
+    several states at once and in imaginary time. At L = 18 the two-state
+    calls take the Hadamard path (chained order launches)."""
     banner(f"graph replay vs eager L={L}")
     fields, c = dipolar_system(L, 21, transverse=(L == 9))
     h = device_ham(ctx, L, fields, c)
