@@ -1001,13 +1001,17 @@ int dense_tile_bits(const spx_ham* h) {
   return tb == 9 ? 9 : 10;
 }
 
-// Measured per state and order (dipolar couplings, µs): L = 16, one state
-// 13.6 gathers / 13.1 Hadamard, four per launch 4.3 / 5.5; L = 17, four
-// 9.4 / 7.6; L = 18, one 17.4 / 23.1, eight 16.7 / 8.5; from L = 19 only the
-// Hadamard path exists.
+// Measured per launch (dipolar couplings, all states of the launch, µs,
+// gathers / Hadamard): L = 15, width 8: 12.5 / 18.8; L = 16, width 2:
+// 11.1 / 13.7, width 4: 17.3 / 22.2, width 8: 28.7 / 22.6; L = 17, width 2:
+// 15.9 / 19.3, width 4: 37.7 / 30.4, width 8: 55.7 / 36.5; L = 18, width 1:
+// 17.4 / 23.1, width 2: 35.1 / 24.4; from L = 19 only the Hadamard path
+// exists.
 int default_dense_path(size_t L, int width) {
   if (L >= 19) return 2;
-  if (L >= 17 && width >= 2) return 2;
+  if (L == 18) return width >= 2 ? 2 : 1;
+  if (L == 17) return width >= 4 ? 2 : 1;
+  if (L == 16) return width >= 8 ? 2 : 1;
   return 1;
 }
 
