@@ -127,10 +127,10 @@ int spx_ham_autotune_batch(spx_ham* h, double bound, int nvec, char* report, siz
  * tuning it first if absent. Schedules reorder floating-point additions only. */
 int spx_autotune_mode(int mode);
 /* Fix a schedule by hand; 0 / -1 fields keep the defaults. `dense`: -1
- * default, 0 table kernels only, 1 the all-pairs gather kernel (L = 14..18),
- * 2 the Hadamard-sandwich path (L = 14..20); both need flip-flop-only
- * couplings (no transverse field, jx = jy) and otherwise fall back to the
- * table kernels. */
+ * default, 0 table kernels only, 1 the all-pairs gather kernel (L = 14..18,
+ * flip-flop couplings only: no transverse field and jx = jy, else the table
+ * kernels), 2 the Walsh-Hadamard path (L = 14..20, any couplings and
+ * fields). */
 int spx_ham_set_schedule(spx_ham* h, int tile_bits, int block, int xcd, int dense, int dense_tile_bits);
 
 /* ------------------------------------------------ Propagator (a8)

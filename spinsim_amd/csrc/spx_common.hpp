@@ -109,7 +109,8 @@ struct spx_ham {
   bool dense_ok = false;         // specialised all-pairs kernel usable
   double* d_coef = nullptr;      // its weights
   bool wht_ok = false;           // Hadamard-sandwich flip-flop path selected
-  double* d_wht_e = nullptr;     // its Hadamard-basis diagonal: E_lo[2^10], E_hi[2^(L-10)], A[L-10][2^10]
+  double* d_wht_e = nullptr;     // its Hadamard-basis diagonals A then B, factored (spx_hadamard.hip)
+  size_t wht_tab = 0;            // doubles per factor table (B starts there)
   double2* d_wht = nullptr;      // its transform scratch: wht_slots pairs of 2^L vectors
   int wht_slots = 0;
   double terms_scale = 0.0;
@@ -155,8 +156,9 @@ struct OrderLaunch {
 
 int ham_prepare(spx_ham* h, double scale);   // diagonal + flip table at scale
 
-// Flip-flop-only Hamiltonians through Walsh-Hadamard transforms
-// (spx_hadamard.hip): three passes per application, any L in range.
+// The off-diagonal part of H through Walsh-Hadamard transforms
+// (spx_hadamard.hip): any couplings and fields, three passes per application
+// (two per Chebyshev order after the first), L in range.
 constexpr size_t kHadamardMinSpins = 14, kHadamardMaxSpins = 20;
 // Chebyshev weights of one vector of an order launch (see launch_order).
 // `defer` (general orders on the Hadamard path only, 0 elsewhere) batches
@@ -182,8 +184,10 @@ struct HadamardLaunch {
   int next[kMaxMany];    // see OrderLaunch::uv_next
 };
 bool hadamard_eligible(size_t L);
-// coef: the L(L-1)/2 flip-flop weights at the apply scale, pair order (0,1),(0,2),...
-int hadamard_prepare(spx_ham* h, const std::vector<double>& coef);
+// Weights at the apply scale after the skip rule: pair q (order (0,1),(0,2),...)
+// anti / aligned, spin s transverse cx / cy (both zero for a skipped field).
+int hadamard_prepare(spx_ham* h, const std::vector<double>& anti, const std::vector<double>& aligned,
+                     const std::vector<double>& cx, const std::vector<double>& cy);
 int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec);
 // Scratch for `slots` vectors of one launch under h (call before any launch
 // or graph capture that needs them; launches check and fail otherwise).
@@ -225,11 +229,12 @@ int tile_bits_for(const spx_ham* h);
 int block_threads(const spx_ham* h);
 int xcd_remap(const spx_ham* h);
 int dense_tile_bits(const spx_ham* h);
-// The flip-flop-only path used when neither the schedule nor SPX_DENSE names
-// one (1 all-pairs gathers, 2 Hadamard transforms), by L and launch width;
-// set_launch_width records an evolve's width and invalidates the prepared
-// tables when that changes the choice.
-int default_dense_path(size_t L, int width);
+// The path used when neither the schedule nor SPX_DENSE names one (0 table
+// kernels, 1 all-pairs gathers, 2 Hadamard transforms), by L, launch width
+// and whether the couplings are flip-flop only; set_launch_width records an
+// evolve's width and invalidates the prepared tables when that changes the
+// choice.
+int default_dense_path(size_t L, int width, bool flipflop_only);
 void set_launch_width(spx_ham* h, int width);
 // the autotuner's process-wide mode and its hook in the propagator
 int autotune_before_evolve(spx_ham* h, double bound, int nvec);

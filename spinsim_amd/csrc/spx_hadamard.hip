@@ -1,40 +1,48 @@
-// Flip-flop couplings through Walsh-Hadamard transforms (gfx950, FP64).
+// The off-diagonal part of H through Walsh-Hadamard transforms (gfx950, FP64).
 //
-// For a Hamiltonian whose only off-diagonal terms are flip-flop pairs
-// (weights w_q = scale (jx + jy) / 4, no transverse field, no flip-both
-// weight), the pair sum the reference applies term by term
-// (src/core/Hamiltonian.cpp:95-127) is
+// In the Pauli basis of the reference's kernels (spin s = bit s, set bit =
+// up), the coupled pair (i, j) contributes anti (|ud> <-> |du>) and aligned
+// (|dd> <-> |uu>) weights (scale (jx + jy) / 4 and scale (jx - jy) / 4) and a
+// transverse field cx sigma_x + cy sigma_y (scale hx / 2, scale hy / 2),
+// applied term by term in src/core/Hamiltonian.cpp:95-127 and
+// src/core/SpinOperators.cpp:211-291. With X, Y the standard Pauli matrices
+// in the bit basis (the reference's sigma_y is -Y there),
 //
-//   H_ff = sum_q w_q (s+_i s-_j + s-_i s+_j) = (H_XX + P H_XX P) / 2,
-//   H_XX = sum_q w_q X_i X_j = W D W / 2^L,
+//   anti (s+s- + s-s+) + aligned (s+s+ + s-s-)
+//       = (anti + aligned)/2 X_i X_j + (anti - aligned)/2 Y_i Y_j,
 //
-// with W the unnormalised Walsh-Hadamard transform on all L bits, D(s) =
-// sum_q w_q z_i(s) z_j(s) (z = +1 for bit 0, -1 for bit 1) and P the
-// diagonal sign (-1)^floor(popcount / 2): Y_i Y_j = P X_i X_j P on the
-// popcount-parity sectors that X_i X_j preserves, and (X X + Y Y) / 2 is the
-// flip-flop term. With E = D / 2^(L+1) (a power-of-two scale, exact):
+// and Y = S X S^+ with S = diag(i^popcount(k)). X strings are diagonal in the
+// Hadamard basis (W X_i W = 2^L Z_i, W the unnormalised Walsh-Hadamard
+// transform on all L bits), so the whole off-diagonal part is
 //
-//   H_ff psi = W E W psi + P (W E W (P psi)).
+//   H_off psi = W A W psi + S W B W (S^+ psi),
+//   A(s) = [sum_q (anti + aligned)/2 z_i z_j + sum_s cx_s z_s] / 2^L,
+//   B(s) = [sum_q (anti - aligned)/2 z_i z_j - sum_s cy_s z_s] / 2^L,
+//
+// z = +1 for bit 0, -1 for bit 1; 2^-L is exact. For flip-flop couplings
+// alone (aligned = 0, no transverse field) A = B.
 //
 // This costs O(L) butterflies per amplitude instead of one partner gather
-// per coupled pair (L(L-1)/2 of them for all-to-all dipolar couplings), and
-// every butterfly is unmasked. The 2^L index is split into a row part (the
-// low TB = 10 bits, a contiguous tile) and a column part (the high G = L - TB
-// bits); W = W_high W_low, so one Chebyshev order is three passes:
-//   1. rows:    U = W_low psi, V = W_low (P psi)              per tile, in LDS
-//   2. columns: X = W_high E W_high X for X in {U, V}         8 columns per workgroup
-//   3. rows:    H psi = sdiag diag psi + W_low U + P W_low V, fused with the
+// per term (L(L-1)/2 pairs for all-to-all couplings, plus L fields during
+// pulses), and every butterfly is unmasked. The 2^L index is split into a
+// row part (the low TB = 10 bits, a contiguous tile) and a column part (the
+// high G = L - TB bits); W = W_high W_low, so one Chebyshev order is three
+// passes:
+//   1. rows:    U = W_low psi, V = W_low (S^+ psi)            per tile, in LDS
+//   2. columns: U = W_high A W_high U, V = W_high B W_high V  8 columns per workgroup
+//   3. rows:    H psi = sdiag diag psi + W_low U + S W_low V, fused with the
 //               same Chebyshev / accumulate epilogues as the gather kernels.
 // In a Chebyshev expansion the input of order n+1 is the output T_n of order
 // n, so pass 3 also runs pass 1 on T_n while it is in registers: an order
 // after the first costs two launches.
-// E is not stored per amplitude: with k = (x, c) (x the G column-pass bits,
-// c the TB row bits), E(k) = E_lo(c) + E_hi(x) + sum_j z_j(x) A_j(c), where
-// E_lo / E_hi collect the pairs inside c / inside x and A_j(c) = sum_i w_(i,
-// TB+j) z_i(c) / 2^(L+1) the pairs across; the tables hold 2^TB + 2^G +
-// G 2^TB values and stay in L2.
-// The transforms are exact sums and differences; results agree with the
-// gather kernels to rounding (a few ulps of the amplitude scale).
+// A and B are not stored per amplitude: with k = (x, c) (x the G column-pass
+// bits, c the TB row bits), A(k) = A_lo(c) + A_hi(x) + sum_j z_j(x) A_j(c),
+// where A_lo / A_hi collect the pairs and fields inside c / inside x and A_j
+// the pairs across (likewise B); the tables hold 2^TB + 2^G + G 2^TB values
+// each and stay in L2.
+// The transforms are exact sums and differences, the phases i^k exact swaps
+// and sign changes; results agree with the gather kernels to rounding (a few
+// ulps of the amplitude scale).
 #include <cmath>
 #include <utility>
 
@@ -53,10 +61,11 @@ struct WhtArgs {
   const double2* in[kMaxMany];
   double2* out[kMaxMany];
   double2* psi[kMaxMany];
-  double2* u[kMaxMany];         // scratch: W E W psi
-  double2* w[kMaxMany];         // scratch: W E W P psi
+  double2* u[kMaxMany];         // scratch: W A W psi (row transforms between passes)
+  double2* w[kMaxMany];         // scratch: W B W S^+ psi
   const double* diag[kMaxMany];
-  const double* e[kMaxMany];    // E_lo[2^TB], E_hi[2^G], A[G][2^TB]
+  const double* e[kMaxMany];    // A factors: A_lo[2^TB], A_hi[2^G], A_j[G][2^TB]
+  const double* eb[kMaxMany];   // B factors, same layout
   double sdiag[kMaxMany];
   double a0[kMaxMany], c1r[kMaxMany], c1i[kMaxMany], cr[kMaxMany], ci[kMaxMany];
   double pr[kMaxMany], pi[kMaxMany], qr[kMaxMany], qi[kMaxMany];
@@ -72,8 +81,15 @@ __device__ __forceinline__ void caxpy(double2& q, double cr, double ci, double2 
 
 enum { K_APPLY = 0, K_ACCUM = 1, K_ORDER1 = 2, K_ORDER2 = 3, K_ORDERN = 4 };
 
-// (-1)^floor(popcount(k) / 2)
-__device__ __forceinline__ double psign(uint32_t k) { return (__popc(k) & 2u) ? -1.0 : 1.0; }
+// v i^e (e taken mod 4): a swap and sign changes, exact.
+__device__ __forceinline__ double2 mul_ipow(double2 v, uint32_t e) {
+  const bool odd = e & 1u, neg = e & 2u;
+  double re = odd ? -v.y : v.x, im = odd ? v.x : v.y;
+  return make_double2(neg ? -re : re, neg ? -im : im);
+}
+// S^+ and S at amplitude k: i^-popcount(k) and i^popcount(k)
+__device__ __forceinline__ uint32_t sdag(uint32_t k) { return 0u - static_cast<uint32_t>(__popc(k)); }
+__device__ __forceinline__ uint32_t sphase(uint32_t k) { return static_cast<uint32_t>(__popc(k)); }
 
 // LDS slot of element i of a row tile: bits 0-2 XOR bits 3-5, so that in
 // every round eight consecutive lanes touch eight distinct 16-byte slots of
@@ -128,7 +144,7 @@ __device__ __forceinline__ void wht_rounds(double2* __restrict__ S, uint32_t t, 
 // round trips (load, wait, store, next load).
 
 // Pass 1: one workgroup per row tile x (2^TB consecutive amplitudes) and
-// vector; threads [0, 128) transform psi, [128, 256) transform P psi.
+// vector; threads [0, 128) transform psi, [128, 256) transform S^+ psi.
 __global__ __launch_bounds__(kRowThreads) void k_wht_rows_fwd(WhtArgs a) {
   constexpr uint32_t T = 1u << kTB, NT = kRowThreads, H = NT / 2, M = T / NT;
   extern __shared__ double2 smem[];
@@ -142,9 +158,8 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_fwd(WhtArgs a) {
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
     const uint32_t i = t + m * NT;
-    const double p = psign(static_cast<uint32_t>(base + i));
     smem[rslot(i)] = x[m];
-    smem[T + rslot(i)] = make_double2(p * x[m].x, p * x[m].y);
+    smem[T + rslot(i)] = mul_ipow(x[m], sdag(static_cast<uint32_t>(base + i)));
   }
   __syncthreads();
   const uint32_t half = t >= H ? 1u : 0u;
@@ -220,7 +235,7 @@ __device__ __forceinline__ void col_forward(double2* S, uint32_t u, uint32_t c, 
     col_forward<G, K + 1>(S, u, c, v);
   }
 }
-// inverse rounds 1 .. NI-1 (round 0 runs in registers after the E product)
+// inverse rounds 1 .. NI-1 (round 0 runs in registers after the product with A or B)
 template <int G, int K>
 __device__ __forceinline__ void col_inverse(double2* S, uint32_t u, uint32_t c, double2 (&v)[8]) {
   if constexpr (K < inv_rounds(G)) {
@@ -236,7 +251,7 @@ __device__ __forceinline__ void col_inverse(double2* S, uint32_t u, uint32_t c, 
 // vector; 2^G threads (column c = t mod 8, free row bits u = t / 8) hold 8
 // columns x 2^G rows and transform the rows in place: the first round on the
 // values as loaded (register bits 0-2: 128-byte row segments), then through
-// LDS; the product with E and the first inverse round in registers; the last
+// LDS; the product with A (U) or B (V) and the first inverse round in registers; the last
 // inverse round (register bits 0-2 again) straight back to memory.
 template <int G>
 __global__ __launch_bounds__(1 << G) void k_wht_cols(WhtArgs a) {
@@ -251,8 +266,8 @@ __global__ __launch_bounds__(1 << G) void k_wht_cols(WhtArgs a) {
   double2 x[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) x[r] = X[(static_cast<size_t>(round_index<0>(u, r)) << kTB) | col];
-  // E at the elements of the forward's last layout: rows u | r << LT
-  const double* __restrict__ tab = a.e[v];
+  // A (or B) at the elements of the forward's last layout: rows u | r << LT
+  const double* __restrict__ tab = (blockIdx.y & 1u) ? a.eb[v] : a.e[v];
   double e[8];
   {
     double part = tab[col];
@@ -333,11 +348,11 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
     const size_t k = base + t + m * NT;
-    const double p = psign(static_cast<uint32_t>(k));
+    const double2 sv = mul_ipow(xw[m], sphase(static_cast<uint32_t>(k)));
     const double f = sdiag * dg[m];
     double2 acc;
-    acc.x = fma(f, xv[m].x, xu[m].x + p * xw[m].x);
-    acc.y = fma(f, xv[m].y, xu[m].y + p * xw[m].y);
+    acc.x = fma(f, xv[m].x, xu[m].x + sv.x);
+    acc.y = fma(f, xv[m].y, xu[m].y + sv.y);
     if constexpr (KIND == K_APPLY) {
       o[m] = acc;
     } else if constexpr (KIND == K_ACCUM) {
@@ -375,9 +390,8 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
 #pragma unroll
       for (uint32_t m = 0; m < M; ++m) {
         const uint32_t i = t + m * NT;
-        const double p = psign(static_cast<uint32_t>(base + i));
         smem[rslot(i)] = o[m];
-        smem[T + rslot(i)] = make_double2(p * o[m].x, p * o[m].y);
+        smem[T + rslot(i)] = mul_ipow(o[m], sdag(static_cast<uint32_t>(base + i)));
       }
       __syncthreads();
       wht_rounds<kTB, 0>(smem + half * T, t - half * H, RowSlot{});
@@ -412,41 +426,54 @@ void launch_bwd(hipStream_t st, const WhtArgs& a, dim3 grid) {
 
 bool hadamard_eligible(size_t L) { return L >= kHadamardMinSpins && L <= kHadamardMaxSpins; }
 
-// The factors of E (see the file comment), from the pair weights in pair
-// order (0,1),(0,2),...; sums over unscaled weights, then the exact scale
-// 2^-(L+1).
-int hadamard_prepare(spx_ham* h, const std::vector<double>& coef) {
-  const size_t L = h->nspins, n = static_cast<size_t>(1) << L;
+// The factors of A and B (see the file comment) from the weights at the
+// apply scale: pair q (order (0,1),(0,2),...) anti / aligned, spin s cx / cy.
+// Sums over the unscaled combinations, then the exact scale 2^-L.
+int hadamard_prepare(spx_ham* h, const std::vector<double>& anti, const std::vector<double>& aligned,
+                     const std::vector<double>& cx, const std::vector<double>& cy) {
+  const size_t L = h->nspins;
   if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard path needs 14 <= L <= 20");
-  const size_t T = size_t(1) << kTB, G = L - kTB, R = size_t(1) << G;
-  const int sc = -static_cast<int>(L + 1);
-  auto weight = [&](size_t lo, size_t hi) { return coef[lo * L - lo * (lo + 1) / 2 + (hi - lo - 1)]; };
+  const size_t T = size_t(1) << kTB, G = L - kTB, R = size_t(1) << G, TS = T + R + G * T;
+  const int sc = -static_cast<int>(L);
+  auto q_of = [&](size_t lo, size_t hi) { return lo * L - lo * (lo + 1) / 2 + (hi - lo - 1); };
   auto z = [](size_t k, size_t bit) { return ((k >> bit) & 1u) ? -1.0 : 1.0; };
-  std::vector<double> tab(T + R + G * T, 0.0);
-  for (size_t c = 0; c < T; ++c) {
-    double e = 0.0;
-    for (size_t i = 0; i < kTB; ++i)
-      for (size_t k = i + 1; k < static_cast<size_t>(kTB); ++k) e += weight(i, k) * z(c, i) * z(c, k);
-    tab[c] = std::ldexp(e, sc);
-  }
-  for (size_t x = 0; x < R; ++x) {
-    double e = 0.0;
-    for (size_t i = 0; i < G; ++i)
-      for (size_t k = i + 1; k < G; ++k) e += weight(kTB + i, kTB + k) * z(x, i) * z(x, k);
-    tab[T + x] = std::ldexp(e, sc);
-  }
-  for (size_t j = 0; j < G; ++j)
+  std::vector<double> tab(2 * TS, 0.0);
+  for (int which = 0; which < 2; ++which) {
+    // which 0: A (X strings), 1: B (Y strings)
+    auto pair = [&](size_t lo, size_t hi) {
+      const size_t q = q_of(lo, hi);
+      return which == 0 ? 0.5 * (anti[q] + aligned[q]) : 0.5 * (anti[q] - aligned[q]);
+    };
+    auto field = [&](size_t s) { return which == 0 ? cx[s] : -cy[s]; };
+    double* t = tab.data() + which * TS;
     for (size_t c = 0; c < T; ++c) {
       double e = 0.0;
-      for (size_t i = 0; i < static_cast<size_t>(kTB); ++i) e += weight(i, kTB + j) * z(c, i);
-      tab[T + R + j * T + c] = std::ldexp(e, sc);
+      for (size_t i = 0; i < static_cast<size_t>(kTB); ++i)
+        for (size_t k = i + 1; k < static_cast<size_t>(kTB); ++k) e += pair(i, k) * z(c, i) * z(c, k);
+      for (size_t s = 0; s < static_cast<size_t>(kTB); ++s) e += field(s) * z(c, s);
+      t[c] = std::ldexp(e, sc);
     }
+    for (size_t x = 0; x < R; ++x) {
+      double e = 0.0;
+      for (size_t i = 0; i < G; ++i)
+        for (size_t k = i + 1; k < G; ++k) e += pair(kTB + i, kTB + k) * z(x, i) * z(x, k);
+      for (size_t j = 0; j < G; ++j) e += field(kTB + j) * z(x, j);
+      t[T + x] = std::ldexp(e, sc);
+    }
+    for (size_t j = 0; j < G; ++j)
+      for (size_t c = 0; c < T; ++c) {
+        double e = 0.0;
+        for (size_t i = 0; i < static_cast<size_t>(kTB); ++i) e += pair(i, kTB + j) * z(c, i);
+        t[T + R + j * T + c] = std::ldexp(e, sc);
+      }
+  }
   spx_ctx* c = h->ctx;
-  if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, (T + (size_t(1) << (kHadamardMaxSpins - kTB)) +
-                                                   (kHadamardMaxSpins - kTB) * T) * sizeof(double)));
+  constexpr size_t kMaxTS = (size_t(1) << kTB) + (size_t(1) << (kHadamardMaxSpins - kTB)) +
+                            (kHadamardMaxSpins - kTB) * (size_t(1) << kTB);
+  if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, 2 * kMaxTS * sizeof(double)));
+  h->wht_tab = TS;
   SPX_HIP(hipMemcpyAsync(h->d_wht_e, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   SPX_HIP(hipStreamSynchronize(c->stream));  // `tab` is pageable and local
-  (void)n;
   return hadamard_reserve(h, 1);
 }
 
@@ -482,6 +509,7 @@ int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
     a.w[i] = h->d_wht + (2 * static_cast<size_t>(b.slot[i]) + 1) * n;
     a.diag[i] = h->d_diag;
     a.e[i] = h->d_wht_e;
+    a.eb[i] = h->d_wht_e + h->wht_tab;
     a.sdiag[i] = b.sdiag[i];
     a.a0[i] = b.w[i].a0;
     a.c1r[i] = b.w[i].c1r;

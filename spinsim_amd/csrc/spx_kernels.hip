@@ -990,11 +990,11 @@ int xcd_remap(const spx_ham* h) {
   return h->sched.xcd >= 0 ? h->sched.xcd : env;
 }
 
-// Dense-pair eligibility and its weight table (pair order (0,1),(0,2),...):
-// no transverse field and no flip-both weight at this scale; L = 14..18 for
-// the all-pairs gather kernel (dense = 1), L = 14..20 for the Hadamard
-// sandwich (dense = 2). SPX_DENSE=0 (or a schedule with dense = 0) switches
-// both off; without either, default_dense_path chooses.
+// The two specialised paths and their weights: the all-pairs gather kernel
+// (dense = 1; L = 14..18, flip-flop couplings only: no transverse field and
+// no flip-both weight at this scale) and the Hadamard transforms (dense = 2;
+// L = 14..20, any couplings and fields). SPX_DENSE=0 (or a schedule with
+// dense = 0) switches both off; without either, default_dense_path chooses.
 int dense_tile_bits(const spx_ham* h) {
   static const int env = env_int("SPX_DENSE_TB", 10);
   const int tb = h->sched.dense_tb > 0 ? h->sched.dense_tb : env;
@@ -1007,7 +1007,8 @@ int dense_tile_bits(const spx_ham* h) {
 // 15.9 / 19.3, width 4: 37.7 / 30.4, width 8: 55.7 / 36.5; L = 18, width 1:
 // 17.4 / 23.1, width 2: 35.1 / 24.4; from L = 19 only the Hadamard path
 // exists.
-int default_dense_path(size_t L, int width) {
+int default_dense_path(size_t L, int width, bool flipflop_only) {
+  if (!flipflop_only) return hadamard_eligible(L) ? 2 : 0;
   if (L >= 19) return 2;
   if (L == 18) return width >= 2 ? 2 : 1;
   if (L == 17) return width >= 4 ? 2 : 1;
@@ -1017,7 +1018,7 @@ int default_dense_path(size_t L, int width) {
 
 void set_launch_width(spx_ham* h, int width) {
   if (h->width == width) return;
-  const bool changes = default_dense_path(h->nspins, h->width) != default_dense_path(h->nspins, width);
+  const bool changes = default_dense_path(h->nspins, h->width, true) != default_dense_path(h->nspins, width, true);
   h->width = width;
   if (changes) h->terms_valid = false;
 }
@@ -1025,34 +1026,43 @@ void set_launch_width(spx_ham* h, int width) {
 static int prepare_dense(spx_ham* h, double scale) {
   static const int env = env_int("SPX_DENSE", -1);
   const size_t L = h->nspins;
-  const int enabled = h->sched.dense >= 0 ? h->sched.dense : env >= 0 ? env : default_dense_path(L, h->width);
   h->dense_ok = false;
   h->wht_ok = false;
-  const bool pairs = enabled == 1 && L >= 14 && L <= 18;
-  const bool hadamard = enabled == 2 && hadamard_eligible(L);
-  if (!pairs && !hadamard) return SPX_OK;
-  for (size_t s = 0; s < L; ++s)
-    if (std::abs(scale * 0.5 * h->hx[s]) > kNegligible || std::abs(scale * 0.5 * h->hy[s]) > kNegligible)
-      return SPX_OK;
+  if (L < 2) return SPX_OK;
+  // weights at this scale after the skip rule (as the flip table takes them)
   const size_t np = L * (L - 1) / 2;
-  std::vector<double> coef(np, 0.0);
+  std::vector<double> anti(np, 0.0), aligned(np, 0.0), cx(L, 0.0), cy(L, 0.0);
+  bool flipflop_only = true;
+  for (size_t s = 0; s < L; ++s) {
+    const double x = scale * 0.5 * h->hx[s], y = scale * 0.5 * h->hy[s];
+    if (!(std::abs(x) > kNegligible || std::abs(y) > kNegligible)) continue;
+    cx[s] = x;
+    cy[s] = y;
+    flipflop_only = false;
+  }
   for (size_t p = 0; p < h->lo.size(); ++p) {
     const double quarter = scale * 0.25;
-    const double ff = quarter * (h->jx[p] + h->jy[p]);
-    const double fb = quarter * (h->jx[p] - h->jy[p]);
-    if (std::abs(fb) > kNegligible) return SPX_OK;
+    double ff = quarter * (h->jx[p] + h->jy[p]);
+    double fb = quarter * (h->jx[p] - h->jy[p]);
+    if (!(std::abs(ff) > kNegligible)) ff = 0.0;
+    if (!(std::abs(fb) > kNegligible)) fb = 0.0;
+    if (fb != 0.0) flipflop_only = false;
     const size_t lo = h->lo[p], hi = h->hi[p];
     const size_t q = lo * L - lo * (lo + 1) / 2 + (hi - lo - 1);
-    coef[q] = std::abs(ff) > kNegligible ? ff : 0.0;
+    anti[q] = ff;
+    aligned[q] = fb;
   }
-  if (hadamard) {
-    SPX_TRY(hadamard_prepare(h, coef));
+  const int path = h->sched.dense >= 0 ? h->sched.dense
+                   : env >= 0          ? env
+                                       : default_dense_path(L, h->width, flipflop_only);
+  if (path == 2 && hadamard_eligible(L)) {
+    SPX_TRY(hadamard_prepare(h, anti, aligned, cx, cy));
     h->wht_ok = true;
     return SPX_OK;
   }
+  if (path != 1 || !flipflop_only || L < 14 || L > 18) return SPX_OK;
   if (!h->d_coef) SPX_HIP(hipMalloc(&h->d_coef, 32 * 31 / 2 * sizeof(double)));
-  SPX_HIP(hipMemcpyAsync(h->d_coef, coef.data(), np * sizeof(double), hipMemcpyHostToDevice,
-                         h->ctx->stream));
+  SPX_HIP(hipMemcpyAsync(h->d_coef, anti.data(), np * sizeof(double), hipMemcpyHostToDevice, h->ctx->stream));
   SPX_HIP(hipStreamSynchronize(h->ctx->stream));
   h->dense_ok = true;
   return SPX_OK;

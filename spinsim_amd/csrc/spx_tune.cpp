@@ -65,7 +65,7 @@ std::string describe(const Schedule& s, const spx_ham* h) {
 std::vector<Schedule> candidates(const spx_ham* h, bool flipflop) {
   std::vector<Schedule> c;
   const int L = static_cast<int>(h->nspins);
-  if (flipflop && hadamard_eligible(h->nspins)) {
+  if (hadamard_eligible(h->nspins)) {
     Schedule s;
     s.dense = 2;
     c.push_back(s);

@@ -2,8 +2,8 @@
 H psi through Walsh-Hadamard transforms instead of one gather per coupled
 pair. Every application kind against the oracle, evolution against the
 table and all-pairs kernels up to L=20, batched states bitwise equal to
-single ones, conservation at L=18, and the fallbacks when the couplings
-are not flip-flop only. FP64; agreement to rounding."""
+single ones, conservation at L=18, transverse fields and flip-both
+weights, and the fallbacks outside its sizes. FP64; agreement to rounding."""
 import numpy as np
 import pytest
 
@@ -132,15 +132,37 @@ def test_l18_conservation_and_time_reversal(ctx):
     assert D.max_abs_diff(s.download(), psi0) < 1e-10
 
 
-@pytest.mark.parametrize("L,transverse,aniso", [(16, True, False), (16, False, True), (13, False, False),
-                                                (21, False, False)])
-def test_falls_back_when_not_eligible(ctx, L, transverse, aniso):
-    banner(f"hadamard fallback L={L} transverse={transverse} aniso={aniso}")
-    fields, c = dipolar_system(L, 7, transverse=transverse, anisotropic=aniso)
+@pytest.mark.parametrize("L", [13, 21])
+def test_falls_back_outside_its_sizes(ctx, L):
+    banner(f"hadamard fallback L={L}")
+    fields, c = dipolar_system(L, 7)
     h = device_ham(ctx, L, fields, c, 2)
     s, o = spx.StateVector(ctx, L, O.random_state(L, 9)), spx.StateVector(ctx, L)
     h.apply_physical(s, o)
     assert not h.kernel_name().startswith("k_wht"), h.kernel_name()
+
+
+@pytest.mark.parametrize("L,transverse,aniso", [(14, True, False), (14, False, True), (16, True, True)])
+def test_transverse_fields_and_flip_both_weights(ctx, L, transverse, aniso):
+    """Pulses (hx, hy) and jx != jy: the Y-string sandwich S W B W S^+ and
+    the fields in A and B, against the oracle's apply and evolution."""
+    banner(f"hadamard general H L={L} transverse={transverse} aniso={aniso}")
+    fields, c = dipolar_system(L, 11, transverse=transverse, anisotropic=aniso)
+    if transverse:
+        fields[:, 1] = np.linspace(-0.6, 0.9, L)   # hy as well as hx
+    ho = O.Ham(L, fields, c)
+    h = device_ham(ctx, L, fields, c, 2)
+    psi = O.random_state(L, 12)
+    sin, sout = spx.StateVector(ctx, L, psi), spx.StateVector(ctx, L)
+    h.apply_physical(sin, sout)
+    assert h.kernel_name().startswith("k_wht_"), h.kernel_name()
+    assert rel_err(sout.download(), ho.apply_physical(psi)) < 1e-13
+    want = psi.copy()
+    rc, _ = ho.evolve(want, 0.05, 2, 0, False, 1e-12)
+    assert rc == 0
+    s = spx.StateVector(ctx, L, psi)
+    spx.Propagator(ctx, L, 1e-12).evolve(h, spx.REAL, s, 0.05, 2)
+    assert float(np.max(np.abs(s.download() - want))) < 1e-12 * max(1.0, float(np.max(np.abs(want))))
 
 
 def test_imaginary_time_matches_the_table_kernels(ctx):
