@@ -1,9 +1,10 @@
 #pragma once
 // A complete description of one experiment (include/spinsim/io/Config.hpp:32-58
-// of the reference). The TOML reader is not part of this build; RunConfig is
-// filled in code.
+// of the reference) and its TOML configuration files (loadConfig /
+// parseConfig, :60-66; the reader is this build's own, src/config.cpp).
 
 #include <cstdint>
+#include <filesystem>
 #include <stdexcept>
 #include <string>
 #include <string_view>
@@ -44,5 +45,11 @@ struct RunConfig {
 
   void validate(std::string_view origin = "<config>") const;
 };
+
+/// Reads a TOML configuration file.
+[[nodiscard]] RunConfig loadConfig(const std::filesystem::path& path);
+
+/// Parses TOML held in memory. `origin` only appears in error messages.
+[[nodiscard]] RunConfig parseConfig(std::string_view text, std::string_view origin);
 
 }  // namespace spinsim

@@ -5,6 +5,7 @@
 //   test_host_api gpu            device cases
 //   test_host_api dump <file>    config-0 echo run, per-realisation rows to <file>
 //                                (compared against the oracle by pytest)
+//   test_host_api config <file>  load a TOML configuration and print its shape
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -19,6 +20,7 @@
 #include "spinsim/core/Bessel.hpp"
 #include "spinsim/core/Propagator.hpp"
 #include "spinsim/core/SpinOperators.hpp"
+#include "spinsim/io/Config.hpp"
 #include "spinsim/io/Experiment.hpp"
 
 using namespace spinsim;
@@ -245,6 +247,138 @@ void cpuTests() {
     bad.sequence = Sequence({});
     checkThrows<ConfigError>([&] { bad.validate("<test>"); }, "no measure");
   });
+
+  // Configuration files, as tests/test_Config.cpp exercises parseConfig.
+  static const char* kBase = R"(
+# three spins, one evolve between two measurements
+[system]
+spins = 3
+
+[field]
+hz = 2.5        # every spin
+
+[measurement]
+observables = ["corr:z"]
+
+[[step]]
+kind = "measure"
+
+[[step]]
+kind = "evolve"
+tau = 0.25
+
+[[step]]
+kind = "measure"
+)";
+  auto parse = [](const char* text) { return parseConfig(text, "<test>"); };
+  run("config: a minimal file and its defaults", [&] {
+    const RunConfig c = parse(kBase);
+    CHECK(c.nspins == 3 && c.baseFields.size() == 3);
+    for (const Field& f : c.baseFields) CHECK(f.x == 0.0 && f.y == 0.0 && f.z == 2.5);
+    CHECK(c.initialState == InitialStateKind::Random && c.realizations == 1 && c.seed == 0);
+    CHECK(c.generateGeometry && c.dipolarScale == 1.0 && c.fieldDisorder == 0.0);
+    CHECK(c.referenceAxis == Axis::Z && c.epsilon == 1e-7);
+    CHECK(c.sequence.measurementCount() == 2);
+    const auto& e = std::get<Evolve>(c.sequence.steps()[1]);
+    CHECK(e.tau == 0.25 && e.nsteps == 1 && e.mode == TimeMode::Real && !e.autonormalize && e.fields.empty());
+  });
+  run("config: per-spin arrays, integers as numbers, every section", [&] {
+    const RunConfig c = parse(R"(
+[system]
+spins = 4
+initial_state = "basis"
+basis_index = 5
+[field]
+hx = [1, 2.5, -3e-1, 4_000]
+hy = 0
+[ensemble]
+realizations = 12
+seed = 987_654
+dipolar_geometry = false
+dipolar_scale = 2
+field_disorder = 0.75
+[measurement]
+reference_axis = 'x'
+observables = [
+  "corr:x",   # multi-line array with comments
+  "mag:2:y",
+  "norm",
+]
+[numerics]
+epsilon = 1.0e-12
+[[step]]
+kind = "pulse"
+rotation = "pi/2:y:3"
+[[step]]
+kind = "evolve"
+tau = 0.5
+steps = 7
+mode = "imaginary"
+autonormalize = true
+[[step]]
+kind = "measure"
+)");
+    CHECK(c.nspins == 4 && c.initialState == InitialStateKind::BasisState && c.basisIndex == 5);
+    CHECK(c.baseFields[0].x == 1.0 && c.baseFields[1].x == 2.5 && c.baseFields[2].x == -0.3 && c.baseFields[3].x == 4000.0);
+    CHECK(c.realizations == 12 && c.seed == 987654 && !c.generateGeometry);
+    CHECK(c.dipolarScale == 2.0 && c.fieldDisorder == 0.75 && c.epsilon == 1e-12);
+    CHECK(c.referenceAxis == Axis::X && c.observables.size() == 3 && c.observables[1] == "mag:2:y");
+    const auto& p = std::get<Pulse>(c.sequence.steps()[0]);
+    CHECK(p.rotation.targets().size() == 1 && p.rotation.targets()[0] == 2);
+    const auto& e = std::get<Evolve>(c.sequence.steps()[1]);
+    CHECK(e.nsteps == 7 && e.mode == TimeMode::Imaginary && e.autonormalize);
+  });
+  run("config: nested repeats and step fields over the baseline", [&] {
+    const RunConfig c = parse(R"(
+[system]
+spins = 2
+[field]
+hz = [5.0, 6.0]
+hy = 0.5
+[measurement]
+observables = ["norm"]
+[[step]]
+kind = "repeat"
+count = 2
+body = [
+  { kind = "evolve", tau = 0.1, hx = 40.0 },
+  { kind = "repeat", count = 3, body = [ { kind = "measure" } ] },
+]
+)");
+    CHECK(c.sequence.measurementCount() == 6);
+    const auto& r = std::get<Repeat>(c.sequence.steps()[0]);
+    const auto& e = std::get<Evolve>(r.body[0]);
+    CHECK(e.fields.size() == 2 && e.fields[0].x == 40.0 && e.fields[1].x == 40.0);
+    CHECK(e.fields[0].y == 0.5 && e.fields[0].z == 5.0 && e.fields[1].z == 6.0);
+  });
+  run("config: problems name their key", [&] {
+    auto fails_with = [&](const char* text, const char* needle) {
+      try {
+        (void)parse(text);
+        CHECK(false);
+      } catch (const ConfigError& e) {
+        const std::string m = e.what();
+        CHECK(m.find(needle) != std::string::npos && m.find("<test>") != std::string::npos);
+      }
+    };
+    fails_with("[system]\nspins = 3\n[field]\nhx = [1.0, 2.0]\n[measurement]\nobservables = [\"norm\"]\n"
+               "[[step]]\nkind = \"measure\"\n", "hx");
+    fails_with("[system]\nspins = 2\n[measurement]\nobservables = [\"corr:w\"]\n[[step]]\nkind = \"measure\"\n",
+               "observables");
+    fails_with("[system]\nspins = 2\n[measurement]\nobservables = [\"norm\"]\n[[step]]\nkind = \"warp\"\n", "warp");
+    fails_with("[system]\nspins = 2\n[measurement]\nobservables = [\"norm\"]\n[[step]]\nkind = \"evolve\"\n"
+               "tau = 0.1\n", "step");
+    fails_with("[system]\nspins = 99\n[measurement]\nobservables = [\"norm\"]\n[[step]]\nkind = \"measure\"\n",
+               "system.spins");
+    fails_with("[system]\nspins = 2\n[measurement]\nobservables = [\"norm\"]\n", "step");
+    fails_with("[system\nspins = 2", "<syntax>");
+    fails_with("[system]\nspins = 2\nspins = 3\n", "<syntax>");
+    fails_with("[system]\nspins = 02\n", "<syntax>");
+    fails_with("[system]\nspins = \"2\n", "<syntax>");
+    fails_with("[system]\nspins = 2 3\n", "<syntax>");
+    fails_with("[system]\nspins = [1, 2\n", "<syntax>");
+    checkThrows<ConfigError>([] { (void)loadConfig("/nonexistent/run.toml"); }, "missing file");
+  });
 }
 
 // --------------------------------------------------------------------- device
@@ -401,6 +535,40 @@ void gpuTests() {
     CHECK(csv.str().rfind("time,corr_x_re_mean,corr_x_re_sd,corr_x_re_stderr,", 0) == 0);
     CHECK(r1.statistics.count() == 12);
   });
+  run("a configuration file runs like the same configuration built in code", [] {
+    const RunConfig fromFile = parseConfig(R"(
+[system]
+spins = 6
+[ensemble]
+realizations = 4
+seed = 20240726
+field_disorder = 0.5
+min_separation = 0.5
+secular_angle = true
+[measurement]
+reference_axis = "x"
+observables = ["corr:x", "norm", "energy"]
+[[step]]
+kind = "pulse"
+rotation = "pi/2:x"
+[[step]]
+kind = "repeat"
+count = 8
+body = [
+  { kind = "evolve", tau = 0.05, steps = 10 },
+  { kind = "pulse", rotation = "pi:y" },
+  { kind = "evolve", tau = 0.05, steps = 10 },
+  { kind = "measure" },
+]
+)",
+                                           "<test>");
+    const RunConfig inCode = cpmgConfig(6, 4);
+    LocalRunner runner(2);
+    const EnsembleResult a = runExperiment(fromFile, runner);
+    const EnsembleResult b = runExperiment(inCode, runner);
+    CHECK(a.columns == b.columns && a.times == b.times);
+    CHECK(a.statistics.count() == 4 && a.statistics.mean() == b.statistics.mean());
+  });
 }
 
 int dump(const char* path) {
@@ -424,9 +592,24 @@ int dump(const char* path) {
 
 }  // namespace
 
+// `config <file>`: load a configuration and print its shape.
+int describe(const char* path) {
+  try {
+    const RunConfig c = loadConfig(path);
+    std::printf("spins=%zu realizations=%zu seed=%llu observables=%zu measurements=%zu steps=%zu\n", c.nspins,
+                c.realizations, static_cast<unsigned long long>(c.seed), c.observables.size(),
+                c.sequence.measurementCount(), c.sequence.steps().size());
+    return 0;
+  } catch (const std::exception& e) {
+    std::printf("error: %s\n", e.what());
+    return 1;
+  }
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   if (mode == "dump" && argc > 2) return dump(argv[2]);
+  if (mode == "config" && argc > 2) return describe(argv[2]);
   if (mode == "cpu" || mode == "all") cpuTests();
   if (mode == "gpu" || mode == "all") gpuTests();
   std::printf("%s: %d failure(s)\n", g_failed ? "FAILED" : "OK", g_failed);

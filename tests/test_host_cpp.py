@@ -26,6 +26,36 @@ def test_host_api_cpu_cases():
     assert r.returncode == 0 and "OK: 0 failure(s)" in r.stdout
 
 
+REFERENCE_CONFIGS = "/root/reference/configs"
+
+
+def measurements(steps):
+    n = 0
+    for st in steps:
+        if st["kind"] == "measure":
+            n += 1
+        elif st["kind"] == "repeat":
+            n += st["count"] * measurements(st["body"])
+    return n
+
+
+@pytest.mark.parametrize("name", ["cpmg8z.toml", "rabi_szsz.toml"])
+def test_reference_configs_load_unchanged(name):
+    """The reference's own configuration files through the drop-in's
+    loadConfig; the expected shape comes from an independent TOML reader."""
+    path = os.path.join(REFERENCE_CONFIGS, name)
+    if not os.path.exists(path):
+        pytest.skip("the reference's configs are not present")
+    import tomli
+    with open(path, "rb") as f:
+        doc = tomli.load(f)
+    want = (f"spins={doc['system']['spins']} realizations={doc['ensemble']['realizations']} "
+            f"seed={doc['ensemble']['seed']} observables={len(doc['measurement']['observables'])} "
+            f"measurements={measurements(doc['step'])} steps={len(doc['step'])}")
+    r = driver("config", path)
+    assert r.returncode == 0 and r.stdout.strip() == want, (r.stdout, want)
+
+
 @pytest.mark.gpu
 def test_host_api_device_cases():
     r = driver("gpu")
