@@ -35,6 +35,11 @@ struct RunConfig {
   /// (runRealization always uses the generator's defaults there).
   double minSeparation = 5.0e-2;
   bool secularAngle = false;
+  /// Fixed couplings applied to every realisation, with no counterpart in
+  /// the reference's RunConfig (there, generateGeometry = false leaves the
+  /// system uncoupled). Only allowed with generateGeometry = false; they use
+  /// no random draws, so the realisation streams are unchanged.
+  std::vector<Coupling> couplings;
 
   Axis referenceAxis = Axis::Z;
   std::vector<std::string> observables;

@@ -7,8 +7,8 @@
 // RunConfig follows the reference key for key: the same defaults, the same
 // scalar-or-per-spin field rule, step fields falling back to the baseline,
 // and a ConfigError naming the key for every problem. [ensemble] also takes
-// min_separation and secular_angle, the geometry options RunConfig carries
-// beyond the reference's.
+// min_separation and secular_angle, and [[coupling]] tables give fixed
+// couplings: options RunConfig carries beyond the reference's.
 #include <cctype>
 #include <cmath>
 #include <cstdint>
@@ -577,6 +577,29 @@ RunConfig parseConfig(std::string_view text, std::string_view origin) {
       const std::optional<std::string> spec = v.as_string();
       if (!spec) throw ConfigError(origin, "measurement.observables", "entries must be strings");
       cfg.observables.push_back(*spec);
+    }
+  }
+
+  // [[coupling]] tables: spins = [i, j] (1-based, as in rotation and
+  // observable specs), jx, jy, jz (0 when absent)
+  if (const TomlArray* list = array_at(&doc, "coupling")) {
+    for (const TomlValue& v : *list) {
+      if (v.kind != TomlValue::Kind::Table) throw ConfigError(origin, "coupling", "every coupling must be a table");
+      const TomlTable* t = v.table.get();
+      const TomlArray* spins = array_at(t, "spins");
+      std::optional<std::int64_t> a, b;
+      if (spins && spins->size() == 2) {
+        a = (*spins)[0].as_int();
+        b = (*spins)[1].as_int();
+      }
+      if (!a || !b) throw ConfigError(origin, "coupling.spins", "expected two spin numbers, 1-based");
+      const auto n = static_cast<std::int64_t>(cfg.nspins);
+      if (*a < 1 || *b < 1 || *a > n || *b > n || *a == *b)
+        throw ConfigError(origin, "coupling.spins",
+                          "needs two different spins between 1 and " + std::to_string(cfg.nspins));
+      cfg.couplings.push_back(Coupling{SpinPair(static_cast<SpinIndex>(*a - 1), static_cast<SpinIndex>(*b - 1)),
+                                       optional_double(t, "jx", 0.0), optional_double(t, "jy", 0.0),
+                                       optional_double(t, "jz", 0.0)});
     }
   }
 

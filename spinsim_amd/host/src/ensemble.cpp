@@ -359,6 +359,17 @@ void RunConfig::validate(std::string_view origin) const {
     throw ConfigError(origin, "step", "the sequence records nothing; add a measure step");
   if (initialState == InitialStateKind::BasisState && basisIndex >= stateCount(nspins))
     throw ConfigError(origin, "system.basis_index", "outside the 2^L state space");
+  if (!couplings.empty() && generateGeometry)
+    throw ConfigError(origin, "coupling", "fixed couplings need ensemble.dipolar_geometry = false");
+  for (std::size_t p = 0; p < couplings.size(); ++p) {
+    const SpinPair pair = couplings[p].pair;
+    if (pair.high() >= nspins || pair.low() == pair.high())
+      throw ConfigError(origin, "coupling", "pair " + std::to_string(p + 1) + " names a spin outside the system");
+    for (std::size_t q = 0; q < p; ++q)
+      if (couplings[q].pair == pair)
+        throw ConfigError(origin, "coupling", "pair " + std::to_string(p + 1) + " repeats pair " +
+                                                  std::to_string(q + 1));
+  }
   for (const std::string& spec : observables) {
     try {
       (void)makeObservable(spec);
@@ -387,6 +398,8 @@ Inputs drawInputs(const RunConfig& cfg, std::size_t index) {
   if (cfg.generateGeometry)
     in.couplings = generateDipolarLattice2D(cfg.nspins, rng, cfg.dipolarScale, cfg.minSeparation,
                                             cfg.secularAngle);
+  else
+    in.couplings = cfg.couplings;
   if (cfg.fieldDisorder != 0.0)
     for (Field& f : in.fields) f.z += cfg.fieldDisorder * rng.gaussian();
   if (cfg.initialState == InitialStateKind::Random) randomState(in.psi0, cfg.nspins, rng);
@@ -403,6 +416,8 @@ void runRealization(const RunConfig& config, std::size_t index, VectorRecorder& 
     for (const Coupling& c : generateDipolarLattice2D(config.nspins, rng, config.dipolarScale,
                                                       config.minSeparation, config.secularAngle))
       hamiltonian.setCoupling(c.pair, c.jx, c.jy, c.jz);
+  else
+    for (const Coupling& c : config.couplings) hamiltonian.setCoupling(c.pair, c.jx, c.jy, c.jz);
   if (config.fieldDisorder != 0.0) applyFieldDisorder(hamiltonian, rng, config.fieldDisorder);
   StateVector state(config.nspins);
   prepareInitialState(state, config.initialState, rng, config.basisIndex);
