@@ -185,9 +185,12 @@ struct HadamardLaunch {
 };
 bool hadamard_eligible(size_t L);
 // Weights at the apply scale after the skip rule: pair q (order (0,1),(0,2),...)
-// anti / aligned, spin s transverse cx / cy (both zero for a skipped field).
+// anti / aligned, spin s transverse cx / cy (both zero for a skipped field);
+// the diagonal's terms unscaled, as rebuildDiagonal takes them: dfield[s] =
+// hz_s / 2 and dpair[q] = jz_q / 4, zero where skipped.
 int hadamard_prepare(spx_ham* h, const std::vector<double>& anti, const std::vector<double>& aligned,
-                     const std::vector<double>& cx, const std::vector<double>& cy);
+                     const std::vector<double>& cx, const std::vector<double>& cy,
+                     const std::vector<double>& dfield, const std::vector<double>& dpair);
 int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec);
 // Scratch for `slots` vectors of one launch under h (call before any launch
 // or graph capture that needs them; launches check and fail otherwise).

@@ -39,7 +39,10 @@
 // bits, c the TB row bits), A(k) = A_lo(c) + A_hi(x) + sum_j z_j(x) A_j(c),
 // where A_lo / A_hi collect the pairs and fields inside c / inside x and A_j
 // the pairs across (likewise B); the tables hold 2^TB + 2^G + G 2^TB values
-// each and stay in L2.
+// each and stay in L2. The diagonal is not read per amplitude either:
+// diag(x, c) = D_lo(c) + D_hi(x) + sum_{i < TB} z_i(c) K_i(x), with
+// K_i(x) = sum_j (pair weight of i and TB + j) z_j(x), from a table of
+// 2^TB + 2^G (1 + TB) values; pass 3 reads D_hi(x) and K(x) once per tile.
 // The transforms are exact sums and differences, the phases i^k exact swaps
 // and sign changes; results agree with the gather kernels to rounding (a few
 // ulps of the amplitude scale).
@@ -63,7 +66,7 @@ struct WhtArgs {
   double2* psi[kMaxMany];
   double2* u[kMaxMany];         // scratch: W A W psi (row transforms between passes)
   double2* w[kMaxMany];         // scratch: W B W S^+ psi
-  const double* diag[kMaxMany];
+  const double* dz[kMaxMany];   // diagonal factors: D_lo[2^TB], then per x: D_hi, K_0..K_{TB-1}
   const double* e[kMaxMany];    // A factors: A_lo[2^TB], A_hi[2^G], A_j[G][2^TB]
   const double* eb[kMaxMany];   // B factors, same layout
   double sdiag[kMaxMany];
@@ -253,8 +256,11 @@ __device__ __forceinline__ void col_inverse(double2* S, uint32_t u, uint32_t c, 
 // values as loaded (register bits 0-2: 128-byte row segments), then through
 // LDS; the product with A (U) or B (V) and the first inverse round in registers; the last
 // inverse round (register bits 0-2 again) straight back to memory.
+// Up to G = 8 the 32 KiB of LDS per workgroup allow five workgroups per CU;
+// the register budget is held to match (at most 102 VGPRs), measured 2%
+// faster at L = 18 than the compiler's own 107 VGPRs and four per CU.
 template <int G>
-__global__ __launch_bounds__(1 << G) void k_wht_cols(WhtArgs a) {
+__global__ __launch_bounds__(1 << G) __attribute__((amdgpu_waves_per_eu(G <= 8 ? 5 : 1))) void k_wht_cols(WhtArgs a) {
   static_assert(inv_lo(G, inv_rounds(G) - 1) == 0, "the inverse ends at bit 0");
   constexpr uint32_t T = 1u << kTB, NT = 1u << G;
   constexpr int LT = G - 3;   // register bits of the forward's last round
@@ -312,7 +318,8 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   const double2* __restrict__ in = a.in[v];
   double2* __restrict__ out = a.out[v];
   double2* __restrict__ psi = a.psi[v];
-  const double* __restrict__ diag = a.diag[v];
+  const double* __restrict__ dlo = a.dz[v];
+  const double* __restrict__ drow = a.dz[v] + T + static_cast<size_t>(blockIdx.x) * (kTB + 1);
   const double sdiag = a.sdiag[v], a0 = a.a0[v], c1r = a.c1r[v], c1i = a.c1i[v], cr = a.cr[v], ci = a.ci[v];
   const int defer = KIND == K_ORDERN ? a.defer[v] : 0;   // uniform over the workgroup
   double2 xu[M], xw[M], xv[M], prev[M], pv[M];
@@ -326,7 +333,7 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   for (uint32_t m = 0; m < M; ++m) {
     const size_t k = base + t + m * NT;
     xv[m] = in[k];
-    dg[m] = diag[k];
+    dg[m] = dlo[t + m * NT];
     if constexpr (KIND == K_ACCUM || KIND == K_ORDERN) prev[m] = out[k];
     if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN)
       if (defer != 1) pv[m] = psi[k];
@@ -343,6 +350,22 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   for (uint32_t m = 0; m < M; ++m) {
     xu[m] = smem[rslot(t + m * NT)];
     xw[m] = smem[T + rslot(t + m * NT)];
+  }
+  {
+    // diag at c = t + m NT: the bits of t are shared by the thread's M
+    // elements, the bits above come from m
+    static_assert(M == 4 && NT == 256, "row tile split assumed below");
+    double kr[kTB];
+#pragma unroll
+    for (int i = 0; i < kTB; ++i) kr[i] = drow[1 + i];
+    double own = drow[0];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) own += ((t >> i) & 1u) ? -kr[i] : kr[i];
+#pragma unroll
+    for (uint32_t m = 0; m < M; ++m) {
+      const double up = ((m & 1u) ? -kr[8] : kr[8]) + ((m & 2u) ? -kr[9] : kr[9]);
+      dg[m] = dg[m] + (own + up);
+    }
   }
   double2 o[M];
 #pragma unroll
@@ -430,14 +453,16 @@ bool hadamard_eligible(size_t L) { return L >= kHadamardMinSpins && L <= kHadama
 // apply scale: pair q (order (0,1),(0,2),...) anti / aligned, spin s cx / cy.
 // Sums over the unscaled combinations, then the exact scale 2^-L.
 int hadamard_prepare(spx_ham* h, const std::vector<double>& anti, const std::vector<double>& aligned,
-                     const std::vector<double>& cx, const std::vector<double>& cy) {
+                     const std::vector<double>& cx, const std::vector<double>& cy,
+                     const std::vector<double>& dfield, const std::vector<double>& dpair) {
   const size_t L = h->nspins;
   if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard path needs 14 <= L <= 20");
   const size_t T = size_t(1) << kTB, G = L - kTB, R = size_t(1) << G, TS = T + R + G * T;
   const int sc = -static_cast<int>(L);
   auto q_of = [&](size_t lo, size_t hi) { return lo * L - lo * (lo + 1) / 2 + (hi - lo - 1); };
   auto z = [](size_t k, size_t bit) { return ((k >> bit) & 1u) ? -1.0 : 1.0; };
-  std::vector<double> tab(2 * TS, 0.0);
+  const size_t DS = T + R * (kTB + 1);
+  std::vector<double> tab(2 * TS + DS, 0.0);
   for (int which = 0; which < 2; ++which) {
     // which 0: A (X strings), 1: B (Y strings)
     auto pair = [&](size_t lo, size_t hi) {
@@ -467,10 +492,36 @@ int hadamard_prepare(spx_ham* h, const std::vector<double>& anti, const std::vec
         t[T + R + j * T + c] = std::ldexp(e, sc);
       }
   }
+  {
+    // the diagonal (rebuildDiagonal's terms: field c at bit s adds -c z_s,
+    // pair weight c adds c z_i z_j), unscaled
+    double* d = tab.data() + 2 * TS;
+    for (size_t c = 0; c < T; ++c) {
+      double e = 0.0;
+      for (size_t s = 0; s < static_cast<size_t>(kTB); ++s) e -= dfield[s] * z(c, s);
+      for (size_t i = 0; i < static_cast<size_t>(kTB); ++i)
+        for (size_t k = i + 1; k < static_cast<size_t>(kTB); ++k) e += dpair[q_of(i, k)] * z(c, i) * z(c, k);
+      d[c] = e;
+    }
+    for (size_t x = 0; x < R; ++x) {
+      double* row = d + T + x * (kTB + 1);
+      double e = 0.0;
+      for (size_t j = 0; j < G; ++j) e -= dfield[kTB + j] * z(x, j);
+      for (size_t i = 0; i < G; ++i)
+        for (size_t k = i + 1; k < G; ++k) e += dpair[q_of(kTB + i, kTB + k)] * z(x, i) * z(x, k);
+      row[0] = e;
+      for (size_t i = 0; i < static_cast<size_t>(kTB); ++i) {
+        double kx = 0.0;
+        for (size_t j = 0; j < G; ++j) kx += dpair[q_of(i, kTB + j)] * z(x, j);
+        row[1 + i] = kx;
+      }
+    }
+  }
   spx_ctx* c = h->ctx;
   constexpr size_t kMaxTS = (size_t(1) << kTB) + (size_t(1) << (kHadamardMaxSpins - kTB)) +
                             (kHadamardMaxSpins - kTB) * (size_t(1) << kTB);
-  if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, 2 * kMaxTS * sizeof(double)));
+  constexpr size_t kMaxDS = (size_t(1) << kTB) + (size_t(1) << (kHadamardMaxSpins - kTB)) * (kTB + 1);
+  if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, (2 * kMaxTS + kMaxDS) * sizeof(double)));
   h->wht_tab = TS;
   SPX_HIP(hipMemcpyAsync(h->d_wht_e, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   SPX_HIP(hipStreamSynchronize(c->stream));  // `tab` is pageable and local
@@ -507,7 +558,7 @@ int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
     a.psi[i] = b.psi[i];
     a.u[i] = h->d_wht + (2 * static_cast<size_t>(b.slot[i])) * n;
     a.w[i] = h->d_wht + (2 * static_cast<size_t>(b.slot[i]) + 1) * n;
-    a.diag[i] = h->d_diag;
+    a.dz[i] = h->d_wht_e + 2 * h->wht_tab;
     a.e[i] = h->d_wht_e;
     a.eb[i] = h->d_wht_e + h->wht_tab;
     a.sdiag[i] = b.sdiag[i];

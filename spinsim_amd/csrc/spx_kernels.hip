@@ -1056,7 +1056,18 @@ static int prepare_dense(spx_ham* h, double scale) {
                    : env >= 0          ? env
                                        : default_dense_path(L, h->width, flipflop_only);
   if (path == 2 && hadamard_eligible(L)) {
-    SPX_TRY(hadamard_prepare(h, anti, aligned, cx, cy));
+    // the diagonal's terms after ham_prepare's skip rule
+    std::vector<double> dfield(L, 0.0), dpair(np, 0.0);
+    for (size_t s = 0; s < L; ++s) {
+      const double cz = 0.5 * h->hz[s];
+      if (std::abs(cz) > kNegligible) dfield[s] = cz;
+    }
+    for (size_t p = 0; p < h->lo.size(); ++p) {
+      const double cz = 0.25 * h->jz[p];
+      const size_t lo = h->lo[p], hi = h->hi[p];
+      if (std::abs(cz) > kNegligible) dpair[lo * L - lo * (lo + 1) / 2 + (hi - lo - 1)] = cz;
+    }
+    SPX_TRY(hadamard_prepare(h, anti, aligned, cx, cy, dfield, dpair));
     h->wht_ok = true;
     return SPX_OK;
   }
