@@ -6,11 +6,13 @@
 // first on the host) and runs the whole program of one realisation with its
 // state and reference vectors in registers: 512 threads, thread t owning
 // amplitudes k = t + 512 i. Each Chebyshev order stages T_{n-1} in LDS for
-// the partner gathers (at most 2^12 x 16 B = 64 KiB); partners that differ
-// only in register bits come straight from registers. The pair structure is
-// compiled per L (8..12), the weights are per realisation and per evolve
-// step. All reductions are fixed-order block trees, so every realisation's
-// numbers are independent of which workgroup ran it and of timing.
+// the partner gathers, the state's and the reference's side by side up to
+// L = 11 so that both advance under one pair of barriers (at L = 12, 64 KiB
+// for one vector, they take turns). An evolve step's pair and field weights
+// and the pair masks are staged in LDS once per step. The weights are per
+// realisation and per evolve step. All reductions are fixed-order block
+// trees, so every realisation's numbers are independent of which workgroup
+// ran it and of timing.
 #include <utility>
 
 #include "spx_batch.hpp"
@@ -88,73 +90,17 @@ struct Res {
   static constexpr int NP = L * (L - 1) / 2;
 };
 
+// State and reference evolve in one order loop up to L = 11 (at L = 12 the
+// registers of both vectors' recursions do not fit beside each other).
+template <int L>
+constexpr bool kBoth = L <= 11;
+template <int L>
+constexpr int kVectors = kBoth<L> ? 2 : 1;
+
 // bit `b` of amplitude k = t + 512 i
 template <int L>
 __device__ __forceinline__ uint32_t kbit(uint32_t t, int i, int b) {
   return b < Res<L>::RB ? (t >> b) & 1u : (static_cast<uint32_t>(i) >> (b - Res<L>::RB)) & 1u;
-}
-
-// Partner k ^ m of the own amplitude i: from registers when m touches only
-// register bits, from LDS otherwise.
-template <int L>
-__device__ __forceinline__ double2 partner(const double2* lds, const double2* v, uint32_t t, int i,
-                                           uint32_t m) {
-  constexpr int RB = Res<L>::RB;
-  const uint32_t ml = m & ((1u << RB) - 1u);
-  const uint32_t mr = m >> RB;
-  if (ml == 0) return v[i ^ static_cast<int>(mr)];
-  return lds[(t ^ ml) + kBT * (static_cast<uint32_t>(i) ^ mr)];
-}
-
-// acc_i = (H x)_k at the own amplitudes, H given by its weights at the
-// scale already folded in (diag unscaled, multiplied by `sdiag`).
-template <int L, int... Q>
-__device__ __forceinline__ void apply_pairs(std::integer_sequence<int, Q...>, const double2* lds,
-                                            const double2* xv, double2* acc, uint32_t t,
-                                            const double* __restrict__ pw) {
-  constexpr int A = Res<L>::A;
-  auto one = [&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int a = pa(L, q), b = pb(L, q);
-    constexpr uint32_t m = (1u << a) | (1u << b);
-    const double anti = pw[2 * q], aligned = pw[2 * q + 1];
-#pragma unroll
-    for (int i = 0; i < A; ++i) {
-      const bool on = kbit<L>(t, i, a) != kbit<L>(t, i, b);
-      const double c = on ? anti : aligned;
-      if (c != 0.0) cmadd(acc[i], c, partner<L>(lds, xv, t, i, m));
-    }
-  };
-  (one(std::integral_constant<int, Q>{}), ...);
-}
-
-template <int L>
-__device__ void apply_h(const double2* lds, const double2* xv, double2* acc, uint32_t t,
-                        const double* dg, double sdiag, const double* __restrict__ w,
-                        const DevEvolve& e) {
-  constexpr int A = Res<L>::A;
-#pragma unroll
-  for (int i = 0; i < A; ++i) {
-    const double f = sdiag * dg[i];
-    acc[i] = make_double2(f * xv[i].x, f * xv[i].y);
-  }
-  if (e.has_field) {
-    const double* fw = w + e.field_off;
-    for (int s = 0; s < L; ++s) {
-      const double cx = fw[2 * s], cy = fw[2 * s + 1];
-      if (cx == 0.0 && cy == 0.0) continue;
-#pragma unroll
-      for (int i = 0; i < A; ++i) {
-        const double2 p = partner<L>(lds, xv, t, i, 1u << s);
-        const double sy = kbit<L>(t, i, s) ? -cy : cy;
-        acc[i].x = fma(cx, p.x, acc[i].x);
-        acc[i].x = fma(-sy, p.y, acc[i].x);
-        acc[i].y = fma(cx, p.y, acc[i].y);
-        acc[i].y = fma(sy, p.x, acc[i].y);
-      }
-    }
-  }
-  apply_pairs<L>(std::make_integer_sequence<int, Res<L>::NP>{}, lds, xv, acc, t, w + e.pair_off);
 }
 
 // sigma_z part of the Pauli-basis operator at the own amplitudes, terms in
@@ -190,13 +136,79 @@ __device__ __forceinline__ void stage(double2* lds, const double2* v, uint32_t t
   __syncthreads();
 }
 
+// Pair and field terms of H x at the own amplitudes k = t + 512 i for an
+// evolve step whose weights sit in LDS: lw[q] = (anti, aligned) of pair q,
+// lw[NP + s] = (cx, cy) of spin s, at the scale, and lm[q] = the pair's bit
+// mask. The whole vector is staged in LDS, so partner k ^ m is read from there
+// for every pair; a pair term is added for every amplitude, with a zero
+// weight where the reference skips it. The pair loop is a runtime loop, so
+// the partner reads of a few consecutive pairs are in flight together
+// without the register cost of unrolling all L(L-1)/2 of them.
+template <int L>
+__device__ __forceinline__ void apply_h_lds(const double2* lds, const double2* xv, double2* acc, uint32_t t,
+                                            const double* dg, double sdiag, const double2* lw,
+                                            const uint32_t* lm, bool fields) {
+  constexpr int A = Res<L>::A, NP = Res<L>::NP;
+#pragma unroll
+  for (int i = 0; i < A; ++i) {
+    const double f = sdiag * dg[i];
+    acc[i] = make_double2(f * xv[i].x, f * xv[i].y);
+  }
+  if (fields) {
+    for (int s = 0; s < L; ++s) {
+      const double2 c = lw[NP + s];
+      if (c.x == 0.0 && c.y == 0.0) continue;
+#pragma unroll
+      for (int i = 0; i < A; ++i) {
+        const uint32_t k = t + kBT * i;
+        const double2 p = lds[k ^ (1u << s)];
+        const double sy = ((k >> s) & 1u) ? -c.y : c.y;
+        acc[i].x = fma(c.x, p.x, acc[i].x);
+        acc[i].x = fma(-sy, p.y, acc[i].x);
+        acc[i].y = fma(c.x, p.y, acc[i].y);
+        acc[i].y = fma(sy, p.x, acc[i].y);
+      }
+    }
+  }
+#pragma unroll 5
+  for (int q = 0; q < NP; ++q) {
+    const uint32_t m = lm[q];
+    const double2 wq = lw[q];
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      const uint32_t k = t + kBT * i;
+      const bool on = __popc(k & m) == 1;
+      cmadd(acc[i], on ? wq.x : wq.y, lds[k ^ m]);
+    }
+  }
+}
+
+// The weights of one evolve step (or of the energy's Hamiltonian) into LDS
+// for apply_h_lds; the leading barrier lets earlier readers finish, and the
+// first staging barrier after it makes them visible.
+template <int L>
+__device__ __forceinline__ void stage_weights(double2* lw, uint32_t* lm, const double* __restrict__ w,
+                                              const DevEvolve& e, uint32_t t) {
+  constexpr int NP = Res<L>::NP;
+  __syncthreads();
+  if (t < static_cast<uint32_t>(NP)) {
+    lw[t] = make_double2(w[e.pair_off + 2 * t], w[e.pair_off + 2 * t + 1]);
+    lm[t] = (1u << pa(L, static_cast<int>(t))) | (1u << pb(L, static_cast<int>(t)));
+  } else if (t < static_cast<uint32_t>(NP + L)) {
+    const uint32_t sp = t - NP;
+    lw[t] = e.has_field ? make_double2(w[e.field_off + 2 * sp], w[e.field_off + 2 * sp + 1])
+                        : make_double2(0.0, 0.0);
+  }
+}
+
 // nsteps Chebyshev steps of one vector (Propagator::applyOneStep /
 // evolve, src/core/Propagator.cpp:46-100). Returns false on a zero norm.
 template <int L>
-__device__ bool evolve_vec(double2* v, double2* lds, double* red, uint32_t t, const double* dg,
-                           const double* __restrict__ w, const DevEvolve& e, const DevOp& op) {
+__device__ bool evolve_vec(double2* v, double2* lds, double* red, double2* lw, uint32_t* lm, uint32_t t,
+                           const double* dg, const double* __restrict__ w, const DevEvolve& e, const DevOp& op) {
   constexpr int A = Res<L>::A;
   const bool active = t < static_cast<uint32_t>(Res<L>::N);
+  stage_weights<L>(lw, lm, w, e, t);
   const double* a = w + e.coef_off;
   const double scale = -2.0 / e.bound;
   double2 tprev[A], tcur[A], acc[A];
@@ -209,7 +221,7 @@ __device__ bool evolve_vec(double2* v, double2* lds, double* red, uint32_t t, co
     double wre = op.mode == 0 ? 0.0 : -1.0, wim = op.mode == 0 ? -1.0 : 0.0;
     for (int n = 1; n <= e.order; ++n) {
       stage<L>(lds, tcur, t);
-      if (active) apply_h<L>(lds, tcur, acc, t, dg, scale, w, e);
+      if (active) apply_h_lds<L>(lds, tcur, acc, t, dg, scale, lw, lm, e.has_field != 0);
       const double cr = 2.0 * a[n] * wre, ci = 2.0 * a[n] * wim;
 #pragma unroll
       for (int i = 0; i < A; ++i) {
@@ -245,6 +257,96 @@ __device__ bool evolve_vec(double2* v, double2* lds, double* red, uint32_t t, co
   return true;
 }
 
+// evolve_vec for the state and the reference together: one staging barrier
+// pair per order for both (T_{n-1} of the state in lds[0, N), of the
+// reference in lds[N, 2N)), the step's weights read from LDS. Each vector
+// sees exactly evolve_vec's arithmetic. Returns false on a zero norm.
+template <int L>
+__device__ bool evolve_both(double2* st, double2* rf, double2* lds, double* red, double2* lw, uint32_t* lm,
+                            uint32_t t,
+                            const double* dg, const double* __restrict__ w, const DevEvolve& e,
+                            const DevOp& op) {
+  constexpr int A = Res<L>::A, N = Res<L>::N;
+  const bool active = t < static_cast<uint32_t>(N);
+  stage_weights<L>(lw, lm, w, e, t);
+  const double* a = w + e.coef_off;
+  const double scale = -2.0 / e.bound;
+  const bool fields = e.has_field != 0;
+  double2 sprev[A], scur[A], rprev[A], rcur[A], acc[A];
+  for (int step = 0; step < op.nsteps; ++step) {
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      scur[i] = st[i];
+      st[i] = make_double2(st[i].x * a[0], st[i].y * a[0]);
+      rcur[i] = rf[i];
+      rf[i] = make_double2(rf[i].x * a[0], rf[i].y * a[0]);
+    }
+    double wre = op.mode == 0 ? 0.0 : -1.0, wim = op.mode == 0 ? -1.0 : 0.0;
+    for (int n = 1; n <= e.order; ++n) {
+      __syncthreads();
+      if (active)
+#pragma unroll
+        for (int i = 0; i < A; ++i) {
+          lds[t + kBT * i] = scur[i];
+          lds[N + t + kBT * i] = rcur[i];
+        }
+      __syncthreads();
+      const double cr = 2.0 * a[n] * wre, ci = 2.0 * a[n] * wim;
+      if (active) {
+        apply_h_lds<L>(lds, scur, acc, t, dg, scale, lw, lm, fields);
+#pragma unroll
+        for (int i = 0; i < A; ++i) {
+          double2 tn;
+          if (n == 1) tn = make_double2(0.5 * acc[i].x, 0.5 * acc[i].y);
+          else tn = make_double2(acc[i].x - sprev[i].x, acc[i].y - sprev[i].y);
+          st[i].x += cr * tn.x - ci * tn.y;
+          st[i].y += cr * tn.y + ci * tn.x;
+          sprev[i] = scur[i];
+          scur[i] = tn;
+        }
+        apply_h_lds<L>(lds + N, rcur, acc, t, dg, scale, lw, lm, fields);
+#pragma unroll
+        for (int i = 0; i < A; ++i) {
+          double2 tn;
+          if (n == 1) tn = make_double2(0.5 * acc[i].x, 0.5 * acc[i].y);
+          else tn = make_double2(acc[i].x - rprev[i].x, acc[i].y - rprev[i].y);
+          rf[i].x += cr * tn.x - ci * tn.y;
+          rf[i].y += cr * tn.y + ci * tn.x;
+          rprev[i] = rcur[i];
+          rcur[i] = tn;
+        }
+      }
+      if (op.mode == 0) {
+        const double pr = wre;
+        wre = wim;
+        wim = -pr;
+      } else {
+        wre = -wre;
+        wim = -wim;
+      }
+    }
+    if (op.autonorm) {
+      double ss = 0.0, sr = 0.0;
+      if (active)
+#pragma unroll
+        for (int i = 0; i < A; ++i) {
+          ss += st[i].x * st[i].x + st[i].y * st[i].y;
+          sr += rf[i].x * rf[i].x + rf[i].y * rf[i].y;
+        }
+      const double2 nn = block_sum(red, ss, sr);
+      const double ns = sqrt(nn.x), nr = sqrt(nn.y);
+      if (ns == 0.0 || nr == 0.0) return false;
+      const double is = 1.0 / ns, ir = 1.0 / nr;
+#pragma unroll
+      for (int i = 0; i < A; ++i) {
+        st[i] = make_double2(st[i].x * is, st[i].y * is);
+        rf[i] = make_double2(rf[i].x * ir, rf[i].y * ir);
+      }
+    }
+  }
+  return true;
+}
+
 // SpinRotation::apply (src/core/Gate.cpp:60-85), targets in ascending order.
 template <int L>
 __device__ void rotate_vec(double2* v, double2* lds, uint32_t t, const DevPulse& p) {
@@ -259,7 +361,7 @@ __device__ void rotate_vec(double2* v, double2* lds, uint32_t t, const DevPulse&
     double2 nv[A];
 #pragma unroll
     for (int i = 0; i < A; ++i) {
-      const double2 q = partner<L>(lds, v, t, i, 1u << s);
+      const double2 q = lds[(t + kBT * i) ^ (1u << s)];
       const double2 own = v[i];
       if (kbit<L>(t, i, s) == 0) {  // own is the down (lo) amplitude
         nv[i] = make_double2((llr * own.x - lli * own.y) + (lhr * q.x - lhi * q.y),
@@ -286,7 +388,7 @@ __device__ __forceinline__ double2 total_spin(const double2* lds, const double2*
       w.y += up ? psi[i].y : -psi[i].y;
       continue;
     }
-    const double2 p = partner<L>(lds, psi, t, i, 1u << s);
+    const double2 p = lds[(t + kBT * i) ^ (1u << s)];
     if (axis == 0) {
       w.x += p.x;
       w.y += p.y;
@@ -302,8 +404,8 @@ __device__ __forceinline__ double2 total_spin(const double2* lds, const double2*
 }
 
 template <int L>
-__device__ void measure(double2* st, double2* rf, double2* lds, double* red, uint32_t t, double* dg,
-                        const RArgs& g, const DevReal& R, int row) {
+__device__ void measure(double2* st, double2* rf, double2* lds, double* red, double2* lw, uint32_t* lm,
+                        uint32_t t, double* dg, const RArgs& g, const DevReal& R, int row) {
   constexpr int A = Res<L>::A;
   const bool active = t < static_cast<uint32_t>(Res<L>::N);
   double* out = g.out + R.out_off + static_cast<long long>(row) * g.width;
@@ -340,7 +442,7 @@ __device__ void measure(double2* st, double2* rf, double2* lds, double* red, uin
           if (axis == 2) {
             w = up ? x : make_double2(-x.x, -x.y);
           } else {
-            const double2 p = partner<L>(lds, st, t, i, 1u << spin);
+            const double2 p = lds[(t + kBT * i) ^ (1u << spin)];
             w = axis == 0 ? p : (up ? make_double2(p.y, -p.x) : make_double2(-p.y, p.x));
           }
           sr += x.x * w.x + x.y * w.y;
@@ -365,11 +467,12 @@ __device__ void measure(double2* st, double2* rf, double2* lds, double* red, uin
     } else {  // energy: Re <H state|state> with the base fields
       const DevEvolve& me = g.evo[R.measure_off];
       build_diag<L>(dg, t, g.w + me.zdiag_off);
+      stage_weights<L>(lw, lm, g.w, me, t);
       stage<L>(lds, st, t);
       double2 hx[A];
       double sr = 0.0;
       if (active) {
-        apply_h<L>(lds, st, hx, t, dg, 1.0, g.w, me);
+        apply_h_lds<L>(lds, st, hx, t, dg, 1.0, lw, lm, me.has_field != 0);
 #pragma unroll
         for (int i = 0; i < A; ++i) sr += hx[i].x * st[i].x + hx[i].y * st[i].y;
       }
@@ -381,7 +484,7 @@ __device__ void measure(double2* st, double2* rf, double2* lds, double* red, uin
 }
 
 template <int L>
-__device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double* red) {
+__device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double* red, double2* lw, uint32_t* lm) {
   constexpr int A = Res<L>::A;
   const uint32_t t = threadIdx.x;
   const bool active = t < static_cast<uint32_t>(Res<L>::N);
@@ -407,8 +510,13 @@ __device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double*
       const DevEvolve e = g.evo[R.evolve_off + op.arg];
       if (e.order == 0 || op.nsteps == 0) continue;
       build_diag<L>(dg, t, g.w + e.zdiag_off);
-      if (!evolve_vec<L>(st, lds, red, t, dg, g.w, e, op) ||
-          !evolve_vec<L>(rf, lds, red, t, dg, g.w, e, op)) {
+      bool ok;
+      if constexpr (kBoth<L>)
+        ok = evolve_both<L>(st, rf, lds, red, lw, lm, t, dg, g.w, e, op);
+      else
+        ok = evolve_vec<L>(st, lds, red, lw, lm, t, dg, g.w, e, op) &&
+             evolve_vec<L>(rf, lds, red, lw, lm, t, dg, g.w, e, op);
+      if (!ok) {
         if (t == 0) atomicExch(g.queue + 1, 1);  // zero norm under autonormalisation
         return;
       }
@@ -417,7 +525,7 @@ __device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double*
       rotate_vec<L>(st, lds, t, p);
       rotate_vec<L>(rf, lds, t, p);
     } else if (op.kind == OP_MEASURE) {
-      measure<L>(st, rf, lds, red, t, dg, g, R, row);
+      measure<L>(st, rf, lds, red, lw, lm, t, dg, g, R, row);
       ++row;
     } else if (op.kind == OP_REPEAT) {
       if (op.arg == 0) {
@@ -438,11 +546,15 @@ __device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double*
 }
 
 // One persistent kernel per L: its registers are sized for that L alone.
+// LDS: 2^L amplitudes (two vectors' worth where both evolve together), the
+// reduction scratch, the evolve step's weights.
 template <int L>
 __global__ __launch_bounds__(kBT) void k_resident(RArgs g) {
   extern __shared__ double2 smem[];
-  double2* lds = smem;                                           // 2^L amplitudes
-  double* red = reinterpret_cast<double*>(smem + Res<L>::N);    // 2 x kBT
+  double2* lds = smem;
+  double* red = reinterpret_cast<double*>(smem + Res<L>::N * kVectors<L>);   // 2 x kBT
+  double2* lw = reinterpret_cast<double2*>(red + 2 * kBT);                   // NP + L weights
+  uint32_t* lm = reinterpret_cast<uint32_t*>(lw + Res<L>::NP + L);           // NP pair masks
   __shared__ int next;
   for (;;) {
     __syncthreads();
@@ -450,7 +562,7 @@ __global__ __launch_bounds__(kBT) void k_resident(RArgs g) {
     __syncthreads();
     const int slot = next;
     if (slot >= g.nreal) return;
-    run_real<L>(g, g.reals[g.order[slot]], lds, red);
+    run_real<L>(g, g.reals[g.order[slot]], lds, red, lw, lm);
   }
 }
 
@@ -463,7 +575,10 @@ int launch_resident(spx_ctx* c, int L, const DevOp* ops, int nops, const DevPuls
                     int* queue, int nblocks) {
   RArgs g{ops, nops, pulses, evo, reals, order, nreal, weights, psi0, out, width,
           obs_codes, obs_spins, nobs, reference_axis, queue};
-  const size_t lds = sizeof(double2) * (static_cast<size_t>(1) << L) + sizeof(double) * 2 * kBT;
+  const int vectors = L <= 11 ? 2 : 1;   // kVectors<L>
+  const size_t lds = sizeof(double2) * (static_cast<size_t>(vectors) << L) + sizeof(double) * 2 * kBT +
+                     sizeof(double2) * static_cast<size_t>(L * (L - 1) / 2 + L) +
+                     sizeof(uint32_t) * static_cast<size_t>(L * (L - 1) / 2);
   switch (L) {
     case 1: k_resident<1><<<nblocks, kBT, lds, c->stream>>>(g); break;
     case 2: k_resident<2><<<nblocks, kBT, lds, c->stream>>>(g); break;
