@@ -79,14 +79,15 @@ int spx_ctx_create(int device, spx_ctx** out) {
 
 int spx_ctx_destroy(spx_ctx* c) {
   if (!c) return SPX_OK;
-  hipSetDevice(c->device);
-  hipStreamSynchronize(c->stream);
+  // teardown is best effort: a failure here leaves nothing to recover
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < kMaxEvents; ++i)
-    if (c->events[i]) hipEventDestroy(c->events[i]);
-  hipFree(c->d_partials);
-  hipFree(c->d_result);
-  hipHostFree(c->h_result);
-  hipStreamDestroy(c->stream);
+    if (c->events[i]) (void)hipEventDestroy(c->events[i]);
+  (void)hipFree(c->d_partials);
+  (void)hipFree(c->d_result);
+  (void)hipHostFree(c->h_result);
+  (void)hipStreamDestroy(c->stream);
   delete c;
   return SPX_OK;
 }

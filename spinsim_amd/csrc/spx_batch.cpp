@@ -491,7 +491,7 @@ int batch_run_resident(spx_batch* b, double* times, double* values) {
                   static_cast<void*>(d_reals), static_cast<void*>(d_order), static_cast<void*>(d_codes),
                   static_cast<void*>(d_spins), static_cast<void*>(d_queue), static_cast<void*>(d_w),
                   static_cast<void*>(d_psi), static_cast<void*>(d_out)})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   SPX_TRY(rc);
   for (size_t gi = 0; gi + 1 < groups.size(); ++gi)
     if (flags[2 * gi + 1]) return fail(SPX_EDOMAIN, "cannot normalize a zero state");
