@@ -18,7 +18,6 @@ using namespace spx;
 
 namespace {
 
-constexpr double kPi = 3.141592653589793238462643383279502884;
 
 int check_state(const spx_state* s) {
   if (!s || !s->d) return fail(SPX_EINVAL, "null state");
@@ -133,7 +132,7 @@ int spx_state_create(spx_ctx* c, size_t nspins, spx_state** out) {
   }
   int rc = launch_fill(c, s->d, s->n);
   if (rc != SPX_OK) {
-    hipFree(s->d);
+    (void)hipFree(s->d);
     delete s;
     return rc;
   }
@@ -145,7 +144,7 @@ int spx_state_create(spx_ctx* c, size_t nspins, spx_state** out) {
 // gone); hipFree waits for outstanding work on the buffer itself.
 int spx_state_destroy(spx_state* s) {
   if (!s) return SPX_OK;
-  hipFree(s->d);
+  (void)hipFree(s->d);
   delete s;
   return SPX_OK;
 }
@@ -266,13 +265,13 @@ int spx_ham_create(spx_ctx* c, size_t nspins, spx_ham** out) {
 
 int spx_ham_destroy(spx_ham* h) {
   if (!h) return SPX_OK;
-  if (h->d_diag) hipFree(h->d_diag);
-  if (h->d_terms) hipFree(h->d_terms);
-  if (h->d_tab) hipFree(h->d_tab);
-  if (h->d_scratch) hipFree(h->d_scratch);
-  if (h->d_coef) hipFree(h->d_coef);
-  if (h->d_wht_e) hipFree(h->d_wht_e);
-  if (h->d_wht) hipFree(h->d_wht);
+  if (h->d_diag) (void)hipFree(h->d_diag);
+  if (h->d_terms) (void)hipFree(h->d_terms);
+  if (h->d_tab) (void)hipFree(h->d_tab);
+  if (h->d_scratch) (void)hipFree(h->d_scratch);
+  if (h->d_coef) (void)hipFree(h->d_coef);
+  if (h->d_wht_e) (void)hipFree(h->d_wht_e);
+  if (h->d_wht) (void)hipFree(h->d_wht);
   delete h;
   return SPX_OK;
 }
@@ -399,7 +398,7 @@ int spx_prop_create(spx_ctx* c, size_t nspins, double epsilon, spx_prop** out) {
 int spx_prop_destroy(spx_prop* p) {
   if (!p) return SPX_OK;
   for (auto& d : p->d_scratch)
-    if (d) hipFree(d);
+    if (d) (void)hipFree(d);
   delete p;
   return SPX_OK;
 }
