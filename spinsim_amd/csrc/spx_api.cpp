@@ -432,7 +432,7 @@ int ensure_scratch(spx_prop* p, size_t nvec) {
 constexpr size_t kGraphMinSteps = 4;
 
 int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* states, size_t nvec,
-                 const std::vector<double>* const* a);
+                 const std::vector<double>* const* a, bool chain_in = false, bool chain_out = false);
 
 int one_step(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, size_t nvec,
              const std::vector<double>& a) {
@@ -532,8 +532,14 @@ int defer_code(size_t n, size_t N) {
 // *a[i] (order a[i]->size() - 1 >= 1), sorted longest expansion first so that
 // the states still expanding at order n are a prefix of the launch. Every
 // state sees exactly the arithmetic of one_step on its own.
+// chain_in: the Hamiltonians' Hadamard scratch already holds the row
+// transforms of every state (left by the previous step's chain_out), so the
+// first order skips its forward row pass. chain_out: each state's last order
+// leaves the row transforms of its finished state there. Both need the
+// Hadamard path and every expansion of order 2 or more; the arithmetic is
+// that of the unchained step.
 int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* states, size_t nvec,
-                 const std::vector<double>* const* a) {
+                 const std::vector<double>* const* a, bool chain_in, bool chain_out) {
   double2* A[kMaxMany];
   double2* B[kMaxMany];
   for (size_t i = 0; i < nvec; ++i) {
@@ -561,6 +567,7 @@ int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* st
     v.psi[i] = states[i]->d;
     v.uv_next[i] = order_of(i) >= 2;   // each order hands its output's row transforms on
   }
+  v.uv_ready = chain_in ? 1 : 0;
   SPX_TRY(launch_order_set(hs, 1, v, w));
   double c1r[kMaxMany], c1i[kMaxMany];
   for (size_t i = 0; i < nvec; ++i) {
@@ -579,7 +586,7 @@ int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* st
   for (size_t i = 0; i < act; ++i) {
     v.in[i] = B[i];
     v.out[i] = A[i];
-    v.uv_next[i] = order_of(i) >= 3;
+    v.uv_next[i] = order_of(i) >= 3 ? 1 : (chain_out ? 2 : 0);
     w[i] = OrderWeights{(*a[i])[0], c1r[i], c1i[i], 2.0 * (*a[i])[2] * wre, 2.0 * (*a[i])[2] * wim};
   }
   SPX_TRY(launch_order_set(hs, 2, v, w));
@@ -599,7 +606,7 @@ int one_step_set(spx_prop* p, spx_ham* const* hs, int mode, spx_state* const* st
       const bool odd = (n % 2) == 1;
       v.in[i] = odd ? A[i] : B[i];
       v.out[i] = odd ? B[i] : A[i];
-      v.uv_next[i] = order_of(i) > n;
+      v.uv_next[i] = order_of(i) > n ? 1 : (chain_out ? 2 : 0);
       w[i] = OrderWeights{0.0, 0.0, 0.0, 2.0 * (*a[i])[n] * wre, 2.0 * (*a[i])[n] * wim};
       w[i].pr = p1r[i];
       w[i].pi = p1i[i];
@@ -675,6 +682,13 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
     groups.emplace_back(g, e);
     g = e;
   }
+  // With one launch group on the Hadamard path, no normalisation between
+  // steps and every expansion of order 2 or more, each step's last orders
+  // leave the row transforms of the finished states for the next step's first
+  // order, which then skips its forward row pass (the Hamiltonians' scratch
+  // slots are then used by this group alone).
+  bool chain = groups.size() == 1 && !autonormalize && nsteps > 1;
+  for (size_t i : idx) chain = chain && hams[i]->wht_ok && coeffs[i].size() > 2;
   for (size_t step = 0; step < nsteps; ++step) {
     for (const auto& [g, e] : groups) {
       spx_ham* hs[kMaxMany];
@@ -685,7 +699,7 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
         ss[k - g] = states[idx[k]];
         cs[k - g] = &coeffs[idx[k]];
       }
-      SPX_TRY(one_step_set(p, hs, mode, ss, e - g, cs));
+      SPX_TRY(one_step_set(p, hs, mode, ss, e - g, cs, chain && step > 0, chain && step + 1 < nsteps));
     }
     if (autonormalize)
       for (size_t i = 0; i < count; ++i) SPX_TRY(spx_state_normalize(states[i], nullptr));

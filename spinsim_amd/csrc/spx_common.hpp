@@ -149,7 +149,9 @@ struct OrderLaunch {
   // Hadamard path only (ignored elsewhere): the previous launch of these
   // vectors already left the row transforms of `in` in the Hamiltonians'
   // scratch (uv_ready), and vector i's launch leaves those of `out` for the
-  // next one (uv_next[i]); consecutive Chebyshev orders chain this way.
+  // next one (uv_next[i] = 1); consecutive Chebyshev orders chain this way.
+  // uv_next[i] = 2 on a vector's last order leaves those of its finished
+  // accumulator instead, the input of the next step's first order.
   int uv_ready;
   int uv_next[kMaxMany];
 };

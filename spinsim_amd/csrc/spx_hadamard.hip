@@ -407,14 +407,18 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   }
   if constexpr (KIND == K_ORDER1 || KIND == K_ORDER2 || KIND == K_ORDERN) {
     if (a.next[v]) {  // uniform over the workgroup
-      // pass 1 of the next order on T_n: every thread has read its transform
-      // values out of LDS before the barrier
+      // pass 1 of the next order on T_n, or (next = 2, a vector's last order)
+      // of the next step's first order on the finished accumulator: every
+      // thread has read its transform values out of LDS before the barrier
       __syncthreads();
 #pragma unroll
       for (uint32_t m = 0; m < M; ++m) {
         const uint32_t i = t + m * NT;
-        smem[rslot(i)] = o[m];
-        smem[T + rslot(i)] = mul_ipow(o[m], sdag(static_cast<uint32_t>(base + i)));
+        double2 src = o[m];
+        if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN)
+          if (a.next[v] == 2) src = pv[m];
+        smem[rslot(i)] = src;
+        smem[T + rslot(i)] = mul_ipow(src, sdag(static_cast<uint32_t>(base + i)));
       }
       __syncthreads();
       wht_rounds<kTB, 0>(smem + half * T, t - half * H, RowSlot{});
@@ -573,7 +577,9 @@ int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
     a.qi[i] = b.w[i].qi;
     a.defer[i] = b.w[i].defer;
     if (a.defer[i] < 0 || a.defer[i] > 3) return fail(SPX_EINVAL, "accumulator batching code out of range");
-    a.next[i] = b.next[i] ? 1 : 0;
+    a.next[i] = b.next[i];
+    if (a.next[i] < 0 || a.next[i] > 2 || (a.next[i] == 2 && kind != K_ORDER2 && kind != K_ORDERN))
+      return fail(SPX_EINVAL, "chained transforms of an accumulator need a Chebyshev order of 2 or more");
   }
   hipStream_t st = h0->ctx->stream;
   const dim3 rows(static_cast<unsigned>(n >> kTB), nvec);

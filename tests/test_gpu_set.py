@@ -65,6 +65,20 @@ def test_uneven_orders_bitwise_equal_to_single_evolves(ctx, dense):
     assert all(n.startswith(prefix) for n in names), names
 
 
+def test_chained_steps_bitwise_equal_to_single_evolves(ctx):
+    """Several steps of one launch group on the Hadamard path: each state's
+    last order leaves the row transforms of the finished state for the next
+    step's first order, which skips its forward row pass. The states must
+    equal single evolves (unchained) bit for bit, transverse field included."""
+    L = 16
+    banner(f"evolve_set L={L} dense=2, chained steps")
+    specs = [(0, 1.0, False), (1, 3.0, False), (3, 1.0, False), (4, 0.5, False), (5, 1.0, True)]
+    orders, names = check_set(ctx, L, specs, 2, nsteps=4)
+    banner(f"orders {orders}")
+    assert min(orders) >= 2 and len(set(orders)) >= 2
+    assert all(n.startswith("k_wht_") for n in names), names
+
+
 def test_more_states_than_one_launch_takes(ctx):
     L = 14
     banner("evolve_set L=14, ten realisations (two launch groups)")
