@@ -398,10 +398,12 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
       if (defer != 1) caxpy(pv[m], cr, ci, tn);
     }
   }
+  // T_n of a vector's last order (next != 1) is read by no later launch
+  const bool keep = !(KIND == K_ORDER2 || KIND == K_ORDERN) || a.next[v] == 1;
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
     const size_t k = base + t + m * NT;
-    out[k] = o[m];
+    if (keep) out[k] = o[m];
     if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN)
       if (defer != 1) psi[k] = pv[m];
   }
