@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # in-place scale kernel of known size), averaged over the benchmark's own
 # steps; profiles/k_dense_L18_order.md and profiles/k_wht_L18_8states_order.md.
 MEASURED_TRAFFIC = {("k_dense<K_ORDERN, 18, 10>", 1): 42417 * 1024,
-                    ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8): 381926 * 1024}
+                    ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8): 384737 * 1024}
 BYTES_PER_AMP_ORDER = 88          # read T_{n-1}, T_{n-2}, diag; write T_n; read+write psi
 BYTES_FIRST, BYTES_SECOND = 40, 72  # order 1 (psi -> T_1), order 2 (fused psi update)
 
