@@ -150,7 +150,9 @@ def control_variate_leg(spx, ctx, L=10, target=1e-3):
     correction + surrogate draws), both actually run at the sample sizes the
     held-out variances prescribe. Training cost is reported beside it."""
     from spinsim_amd import surrogate as S
-    n_tr, n_va = 12288, 2048
+    # 24576 training curves: the held-out worst-point variance reduction is
+    # 8.0 against 6.3 with 12288 (same 60 epochs); training stays a one-off
+    n_tr, n_va = 24576, 2048
     log(f"[bench] control variate: training set of {n_tr} + {n_va} exact L={L} curves")
     Ytr, t_ytr = exact_curves(spx, ctx, L, 0, n_tr)
     Yva, t_yva = exact_curves(spx, ctx, L, n_tr, n_va)
