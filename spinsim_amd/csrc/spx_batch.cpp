@@ -469,14 +469,49 @@ int batch_run_resident(spx_batch* b, double* times, double* values) {
   if (rc == SPX_OK) rc = upload(c, &d_psi, psi);
   if (rc == SPX_OK) rc = upload(c, &d_out, std::vector<double>(nout ? nout : 1, 0.0));
   if (rc == SPX_OK) rc = hipStreamSynchronize(c->stream) == hipSuccess ? SPX_OK : fail(SPX_EDEVICE, "upload");
+  // One persistent launch per L. The launches after the first go on streams
+  // of their own (default priority), so that the tail of one size, when a
+  // few workgroups still run its last realisations, overlaps the next size's
+  // work. They write disjoint rows of d_out and count on their own queue
+  // words, and the stream of the context waits for all of them.
+  std::vector<hipStream_t> extra;
+  std::vector<hipEvent_t> events;
+  auto new_event = [&](hipEvent_t* e) -> int {
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return fail(SPX_EDEVICE, "hipEventCreate");
+    events.push_back(*e);
+    return SPX_OK;
+  };
+  hipEvent_t uploaded = nullptr;
+  if (rc == SPX_OK && groups.size() > 2) {
+    rc = new_event(&uploaded);
+    if (rc == SPX_OK && hipEventRecord(uploaded, c->stream) != hipSuccess) rc = fail(SPX_EDEVICE, "hipEventRecord");
+  }
   for (size_t gi = 0; rc == SPX_OK && gi + 1 < groups.size(); ++gi) {
     const int beg = groups[gi], cnt = groups[gi + 1] - groups[gi];
     const int L = reals[order[beg]].nspins;
     const int nblocks = std::min(cnt, 2 * c->arch_cus);
+    hipStream_t st = c->stream;
+    if (gi > 0) {
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        rc = fail(SPX_EDEVICE, "hipStreamCreate");
+        break;
+      }
+      extra.push_back(st);
+      if (hipStreamWaitEvent(st, uploaded, 0) != hipSuccess) {
+        rc = fail(SPX_EDEVICE, "hipStreamWaitEvent");
+        break;
+      }
+    }
     rc = launch_resident(c, L, d_ops, static_cast<int>(ops.size()), d_pulses, d_evo, d_reals,
                          d_order + beg, cnt, d_w, d_psi, d_out, static_cast<int>(b->width), d_codes,
                          d_spins, static_cast<int>(b->obs_codes.size()), b->reference_axis,
-                         d_queue + 2 * gi, nblocks);
+                         d_queue + 2 * gi, nblocks, st);
+    if (rc == SPX_OK && gi > 0) {
+      hipEvent_t done = nullptr;
+      rc = new_event(&done);
+      if (rc == SPX_OK && (hipEventRecord(done, st) != hipSuccess || hipStreamWaitEvent(c->stream, done, 0) != hipSuccess))
+        rc = fail(SPX_EDEVICE, "joining the per-size streams");
+    }
   }
   std::vector<int> flags(queue0.size(), 0);
   if (rc == SPX_OK) {
@@ -487,6 +522,13 @@ int batch_run_resident(spx_batch* b, double* times, double* values) {
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
       rc = fail(SPX_EDEVICE, std::string("resident batch: ") + hipGetErrorString(e3));
   }
+  // every launch has finished (or none was made) before the buffers go
+  for (hipStream_t st : extra) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+  (void)hipStreamSynchronize(c->stream);
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
   for (void* p : {static_cast<void*>(d_ops), static_cast<void*>(d_pulses), static_cast<void*>(d_evo),
                   static_cast<void*>(d_reals), static_cast<void*>(d_order), static_cast<void*>(d_codes),
                   static_cast<void*>(d_spins), static_cast<void*>(d_queue), static_cast<void*>(d_w),

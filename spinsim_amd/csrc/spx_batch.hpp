@@ -117,5 +117,5 @@ int launch_resident(spx_ctx* c, int L, const DevOp* ops, int nops, const DevPuls
                     const DevEvolve* evo, const DevReal* reals, const int* order, int nreal,
                     const double* weights, const double* psi0, double* out, int width,
                     const int* obs_codes, const int* obs_spins, int nobs, int reference_axis,
-                    int* queue, int nblocks);
+                    int* queue, int nblocks, hipStream_t stream);
 }  // namespace spx

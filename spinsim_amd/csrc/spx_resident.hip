@@ -572,7 +572,7 @@ int launch_resident(spx_ctx* c, int L, const DevOp* ops, int nops, const DevPuls
                     const DevEvolve* evo, const DevReal* reals, const int* order, int nreal,
                     const double* weights, const double* psi0, double* out, int width,
                     const int* obs_codes, const int* obs_spins, int nobs, int reference_axis,
-                    int* queue, int nblocks) {
+                    int* queue, int nblocks, hipStream_t stream) {
   RArgs g{ops, nops, pulses, evo, reals, order, nreal, weights, psi0, out, width,
           obs_codes, obs_spins, nobs, reference_axis, queue};
   const int vectors = L <= 11 ? 2 : 1;   // kVectors<L>
@@ -580,18 +580,18 @@ int launch_resident(spx_ctx* c, int L, const DevOp* ops, int nops, const DevPuls
                      sizeof(double2) * static_cast<size_t>(L * (L - 1) / 2 + L) +
                      sizeof(uint32_t) * static_cast<size_t>(L * (L - 1) / 2);
   switch (L) {
-    case 1: k_resident<1><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 2: k_resident<2><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 3: k_resident<3><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 4: k_resident<4><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 5: k_resident<5><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 6: k_resident<6><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 7: k_resident<7><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 8: k_resident<8><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 9: k_resident<9><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 10: k_resident<10><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 11: k_resident<11><<<nblocks, kBT, lds, c->stream>>>(g); break;
-    case 12: k_resident<12><<<nblocks, kBT, lds, c->stream>>>(g); break;
+    case 1: k_resident<1><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 2: k_resident<2><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 3: k_resident<3><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 4: k_resident<4><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 5: k_resident<5><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 6: k_resident<6><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 7: k_resident<7><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 8: k_resident<8><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 9: k_resident<9><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 10: k_resident<10><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 11: k_resident<11><<<nblocks, kBT, lds, stream>>>(g); break;
+    case 12: k_resident<12><<<nblocks, kBT, lds, stream>>>(g); break;
     default: return fail(SPX_EINVAL, "resident runner holds at most 12 spins");
   }
   SPX_HIP(hipGetLastError());
