@@ -99,88 +99,10 @@ __device__ __forceinline__ uint32_t sphase(uint32_t k) { return static_cast<uint
 // one 128-byte bank line.
 __device__ __forceinline__ uint32_t rslot(uint32_t i) { return i ^ ((i >> 3) & 7u); }
 
-struct RowSlot {
-  __device__ __forceinline__ uint32_t operator()(uint32_t i, uint32_t) const { return rslot(i); }
-};
-struct ColSlot {
-  __device__ __forceinline__ uint32_t operator()(uint32_t i, uint32_t c) const { return (i << 3) | c; }
-};
-
-// In-place unnormalised Walsh-Hadamard transform over NB index bits of
-// values in LDS, three bits per round (eight values per thread). Thread t
-// of a group of 2^(CB + NB - 3) threads owns batch column c = t mod 2^CB and
-// the NB - 3 index bits t >> CB; round b0 takes index bits [lo, lo + 3) into
-// registers (lo = b0, or NB - 3 for a short last round, whose lower bits are
-// already done). Every round ends with a workgroup barrier, so all threads
-// of the workgroup call this the same number of times.
-template <int NB, int CB, typename Slot>
-__device__ __forceinline__ void wht_rounds(double2* __restrict__ S, uint32_t t, Slot slot) {
-  static_assert(NB >= 3, "at least three transform bits");
-  const uint32_t c = t & ((1u << CB) - 1u);
-  const uint32_t u = t >> CB;
-#pragma unroll
-  for (int b0 = 0; b0 < NB; b0 += 3) {
-    const int lo = b0 + 3 <= NB ? b0 : NB - 3;
-    const uint32_t base = ((u >> lo) << (lo + 3)) | (u & ((1u << lo) - 1u));
-    double2 v[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = S[slot(base | (static_cast<uint32_t>(r) << lo), c)];
-#pragma unroll
-    for (int s = b0 - lo; s < 3; ++s) {
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        if (r & (1 << s)) continue;
-        const double2 a = v[r], b = v[r | (1 << s)];
-        v[r] = make_double2(a.x + b.x, a.y + b.y);
-        v[r | (1 << s)] = make_double2(a.x - b.x, a.y - b.y);
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r) S[slot(base | (static_cast<uint32_t>(r) << lo), c)] = v[r];
-    __syncthreads();
-  }
-}
-
 // Global loads of a pass are all issued into registers before the first
 // dependent LDS write or global store: the compiler cannot prove that the
 // state pointers do not alias, and would otherwise serialise the memory
 // round trips (load, wait, store, next load).
-
-// Pass 1: one workgroup per row tile x (2^TB consecutive amplitudes) and
-// vector; threads [0, 128) transform psi, [128, 256) transform S^+ psi.
-__global__ __launch_bounds__(kRowThreads) void k_wht_rows_fwd(WhtArgs a) {
-  constexpr uint32_t T = 1u << kTB, NT = kRowThreads, H = NT / 2, M = T / NT;
-  extern __shared__ double2 smem[];
-  const uint32_t t = threadIdx.x;
-  const int v = blockIdx.y;
-  const size_t base = static_cast<size_t>(blockIdx.x) << kTB;
-  const double2* __restrict__ in = a.in[v];
-  double2 x[M];
-#pragma unroll
-  for (uint32_t m = 0; m < M; ++m) x[m] = in[base + t + m * NT];
-#pragma unroll
-  for (uint32_t m = 0; m < M; ++m) {
-    const uint32_t i = t + m * NT;
-    smem[rslot(i)] = x[m];
-    smem[T + rslot(i)] = mul_ipow(x[m], sdag(static_cast<uint32_t>(base + i)));
-  }
-  __syncthreads();
-  const uint32_t half = t >= H ? 1u : 0u;
-  wht_rounds<kTB, 0>(smem + half * T, t - half * H, RowSlot{});
-  double2 yu[M], yv[M];
-#pragma unroll
-  for (uint32_t m = 0; m < M; ++m) {
-    yu[m] = smem[rslot(t + m * NT)];
-    yv[m] = smem[T + rslot(t + m * NT)];
-  }
-  double2* __restrict__ U = a.u[v];
-  double2* __restrict__ V = a.w[v];
-#pragma unroll
-  for (uint32_t m = 0; m < M; ++m) {
-    U[base + t + m * NT] = yu[m];
-    V[base + t + m * NT] = yv[m];
-  }
-}
 
 // Butterflies of one round on the thread's eight values v[r] (r = its three
 // register bits): stages s in [S0, S1).
@@ -195,6 +117,86 @@ __device__ __forceinline__ void butterflies(double2 (&v)[8]) {
       v[r] = make_double2(a.x + b.x, a.y + b.y);
       v[r | (1 << s)] = make_double2(a.x - b.x, a.y - b.y);
     }
+  }
+}
+
+// Row tiles with index bits 8 and 9 in registers (the four values a thread
+// holds, k = t + 256 m): those two butterfly stages run on the registers and
+// only bits 0..7 go through LDS, in three rounds over four 256-value batches.
+__device__ __forceinline__ void bfly_m(double2 (&x)[4]) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (m & (1 << s)) continue;
+      const double2 a = x[m], b = x[m | (1 << s)];
+      x[m] = make_double2(a.x + b.x, a.y + b.y);
+      x[m | (1 << s)] = make_double2(a.x - b.x, a.y - b.y);
+    }
+}
+// One LDS round over index bits [LO, LO + 3) of a 256-value batch: the thread
+// (batch, sub) takes eight values into registers and runs stages S0..2 of
+// them (a stage below S0 was done by the round before).
+template <int LO, int S0>
+__device__ __forceinline__ void low8_round(double2* __restrict__ S, uint32_t batch, uint32_t sub) {
+  const uint32_t base = (batch << 8) | ((sub >> LO) << (LO + 3)) | (sub & ((1u << LO) - 1u));
+  double2 v[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = S[rslot(base | (static_cast<uint32_t>(r) << LO))];
+  butterflies<S0, 3>(v);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) S[rslot(base | (static_cast<uint32_t>(r) << LO))] = v[r];
+  __syncthreads();
+}
+// u = thread index within the half (128 threads, 8 values each)
+__device__ __forceinline__ void wht_low8(double2* S, uint32_t u) {
+  const uint32_t batch = u >> 5, sub = u & 31u;
+  low8_round<0, 0>(S, batch, sub);
+  low8_round<3, 0>(S, batch, sub);
+  low8_round<5, 1>(S, batch, sub);
+}
+
+// Pass 1: one workgroup per row tile x (2^TB consecutive amplitudes) and
+// vector; threads [0, 128) transform psi, [128, 256) transform S^+ psi.
+__global__ __launch_bounds__(kRowThreads) void k_wht_rows_fwd(WhtArgs a) {
+  constexpr uint32_t T = 1u << kTB, NT = kRowThreads, H = NT / 2, M = T / NT;
+  extern __shared__ double2 smem[];
+  const uint32_t t = threadIdx.x;
+  const int v = blockIdx.y;
+  const size_t base = static_cast<size_t>(blockIdx.x) << kTB;
+  const double2* __restrict__ in = a.in[v];
+  double2 x[M];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) x[m] = in[base + t + m * NT];
+  static_assert(M == 4 && NT == 256, "bits 8 and 9 of the row tile are the register index m");
+  double2 yu[M], yv[M];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint32_t i = t + m * NT;
+    yu[m] = x[m];
+    yv[m] = mul_ipow(x[m], sdag(static_cast<uint32_t>(base + i)));
+  }
+  bfly_m(yu);
+  bfly_m(yv);
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    smem[rslot(t + m * NT)] = yu[m];
+    smem[T + rslot(t + m * NT)] = yv[m];
+  }
+  __syncthreads();
+  const uint32_t half = t >= H ? 1u : 0u;
+  wht_low8(smem + half * T, t - half * H);
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    yu[m] = smem[rslot(t + m * NT)];
+    yv[m] = smem[T + rslot(t + m * NT)];
+  }
+  double2* __restrict__ U = a.u[v];
+  double2* __restrict__ V = a.w[v];
+#pragma unroll
+  for (uint32_t m = 0; m < M; ++m) {
+    U[base + t + m * NT] = yu[m];
+    V[base + t + m * NT] = yv[m];
   }
 }
 
@@ -338,6 +340,9 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
     if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN)
       if (defer != 1) pv[m] = psi[k];
   }
+  static_assert(M == 4 && NT == 256, "bits 8 and 9 of the row tile are the register index m");
+  bfly_m(xu);
+  bfly_m(xw);
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
     smem[rslot(t + m * NT)] = xu[m];
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   }
   __syncthreads();
   const uint32_t half = t >= H ? 1u : 0u;
-  wht_rounds<kTB, 0>(smem + half * T, t - half * H, RowSlot{});
+  wht_low8(smem + half * T, t - half * H);
 #pragma unroll
   for (uint32_t m = 0; m < M; ++m) {
     xu[m] = smem[rslot(t + m * NT)];
@@ -419,11 +424,18 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
         double2 src = o[m];
         if constexpr (KIND == K_ORDER2 || KIND == K_ORDERN)
           if (a.next[v] == 2) src = pv[m];
-        smem[rslot(i)] = src;
-        smem[T + rslot(i)] = mul_ipow(src, sdag(static_cast<uint32_t>(base + i)));
+        xu[m] = src;
+        xw[m] = mul_ipow(src, sdag(static_cast<uint32_t>(base + i)));
+      }
+      bfly_m(xu);
+      bfly_m(xw);
+#pragma unroll
+      for (uint32_t m = 0; m < M; ++m) {
+        smem[rslot(t + m * NT)] = xu[m];
+        smem[T + rslot(t + m * NT)] = xw[m];
       }
       __syncthreads();
-      wht_rounds<kTB, 0>(smem + half * T, t - half * H, RowSlot{});
+      wht_low8(smem + half * T, t - half * H);
 #pragma unroll
       for (uint32_t m = 0; m < M; ++m) {
         xu[m] = smem[rslot(t + m * NT)];
