@@ -33,12 +33,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "H·ψ GB/s vs HBM peak at L=18; s to ±1e-3 echo error bar, CV vs plain MC"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per general-order launch at L=18, keyed by (order kernels, states
-# per launch): FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from the
-# in-place scale kernel of known size), averaged over the benchmark's own
-# steps; profiles/k_dense_L18_order.md and profiles/k_wht_L18_8states_order.md.
-MEASURED_TRAFFIC = {("k_dense<K_ORDERN, 18, 10>", 1): 42417 * 1024,
-                    ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8): 384737 * 1024}
+# HBM bytes per general order over the set at L=18, keyed by (order kernels,
+# states in the set, states per launch): FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
+# (factors from the in-place scale kernel of known size), summed over the
+# set's launches and averaged over the benchmark's own steps;
+# profiles/k_dense_L18_order.md and profiles/k_wht_L18_8states_order.md.
+MEASURED_TRAFFIC = {("k_dense<K_ORDERN, 18, 10>", 1, 1): 42417 * 1024,
+                    ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8, 8): 384722 * 1024,
+                    ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8, 4): 385260 * 1024}
 BYTES_PER_AMP_ORDER = 88          # read T_{n-1}, T_{n-2}, diag; write T_n; read+write psi
 BYTES_FIRST, BYTES_SECOND = 40, 72  # order 1 (psi -> T_1), order 2 (fused psi update)
 
@@ -311,6 +313,8 @@ def main():
     ap.add_argument("--set", type=int, default=8, metavar="S",
                     help="realisations stepped together, one launch per order (1..8; 1 = a single state)")
     ap.add_argument("--graphs", action="store_true", help="replay each step as a captured HIP graph")
+    ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
+                    help="2: a set of 4..8 runs as two halves on two streams (the default); 1: one stream")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="L",
                     help="only time the in-place scale kernel on 2^L amplitudes")
     args = ap.parse_args()
@@ -344,15 +348,18 @@ def main():
     coeffs = [c for c, _ in built]
     effs = [e for _, e in built]
     orders = [c.size - 1 for c in coeffs]
+    # states per launch: with two streams a set of 4..8 runs as two halves
+    W = (S + 1) // 2 if args.streams == 2 and S >= 4 else S
     tune_report = None
     if not args.no_tune:
-        # the schedule for launches of S states; the other realisations adopt
+        # the schedule for launches of W states; the other realisations adopt
         # it from the tuner's cache (same shape and width)
-        tune_report = hams[0].autotune(effs[0], nvec=S)
+        tune_report = hams[0].autotune(effs[0], nvec=W)
         spx.autotune_mode(1)
-        log(f"[bench] autotune (width {S}): {tune_report[:300]}")
+        log(f"[bench] autotune (width {W}): {tune_report[:300]}")
     prop = spx.Propagator(ctx, L, args.epsilon)
     prop.set_graphs(args.graphs)
+    prop.set_streams(args.streams)
     log(f"[bench] bounds={[round(h.spectral_bound(), 3) for h in hams]} orders={orders}")
 
     prop.steps_set(hams, spx.REAL, states, coeffs, effs, args.warmup)
@@ -371,8 +378,8 @@ def main():
     # Dominant launch: a general order over the S states (T_n = 2 Hhat T_{n-1}
     # - T_{n-2}; psi += c T_n). Its time is the difference between two runs of
     # the same launch stream whose expansions are cut to common lengths m and
-    # s < m, so every launch carries all S states and the first- and
-    # second-order launches cancel out.
+    # s < m, so every launch carries all the states of its set (or half) and
+    # the first- and second-order launches cancel out.
     reps = max(20, args.steps // 4)
     m = min(orders)
     sh = max(3, m // 2)
@@ -414,15 +421,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"L={L} dipolar XXZ real-time Chebyshev steps, dt={args.dt}, "
                                f"eps={args.epsilon:g}, autonormalize off; {S} realisations (seed {SEED}, "
-                               f"index 0..{S - 1}) stepped together, one launch per order for the set",
+                               f"index 0..{S - 1}) stepped together, " + (
+                                   f"one launch per order for each half of {W} on its own stream" if W < S
+                                   else "one launch per order for the set"),
                    "nspins": L, "pairs": int(lo.size), "realisations": S, "orders": [int(o) for o in orders],
                    "spectral_bounds": [round(h.spectral_bound(), 4) for h in hams],
                    "schedule": tune_report.split(";")[0] if tune_report else "default"},
         "hpsi_per_s": round(args.steps * sum(orders) / (ms * 1e-3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((kernel_name, S)),
-                     "kernel": kernel_name + f" (general Chebyshev order, {S} states per launch)",
+                     "traffic": MEASURED_TRAFFIC.get((kernel_name, S, W)),
+                     "kernel": kernel_name + (f" (general Chebyshev order over {S} states: two halves of {W}, "
+                                              "one launch sequence each, on two streams)" if W < S else
+                                              f" (general Chebyshev order, {S} states per launch)"),
                      "bytes_per_launch": kernel_bytes, "us_per_launch": round(per_order_ms * 1e3, 3)},
     }
     if single:

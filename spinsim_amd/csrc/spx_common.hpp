@@ -77,6 +77,10 @@ struct spx_ctx {
   double* h_result = nullptr;     // pinned [16]
   hipEvent_t events[spx::kMaxEvents] = {};
   int arch_cus = 0;
+  // second stream of a set evolve split in two (created on first use), and
+  // the events that order it after, and join it back into, `stream`
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 struct spx_state {
@@ -131,6 +135,7 @@ struct spx_prop {
   size_t last_order = 0;
   double2* d_scratch[2 * spx::kMaxMany] = {};
   int graphs = 0;   // record one step as a HIP graph and replay it
+  int streams = 2;  // 2: a set evolve may run as two halves on two streams
 };
 
 namespace spx {

@@ -121,6 +121,7 @@ SIGNATURES = {
     "spx_ham_autotune_batch": (C.c_int, [_vp, C.c_double, C.c_int, C.c_char_p, C.c_size_t]),
     "spx_prop_last_order": (C.c_int, [_vp, _szp]),
     "spx_prop_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "spx_prop_set_streams": (C.c_int, [_vp, C.c_int]),
     "spx_prop_epsilon": (C.c_int, [_vp, _dp]),
     "spx_prop_steps": (C.c_int, [_vp, _vp, C.c_int, _vp, _dp, C.c_size_t, C.c_double, C.c_size_t,
                                  C.c_int]),
@@ -430,6 +431,11 @@ class Propagator:
 
     def set_graphs(self, enable: bool) -> None:
         check(lib().spx_prop_set_graphs(self.h, int(enable)))
+
+    def set_streams(self, streams: int) -> None:
+        """2 (default): a set of 4..8 states on the Hadamard path runs as two
+        overlapping halves on two streams; 1: one stream."""
+        check(lib().spx_prop_set_streams(self.h, int(streams)))
 
     def evolve_many(self, ham: Hamiltonian, mode: int, states: Sequence[StateVector], tau: float,
                     nsteps: int = 1, autonormalize: bool = False):

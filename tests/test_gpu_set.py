@@ -102,6 +102,32 @@ def test_l18_hadamard_set_of_four(ctx):
     check_set(ctx, L, [(i, 1.0, False) for i in range(4)], 2, nsteps=2)
 
 
+@pytest.mark.parametrize("count", [4, 7, 8])
+def test_two_stream_halves_bitwise_equal_to_one_stream(ctx, count):
+    """A set of 4..8 states under distinct Hamiltonians on the Hadamard path
+    runs as two halves on two streams (the default); with one stream it runs
+    as one launch group. The halves must give the same bits, also over
+    chained steps, and the context's stream must see their results (the
+    download below is ordered on it)."""
+    L = 16
+    banner(f"evolve_set L={L} dense=2, {count} states, two streams vs one")
+    specs = [(i, 1.0 + 0.5 * (i % 3), i == 2) for i in range(count)]
+    hams = [device_ham(ctx, L, *realisation(L, i, s, t), 2) for i, s, t in specs]
+    psis = [O.random_state(L, 700 + k) for k in range(count)]
+    out = {}
+    for streams in (1, 2):
+        prop = spx.Propagator(ctx, L, 1e-12)
+        prop.set_streams(streams)
+        states = [spx.StateVector(ctx, L, p) for p in psis]
+        prop.evolve_set(hams, spx.REAL, states, 0.05, 3)
+        out[streams] = [st.download() for st in states]
+    for k in range(count):
+        assert np.array_equal(out[1][k], out[2][k]), k
+    assert all(h.kernel_name().startswith("k_wht_") for h in hams)
+    with pytest.raises(spx.InvalidArgument):
+        spx.Propagator(ctx, L, 1e-12).set_streams(3)
+
+
 def test_same_hamiltonian_twice_and_argument_errors(ctx):
     L = 14
     fields, c = realisation(L, 9)
