@@ -166,10 +166,11 @@ int spx_prop_last_order(const spx_prop* p, size_t* order);                      
  * result); on this part the host launches already keep the device busy. */
 int spx_prop_set_graphs(spx_prop* p, int enable);
 /* No reference counterpart: with streams = 2 (the default), a set evolve
- * (spx_prop_evolve_set, spx_prop_steps_set) of 4..8 states under distinct
- * Hamiltonians on the Hadamard path, without autonormalisation, runs as two
- * halves on the context's stream and a second stream of the context, so that
- * their launches overlap; the second stream is joined back before return.
+ * (spx_prop_evolve_set, spx_prop_steps_set) of 4..8 states on the Hadamard
+ * path, without autonormalisation, that splits into two parts of two or more
+ * states with no Hamiltonian in both, runs as those two halves on the
+ * context's stream and a second stream of the context, so that their
+ * launches overlap; the second stream is joined back before return.
  * streams = 1 issues every launch on the context's stream. Results are
  * bitwise the same either way. */
 int spx_prop_set_streams(spx_prop* p, int streams);

@@ -692,15 +692,17 @@ int prepare_set(spx_ham* const* hams, const std::vector<size_t>& idx, const std:
 //
 // Two streams: an order over a set is two dependent launches (column pass,
 // then row pass), and each ends in a tail where its last workgroups drain
-// while the rest of the chip idles. A set of 4..kMaxMany states under
-// distinct Hamiltonians on the Hadamard path therefore runs as two halves,
-// one on the context's stream and one on a second stream, so that one
-// half's tails overlap the other half's passes (at L = 18, eight states: 10%
-// less time per step). Each half is what the set's own launches would do to
-// its states (bitwise the same results); the halves share no buffer (each
-// Hamiltonian's transform scratch serves one state, the propagator's scratch
-// pairs are split by position). The second stream starts after the work
-// already on the context's stream and is joined back into it before return.
+// while the rest of the chip idles. A set of 4..kMaxMany states on the
+// Hadamard path that splits into two parts of at least two states with no
+// Hamiltonian in both (distinct realisations, or each realisation's state
+// and reference side by side) therefore runs as two halves, one on the
+// context's stream and one on a second stream, so that one half's tails
+// overlap the other half's passes (at L = 18, eight states: 10% less time
+// per step). Each half is what the set's own launches would do to its states
+// (bitwise the same results); the halves share no buffer (a Hamiltonian's
+// transform scratch serves its own states, the propagator's scratch pairs
+// are split by position). The second stream starts after the work already
+// on the context's stream and is joined back into it before return.
 int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* states, size_t count,
                     const std::vector<std::vector<double>>& coeffs, const std::vector<double>& bounds,
                     size_t nsteps, int autonormalize) {
@@ -719,16 +721,24 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
   std::stable_sort(idx.begin(), idx.end(),
                    [&](size_t x, size_t y) { return coeffs[x].size() > coeffs[y].size(); });
   const size_t n = idx.size();
-  bool distinct = true;
-  for (size_t a = 0; a < n; ++a)
-    for (size_t b = 0; b < a; ++b) distinct = distinct && hams[idx[a]] != hams[idx[b]];
-  const bool try_split = p->streams == 2 && !autonormalize && distinct && n >= 4 && n <= kMaxMany;
+  // split point for two streams: the most even one that leaves no
+  // Hamiltonian on both sides (0: none)
+  auto separates = [&](size_t h) {
+    for (size_t x = 0; x < h; ++x)
+      for (size_t y = h; y < n; ++y)
+        if (hams[idx[x]] == hams[idx[y]]) return false;
+    return true;
+  };
+  size_t half = 0;
+  if (p->streams == 2 && !autonormalize && n >= 4 && n <= static_cast<size_t>(kMaxMany))
+    for (size_t h = 2; h + 2 <= n; ++h)
+      if (separates(h) && (half == 0 || std::max(h, n - h) < std::max(half, n - half))) half = h;
   int width = static_cast<int>(n < static_cast<size_t>(kMaxMany) ? n : kMaxMany);
-  if (try_split) width = static_cast<int>((n + 1) / 2);
+  if (half) width = static_cast<int>(std::max(half, n - half));
   SPX_TRY(prepare_set(hams, idx, bounds, width));
-  bool split = try_split;
+  bool split = half != 0;
   for (size_t i : idx) split = split && hams[i]->wht_ok;
-  if (try_split && !split) {
+  if (half && !split) {
     width = static_cast<int>(n);
     SPX_TRY(prepare_set(hams, idx, bounds, width));
   }
@@ -738,26 +748,44 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
       for (size_t j : idx) copies += hams[j] == hams[i] ? 1 : 0;
       SPX_TRY(hadamard_reserve(hams[i], copies < kMaxMany ? copies : kMaxMany));
     }
-  // groups of up to `width` consecutive states whose Hamiltonians can share launches
-  std::vector<std::pair<size_t, size_t>> groups;
-  for (size_t g = 0; g < n;) {
-    size_t e = g + 1;
-    while (e < n && e - g < static_cast<size_t>(width)) {
-      spx_ham* hs[kMaxMany];
-      for (size_t k = g; k <= e; ++k) hs[k - g] = hams[idx[k]];
-      if (!order_set_compatible(hs, e - g + 1)) break;
-      ++e;
+  // groups of up to `width` consecutive states whose Hamiltonians can share
+  // launches, within each half when split; the third member says which
+  // stream a group's launches go to
+  struct Group {
+    size_t g, e;
+    bool side;
+  };
+  std::vector<Group> groups;
+  auto add_groups = [&](size_t beg, size_t end, bool side) {
+    for (size_t g = beg; g < end;) {
+      size_t e = g + 1;
+      while (e < end && e - g < static_cast<size_t>(width)) {
+        spx_ham* hs[kMaxMany];
+        for (size_t k = g; k <= e; ++k) hs[k - g] = hams[idx[k]];
+        if (!order_set_compatible(hs, e - g + 1)) break;
+        ++e;
+      }
+      groups.push_back({g, e, side});
+      g = e;
     }
-    groups.emplace_back(g, e);
-    g = e;
+  };
+  if (split) {
+    add_groups(0, half, false);
+    add_groups(half, n, true);
+  } else {
+    add_groups(0, n, false);
   }
   // On the Hadamard path, with no normalisation between steps and every
   // expansion of order 2 or more, each step's last orders leave the row
   // transforms of the finished states for the next step's first order, which
   // then skips its forward row pass. This needs each Hamiltonian's scratch
-  // slots to serve one launch group: one group, or distinct Hamiltonians.
-  bool chain = (groups.size() == 1 || distinct) && !autonormalize && nsteps > 1;
+  // slots to serve one launch group: no Hamiltonian in two groups.
+  bool chain = !autonormalize && nsteps > 1;
   for (size_t i : idx) chain = chain && hams[i]->wht_ok && coeffs[i].size() > 2;
+  for (size_t a = 0; a < groups.size() && chain; ++a)
+    for (size_t b = 0; b < a && chain; ++b)
+      for (size_t x = groups[a].g; x < groups[a].e && chain; ++x)
+        for (size_t y = groups[b].g; y < groups[b].e && chain; ++y) chain = hams[idx[x]] != hams[idx[y]];
   if (split) {
     SPX_TRY(ensure_side(p->ctx));
     SPX_HIP(hipEventRecord(p->ctx->fork, p->ctx->stream));
@@ -765,8 +793,9 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
   }
   int rc = SPX_OK;
   for (size_t step = 0; step < nsteps && rc == SPX_OK; ++step) {
-    for (size_t gi = 0; gi < groups.size() && rc == SPX_OK; ++gi) {
-      const auto [g, e] = groups[gi];
+    for (const Group& grp : groups) {
+      if (rc != SPX_OK) break;
+      const size_t g = grp.g, e = grp.e;
       spx_ham* hs[kMaxMany];
       spx_state* ss[kMaxMany];
       const std::vector<double>* cs[kMaxMany];
@@ -775,7 +804,7 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
         ss[k - g] = states[idx[k]];
         cs[k - g] = &coeffs[idx[k]];
       }
-      OnStream on(p->ctx, split && gi % 2 == 1 ? p->ctx->side : p->ctx->stream);
+      OnStream on(p->ctx, grp.side ? p->ctx->side : p->ctx->stream);
       rc = one_step_set(p, hs, mode, ss, e - g, cs, chain && step > 0, chain && step + 1 < nsteps,
                         split ? g : 0);
     }

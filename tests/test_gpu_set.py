@@ -128,6 +128,28 @@ def test_two_stream_halves_bitwise_equal_to_one_stream(ctx, count):
         spx.Propagator(ctx, L, 1e-12).set_streams(3)
 
 
+@pytest.mark.parametrize("pairs", [2, 3, 4])
+def test_two_stream_halves_with_state_and_reference_pairs(ctx, pairs):
+    """Each realisation's state and reference under its one Hamiltonian, side
+    by side (as the batch walker passes them): the halves split between
+    pairs (three pairs: four states and two) and match one stream bitwise
+    over chained steps."""
+    L = 16
+    banner(f"evolve_set L={L} dense=2, {pairs} state/reference pairs, two streams vs one")
+    hams = [device_ham(ctx, L, *realisation(L, i, 1.0 + 0.5 * i, i == 1), 2) for i in range(pairs)]
+    hs = [h for h in hams for _ in range(2)]
+    psis = [O.random_state(L, 800 + k) for k in range(2 * pairs)]
+    out = {}
+    for streams in (1, 2):
+        prop = spx.Propagator(ctx, L, 1e-12)
+        prop.set_streams(streams)
+        states = [spx.StateVector(ctx, L, p) for p in psis]
+        prop.evolve_set(hs, spx.REAL, states, 0.05, 3)
+        out[streams] = [st.download() for st in states]
+    for k in range(2 * pairs):
+        assert np.array_equal(out[1][k], out[2][k]), k
+
+
 def test_same_hamiltonian_twice_and_argument_errors(ctx):
     L = 14
     fields, c = realisation(L, 9)
