@@ -285,6 +285,7 @@ def copy_probe(spx, L, reps=50):
     kernel that calibrates the HBM counters and the achievable bandwidth."""
     ctx = spx.Context(0)
     st = spx.StateVector(ctx, L, np.ones(2 << L) / math.sqrt(float(2 << L)))
+    out = {"probe": "k_scale", "nspins": L}
     for _ in range(5):
         st.scale(1.0)
     ctx.record(0)
@@ -294,8 +295,9 @@ def copy_probe(spx, L, reps=50):
     ctx.synchronize()
     us = ctx.elapsed_ms(0, 1) * 1e3 / reps
     nbytes = 32 * (1 << L)
-    print(json.dumps({"probe": "k_scale", "nspins": L, "bytes_per_launch": nbytes, "us_per_launch": round(us, 3),
-                      "GB/s": round(nbytes / (us * 1e-6) / 1e9, 1)}), flush=True)
+    out.update({"bytes_per_launch": nbytes, "us_per_launch": round(us, 3),
+                "GB/s": round(nbytes / (us * 1e-6) / 1e9, 1)})
+    return out
 
 
 def main():
@@ -326,7 +328,7 @@ def main():
 
     from spinsim_amd import spx
     if args.copy_probe:
-        copy_probe(spx, args.copy_probe)
+        print(json.dumps(copy_probe(spx, args.copy_probe)), flush=True)
         return
     L = args.nspins
     n = 1 << L
@@ -396,6 +398,10 @@ def main():
     achieved = kernel_bytes / (per_order_ms * 1e-3) / 1e9
     log(f"[bench] general order over {S} states: {per_order_ms * 1e3:.2f} us -> {achieved:.1f} GB/s")
     spx.autotune_mode(0)
+    # the achievable streaming rate beside the datasheet peak: the in-place
+    # scale kernel over 2^24 amplitudes (256 MiB each way, past every cache)
+    copy = copy_probe(spx, 24)
+    log(f"[bench] copy probe: {copy}")
 
     single = None
     if S > 1:
@@ -434,7 +440,8 @@ def main():
                      "kernel": kernel_name + (f" (general Chebyshev order over {S} states: two halves of {W}, "
                                               "one launch sequence each, on two streams)" if W < S else
                                               f" (general Chebyshev order, {S} states per launch)"),
-                     "bytes_per_launch": kernel_bytes, "us_per_launch": round(per_order_ms * 1e3, 3)},
+                     "bytes_per_launch": kernel_bytes, "us_per_launch": round(per_order_ms * 1e3, 3),
+                     "copy_GBs": copy["GB/s"], "frac_of_copy": round(achieved / copy["GB/s"], 4)},
     }
     if single:
         line["single_state"] = single
