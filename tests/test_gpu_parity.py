@@ -428,6 +428,44 @@ def test_imaginary_time_relaxes_to_the_ground_state(ctx):
     assert overlap > 1.0 - 1e-9
 
 
+def test_l18_drift_over_1e4_steps_is_the_expansions_own(ctx):
+    """The single-large-state drift check at full length: eight L = 18
+    realisations (indices 0..7, two halves of four on two streams), 1e4 real
+    steps of 0.05 at eps 1e-12, autonormalisation off. The truncated
+    expansion changes the norm and the energy by a fixed amount per step (it
+    depends only on the state's spectral weights, which the evolution keeps),
+    so after 1e4 steps each realisation's drift must be 5000 times what the
+    oracle's own two steps give for it: the device reproduces the method's
+    drift to 1e-3, and adds none of its own."""
+    L, n = 18, 1 << 18
+    steps = 10_000
+    banner("L=18 drift: oracle two steps per realisation (CPU, about 10 s)")
+    hams, states, want_dn, want_de, n0, e0 = [], [], [], [], [], []
+    for i in range(8):
+        fields, c = dipolar_system(L, i)
+        psi = O.random_state(L, 900 + i)
+        oh = O.Ham(L, fields, c)
+        energy = lambda v: float(np.dot(v, oh.apply_physical(v)))
+        x = psi.copy()
+        rc, _ = oh.evolve(x, 0.05, 2, eps=1e-12)
+        assert rc == 0
+        want_dn.append((np.linalg.norm(x) - np.linalg.norm(psi)) * steps / 2)
+        want_de.append((energy(x) - energy(psi)) * steps / 2)
+        hams.append(device_ham(ctx, L, fields, c))
+        states.append(spx.StateVector(ctx, L, psi))
+        n0.append(states[-1].norm())
+        e0.append(spx.measure_energy(hams[-1], states[-1]))
+    banner("L=18 drift: 1e4 steps of eight realisations on the device")
+    prop = spx.Propagator(ctx, L, 1e-12)
+    for _ in range(10):
+        prop.evolve_set(hams, spx.REAL, states, 0.05, steps // 10)
+    for i in range(8):
+        dn = states[i].norm() - n0[i]
+        de = spx.measure_energy(hams[i], states[i]) - e0[i]
+        assert abs(dn - want_dn[i]) <= 1e-3 * abs(want_dn[i]) + 1e-12, (i, dn, want_dn[i])
+        assert abs(de - want_de[i]) <= 1e-3 * abs(want_de[i]) + 1e-12, (i, de, want_de[i])
+
+
 def test_imaginary_time_ground_state_l18(ctx):
     """The ground-state run of the single-large-state workload at its own size:
     L = 18, realisation 7 of the dipolar ensemble (153 pairs), a state and a
