@@ -101,10 +101,17 @@ def cpu_baseline(L, fields, lo, hi, jx, jy, jz, order_bytes, target_s=15.0):
     wall = run(orders)
     total_orders = orders * threads
     gbs = total_orders * order_bytes / wall / 1e9
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": kind,
             "sample": f"{orders} Chebyshev orders x {threads} threads (one L={L} state each), "
                       f"{wall:.1f} s wall; {wall / orders * 1e3:.1f} ms per order per thread",
-            "ms_per_order_per_thread": round(wall / orders * 1e3, 3)}
+            "ms_per_order_per_thread": round(wall / orders * 1e3, 3),
+            "cpu_model": model, "host_cpus_visible": len(os.sched_getaffinity(0))}
 
 
 # ------------------------------------------------ control-variate leg
