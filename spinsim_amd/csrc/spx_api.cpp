@@ -830,6 +830,10 @@ int check_set(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* sta
     if (states[i]->nspins != hams[i]->nspins) return fail(SPX_EINVAL, "state and Hamiltonian disagree on spin count");
     if (states[i]->nspins != p->nspins) return fail(SPX_EINVAL, "state does not match the propagator's size");
     if (hams[i]->ctx != p->ctx) return fail(SPX_EINVAL, "Hamiltonian belongs to another context");
+    // the launches and the join of a split set are ordered on the
+    // propagator's context; a state read through another context's stream
+    // would not wait for them
+    if (states[i]->ctx != p->ctx) return fail(SPX_EINVAL, "state belongs to another context");
     for (size_t j = 0; j < i; ++j)
       if (states[j] == states[i]) return fail(SPX_EINVAL, "a state appears twice");
   }

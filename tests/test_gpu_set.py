@@ -169,6 +169,11 @@ def test_same_hamiltonian_twice_and_argument_errors(ctx):
     other = spx.StateVector(ctx, L + 1, O.random_state(L + 1, 3))
     with pytest.raises(spx.InvalidArgument):
         prop.evolve_set([h], spx.REAL, [other], 0.05, 1)
+    # a state of another context: its stream would not wait for the set's launches
+    ctx2 = spx.Context(0)
+    foreign = spx.StateVector(ctx2, L, O.random_state(L, 4))
+    with pytest.raises(spx.InvalidArgument, match="another context"):
+        prop.evolve_set([h], spx.REAL, [foreign], 0.05, 1)
 
 
 def test_steps_set_over_prebuilt_expansions_matches_evolve_set(ctx):
