@@ -692,17 +692,22 @@ int prepare_set(spx_ham* const* hams, const std::vector<size_t>& idx, const std:
 //
 // Two streams: an order over a set is two dependent launches (column pass,
 // then row pass), and each ends in a tail where its last workgroups drain
-// while the rest of the chip idles. A set of 4..kMaxMany states on the
-// Hadamard path that splits into two parts of at least two states with no
-// Hamiltonian in both (distinct realisations, or each realisation's state
-// and reference side by side) therefore runs as two halves, one on the
-// context's stream and one on a second stream, so that one half's tails
-// overlap the other half's passes (at L = 18, eight states: 10% less time
-// per step). Each half is what the set's own launches would do to its states
-// (bitwise the same results); the halves share no buffer (a Hamiltonian's
-// transform scratch serves its own states, the propagator's scratch pairs
-// are split by position). The second stream starts after the work already
-// on the context's stream and is joined back into it before return.
+// while the rest of the chip idles. Sets on the Hadamard path therefore run
+// on the context's stream and a second stream, so that one part's tails
+// overlap the other part's passes:
+//   * 4..kMaxMany states that split into two parts of at least two states
+//     with no Hamiltonian in both (distinct realisations, or each
+//     realisation's state and reference side by side) run as those two
+//     halves (at L = 18, eight states: 10% less time per step);
+//   * more than kMaxMany states, with no Hamiltonian in two launch groups,
+//     run their groups alternately on the two streams, in groups of half the
+//     set up to 2 kMaxMany states (at L = 18, sixteen states: 10%).
+// Each group is what the set's own launches would do to its states (bitwise
+// the same results); groups on different streams share no buffer (a
+// Hamiltonian's transform scratch serves its own states; the propagator's
+// scratch pairs are split by position, or by stream). The second stream
+// starts after the work already on the context's stream and is joined back
+// into it before return.
 int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* const* states, size_t count,
                     const std::vector<std::vector<double>>& coeffs, const std::vector<double>& bounds,
                     size_t nsteps, int autonormalize) {
@@ -729,17 +734,22 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
         if (hams[idx[x]] == hams[idx[y]]) return false;
     return true;
   };
+  const size_t kM = static_cast<size_t>(kMaxMany);
+  const bool two = p->streams == 2 && !autonormalize;
   size_t half = 0;
-  if (p->streams == 2 && !autonormalize && n >= 4 && n <= static_cast<size_t>(kMaxMany))
+  if (two && n >= 4 && n <= kM)
     for (size_t h = 2; h + 2 <= n; ++h)
       if (separates(h) && (half == 0 || std::max(h, n - h) < std::max(half, n - half))) half = h;
-  int width = static_cast<int>(n < static_cast<size_t>(kMaxMany) ? n : kMaxMany);
+  const bool alternate = two && n > kM;   // groups alternate between the streams
+  const int plain_width = static_cast<int>(n < kM ? n : kM);
+  int width = plain_width;
   if (half) width = static_cast<int>(std::max(half, n - half));
+  if (alternate && n <= 2 * kM) width = static_cast<int>((n + 1) / 2);
   SPX_TRY(prepare_set(hams, idx, bounds, width));
-  bool split = half != 0;
+  bool split = half != 0 || alternate;
   for (size_t i : idx) split = split && hams[i]->wht_ok;
-  if (half && !split) {
-    width = static_cast<int>(n);
+  if ((half != 0 || alternate) && !split && width != plain_width) {
+    width = plain_width;
     SPX_TRY(prepare_set(hams, idx, bounds, width));
   }
   for (size_t i : idx)
@@ -769,7 +779,7 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
       g = e;
     }
   };
-  if (split) {
+  if (split && half) {
     add_groups(0, half, false);
     add_groups(half, n, true);
   } else {
@@ -780,12 +790,18 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
   // transforms of the finished states for the next step's first order, which
   // then skips its forward row pass. This needs each Hamiltonian's scratch
   // slots to serve one launch group: no Hamiltonian in two groups.
-  bool chain = !autonormalize && nsteps > 1;
+  bool shared = false;   // a Hamiltonian in two launch groups
+  for (size_t a = 0; a < groups.size() && !shared; ++a)
+    for (size_t b = 0; b < a && !shared; ++b)
+      for (size_t x = groups[a].g; x < groups[a].e && !shared; ++x)
+        for (size_t y = groups[b].g; y < groups[b].e && !shared; ++y) shared = hams[idx[x]] == hams[idx[y]];
+  if (split && alternate) {
+    split = groups.size() >= 2 && !shared;
+    for (size_t gi = 0; gi < groups.size(); ++gi) groups[gi].side = split && gi % 2 == 1;
+    if (split) SPX_TRY(ensure_scratch(p, 2 * kM));
+  }
+  bool chain = !autonormalize && nsteps > 1 && !shared;
   for (size_t i : idx) chain = chain && hams[i]->wht_ok && coeffs[i].size() > 2;
-  for (size_t a = 0; a < groups.size() && chain; ++a)
-    for (size_t b = 0; b < a && chain; ++b)
-      for (size_t x = groups[a].g; x < groups[a].e && chain; ++x)
-        for (size_t y = groups[b].g; y < groups[b].e && chain; ++y) chain = hams[idx[x]] != hams[idx[y]];
   if (split) {
     SPX_TRY(ensure_side(p->ctx));
     SPX_HIP(hipEventRecord(p->ctx->fork, p->ctx->stream));
@@ -805,8 +821,10 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
         cs[k - g] = &coeffs[idx[k]];
       }
       OnStream on(p->ctx, grp.side ? p->ctx->side : p->ctx->stream);
-      rc = one_step_set(p, hs, mode, ss, e - g, cs, chain && step > 0, chain && step + 1 < nsteps,
-                        split ? g : 0);
+      // scratch pairs: by position within a split set, by stream when
+      // groups alternate
+      const size_t slot0 = !split ? 0 : alternate ? (grp.side ? kM : 0) : g;
+      rc = one_step_set(p, hs, mode, ss, e - g, cs, chain && step > 0, chain && step + 1 < nsteps, slot0);
     }
     if (autonormalize)
       for (size_t i = 0; i < count && rc == SPX_OK; ++i) rc = spx_state_normalize(states[i], nullptr);

@@ -133,7 +133,9 @@ struct spx_prop {
   size_t nspins = 0;
   double epsilon = 1e-7;
   size_t last_order = 0;
-  double2* d_scratch[2 * spx::kMaxMany] = {};
+  // T_{n-1}, T_{n-2} pairs: slots [0, kMaxMany) for the launches on the
+  // context's stream, [kMaxMany, 2 kMaxMany) for a second stream's groups
+  double2* d_scratch[4 * spx::kMaxMany] = {};
   int graphs = 0;   // record one step as a HIP graph and replay it
   int streams = 2;  // 2: a set evolve may run as two halves on two streams
 };

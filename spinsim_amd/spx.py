@@ -433,9 +433,10 @@ class Propagator:
         check(lib().spx_prop_set_graphs(self.h, int(enable)))
 
     def set_streams(self, streams: int) -> None:
-        """2 (default): a set of 4..8 states on the Hadamard path that splits
-        into two parts with no Hamiltonian in both runs as two overlapping
-        halves on two streams; 1: one stream."""
+        """2 (default): a set on the Hadamard path runs on two streams whose
+        launches overlap (4..8 states as two halves with no Hamiltonian in
+        both; more than 8 as launch groups alternating between the streams);
+        1: one stream."""
         check(lib().spx_prop_set_streams(self.h, int(streams)))
 
     def evolve_many(self, ham: Hamiltonian, mode: int, states: Sequence[StateVector], tau: float,

@@ -102,13 +102,15 @@ def test_l18_hadamard_set_of_four(ctx):
     check_set(ctx, L, [(i, 1.0, False) for i in range(4)], 2, nsteps=2)
 
 
-@pytest.mark.parametrize("count", [4, 7, 8])
+@pytest.mark.parametrize("count", [4, 7, 8, 12, 18])
 def test_two_stream_halves_bitwise_equal_to_one_stream(ctx, count):
     """A set of 4..8 states under distinct Hamiltonians on the Hadamard path
     runs as two halves on two streams (the default); with one stream it runs
-    as one launch group. The halves must give the same bits, also over
-    chained steps, and the context's stream must see their results (the
-    download below is ordered on it)."""
+    as one launch group. Larger sets alternate their launch groups between
+    the streams (12: two groups of 6; 18: groups of 8, 8 and 2). The streams
+    must give the same bits as one stream, also over chained steps, and the
+    context's stream must see their results (the download below is ordered
+    on it)."""
     L = 16
     banner(f"evolve_set L={L} dense=2, {count} states, two streams vs one")
     specs = [(i, 1.0 + 0.5 * (i % 3), i == 2) for i in range(count)]
