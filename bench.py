@@ -404,6 +404,22 @@ def main():
     kernel_name = hams[0].kernel_name()
     achieved = kernel_bytes / (per_order_ms * 1e-3) / 1e9
     log(f"[bench] general order over {S} states: {per_order_ms * 1e3:.2f} us -> {achieved:.1f} GB/s")
+    # the same difference with every launch on one stream: the figure that a
+    # kernel trace's per-dispatch durations add up to (overlapping dispatches
+    # of two streams do not)
+    one_stream_us = None
+    if W < S:
+        prop.set_streams(1)
+        prop.steps_set(hams, spx.REAL, states, coeffs, effs, 2)   # untimed: launch width 8
+        ctx.record(2)
+        prop.steps_set(hams, spx.REAL, states, [c[: m + 1] for c in coeffs], effs, reps)
+        ctx.record(3)
+        prop.steps_set(hams, spx.REAL, states, [c[: sh + 1] for c in coeffs], effs, reps)
+        ctx.record(4)
+        ctx.synchronize()
+        prop.set_streams(args.streams)
+        one_stream_us = (ctx.elapsed_ms(2, 3) - ctx.elapsed_ms(3, 4)) / (reps * (m - sh)) * 1e3
+        log(f"[bench] the same order with one stream: {one_stream_us:.2f} us")
     spx.autotune_mode(0)
     # the achievable streaming rate beside the datasheet peak: the in-place
     # scale kernel over 2^24 amplitudes (256 MiB each way, past every cache)
@@ -448,7 +464,8 @@ def main():
                                               "one launch sequence each, on two streams)" if W < S else
                                               f" (general Chebyshev order, {S} states per launch)"),
                      "bytes_per_launch": kernel_bytes, "us_per_launch": round(per_order_ms * 1e3, 3),
-                     "copy_GBs": copy["GB/s"], "frac_of_copy": round(achieved / copy["GB/s"], 4)},
+                     "copy_GBs": copy["GB/s"], "frac_of_copy": round(achieved / copy["GB/s"], 4),
+                     "us_per_launch_one_stream": round(one_stream_us, 3) if one_stream_us else None},
     }
     if single:
         line["single_state"] = single
