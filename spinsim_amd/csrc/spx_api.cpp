@@ -741,16 +741,35 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
     for (size_t h = 2; h + 2 <= n; ++h)
       if (separates(h) && (half == 0 || std::max(h, n - h) < std::max(half, n - half))) half = h;
   const bool alternate = two && n > kM;   // groups alternate between the streams
-  const int plain_width = static_cast<int>(n < kM ? n : kM);
-  int width = plain_width;
-  if (half) width = static_cast<int>(std::max(half, n - half));
-  if (alternate && n <= 2 * kM) width = static_cast<int>((n + 1) / 2);
+  // The path is chosen at the one-stream width first, so that a schedule
+  // the tuner picks there (it sticks to the Hamiltonian) is the one-stream
+  // choice; the narrower launches of a split follow only if every state
+  // stays on the Hadamard path at their width.
+  int width = static_cast<int>(n < kM ? n : kM);
   SPX_TRY(prepare_set(hams, idx, bounds, width));
   bool split = half != 0 || alternate;
   for (size_t i : idx) split = split && hams[i]->wht_ok;
-  if ((half != 0 || alternate) && !split && width != plain_width) {
-    width = plain_width;
-    SPX_TRY(prepare_set(hams, idx, bounds, width));
+  if (split) {
+    const int narrow = static_cast<int>(half ? std::max(half, n - half) : n <= 2 * kM ? (n + 1) / 2 : kM);
+    // without a tuned schedule, a width at which the flip-flop default turns
+    // to the gathers would only rebuild every Hamiltonian's tables twice per
+    // call and fall back: such sets keep one stream
+    for (size_t i : idx)
+      split = split && (hams[i]->sched_set || default_dense_path(hams[i]->nspins, narrow, true) == 2);
+  }
+  if (split) {
+    const int narrow = static_cast<int>(half ? std::max(half, n - half) : n <= 2 * kM ? (n + 1) / 2 : kM);
+    if (narrow != width) {
+      SPX_TRY(prepare_set(hams, idx, bounds, narrow));
+      bool stays = true;
+      for (size_t i : idx) stays = stays && hams[i]->wht_ok;
+      if (stays) {
+        width = narrow;
+      } else {
+        split = false;
+        SPX_TRY(prepare_set(hams, idx, bounds, width));
+      }
+    }
   }
   for (size_t i : idx)
     if (hams[i]->wht_ok) {
