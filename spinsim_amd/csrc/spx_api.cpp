@@ -749,16 +749,15 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
   SPX_TRY(prepare_set(hams, idx, bounds, width));
   bool split = half != 0 || alternate;
   for (size_t i : idx) split = split && hams[i]->wht_ok;
+  // launch width of the split: the larger half, or half the set (at most
+  // kMaxMany) when groups alternate
+  const int narrow = static_cast<int>(half ? std::max(half, n - half) : n <= 2 * kM ? (n + 1) / 2 : kM);
+  // without a tuned schedule, a width at which the flip-flop default turns
+  // to the gathers would only rebuild every Hamiltonian's tables twice per
+  // call and fall back: such sets keep one stream
+  for (size_t i : idx)
+    split = split && (hams[i]->sched_set || default_dense_path(hams[i]->nspins, narrow, true) == 2);
   if (split) {
-    const int narrow = static_cast<int>(half ? std::max(half, n - half) : n <= 2 * kM ? (n + 1) / 2 : kM);
-    // without a tuned schedule, a width at which the flip-flop default turns
-    // to the gathers would only rebuild every Hamiltonian's tables twice per
-    // call and fall back: such sets keep one stream
-    for (size_t i : idx)
-      split = split && (hams[i]->sched_set || default_dense_path(hams[i]->nspins, narrow, true) == 2);
-  }
-  if (split) {
-    const int narrow = static_cast<int>(half ? std::max(half, n - half) : n <= 2 * kM ? (n + 1) / 2 : kM);
     if (narrow != width) {
       SPX_TRY(prepare_set(hams, idx, bounds, narrow));
       bool stays = true;
