@@ -32,22 +32,40 @@ def _run(cmd):
     return r
 
 
-def build_spx(verbose: bool = False) -> str:
+def _stale(obj: str, src: str) -> bool:
+    """True when `obj` is older than its source or any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(d, f) for d in (CSRC, INCLUDE) for f in os.listdir(d)
+                    if f.endswith((".h", ".hpp"))]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build_spx(verbose: bool = False, changed_only: bool = False) -> str:
+    """Compile every source (in parallel, MAX_JOBS at a time) and link.
+    changed_only: recompile only objects older than their inputs."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(LIBDIR, "obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(objdir, src.replace(".", "_") + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmd = [HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", *lang, path, "-o", obj]
+        objs.append(obj)
+        if changed_only and not _stale(obj, path):
+            continue
+        cmds.append([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", *lang, path, "-o", obj])
+    jobs = max(1, min(len(cmds) or 1, int(os.environ.get("MAX_JOBS", "8"))))
+    for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)
-        _run(cmd)
-        objs.append(obj)
+    with ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(_run, cmds))
     out = os.path.join(LIBDIR, "libspx.so")
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
@@ -56,7 +74,7 @@ def build_spx(verbose: bool = False) -> str:
 
 
 def main():
-    print(build_spx(verbose=True))
+    print(build_spx(verbose=True, changed_only="--changed" in sys.argv))
 
 
 if __name__ == "__main__":

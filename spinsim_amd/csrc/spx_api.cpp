@@ -463,9 +463,14 @@ int evolve_impl(spx_prop* p, spx_ham* h, int mode, spx_state* const* states, siz
                 const std::vector<double>* prebuilt, double prebuilt_bound, double tau,
                 size_t nsteps, int autonormalize) {
   if (!p || !h || !states) return fail(SPX_EINVAL, "null argument");
+  // the order launches go to the Hamiltonian's stream and the first-order
+  // update to the propagator's: both, and every state, must share one
+  // context or the launches of one step would run on unordered streams
+  if (h->ctx != p->ctx) return fail(SPX_EINVAL, "Hamiltonian belongs to another context");
   if (nstates == 0) return SPX_OK;
   for (size_t i = 0; i < nstates; ++i) {
     SPX_TRY(check_state(states[i]));
+    if (states[i]->ctx != p->ctx) return fail(SPX_EINVAL, "state belongs to another context");
     if (states[i]->nspins != h->nspins)
       return fail(SPX_EINVAL, "state and Hamiltonian disagree on spin count");
     if (states[i]->nspins != p->nspins)

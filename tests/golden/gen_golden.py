@@ -75,7 +75,10 @@ def main():
             R.ref_ham_apply(mode, L, O._d(np.ascontiguousarray(fields)), lo.size, O._u32(lo),
                             O._u32(hi), O._d(jx), O._d(jy), O._d(jz), bound, O._d(psi), O._d(o))
             out[f"{key}_out{mode}"] = o
-        # a few Chebyshev steps through the reference's own state operations
+        # a few Chebyshev steps through the reference's own state operations.
+        # The expansion is built by the oracle (the reference's Chebyshev.cpp
+        # needs <format>, absent here) and recorded as the steps' INPUT, not
+        # as reference output; the step results are the reference's.
         for mode, tau in ((0, 0.05), (0, -0.3), (1, 0.1)):
             rc, coeffs, eff = O.cheb_build(mode, bound, tau, 1e-10)
             p = psi.copy()
@@ -83,7 +86,7 @@ def main():
                            O._u32(hi), O._d(jx), O._d(jy), O._d(jz), O._d(coeffs), coeffs.size - 1,
                            eff, 3, O._d(p))
             out[f"{key}_step_m{mode}_t{tau}"] = p
-            out[f"{key}_coeffs_m{mode}_t{tau}"] = coeffs
+            out[f"{key}_stepinput_oracle_coeffs_m{mode}_t{tau}"] = coeffs
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path} ({len(out)} arrays)")

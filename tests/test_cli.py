@@ -235,3 +235,65 @@ def test_coupling_tables_match_the_dense_reference(tmp_path):
         assert np.allclose(means[k], want, rtol=0, atol=1e-9), (k, means[k], want)
     # the magnetisations move: the couplings were applied
     assert np.ptp(means[:, 0]) > 0.1
+
+
+# The reference's demo configuration (configs/cpmg8z.toml), written out here
+# key for key: L = 10, random initial state, dipolar geometry, field_disorder
+# 100, a measure at t = 0, then 20 x [0.05 free, 1.07 pi pulse about x of
+# hx = 10700, 0.05 free, measure].
+CPMG8Z = """
+[system]
+spins = 10
+initial_state = "random"
+[field]
+hx = 0.0
+hy = 0.0
+hz = 0.0
+[ensemble]
+realizations = 180
+seed = 20240726
+dipolar_geometry = true
+dipolar_scale = 1.0
+field_disorder = 100.0
+[measurement]
+reference_axis = "x"
+observables = ["corr:x", "norm"]
+[numerics]
+epsilon = 1e-7
+[[step]]
+kind = "measure"
+[[step]]
+kind = "repeat"
+count = 20
+body = [
+  { kind = "evolve", tau = 0.05, steps = 1 },
+  { kind = "evolve", tau = 0.000314159265, steps = 1, hx = 10700.0 },
+  { kind = "evolve", tau = 0.05, steps = 1 },
+  { kind = "measure" },
+]
+"""
+
+
+@pytest.mark.gpu
+def test_demo_run_reproduces_the_documented_csv_row(tmp_path):
+    """`spinsim run cpmg8z.toml --realizations 20` on the device: the t = 0
+    row is, character for character, the one the reference documents
+    (docs/configuration.md:229-231), and every later row agrees with the
+    oracle's ensemble to the CSV's digits (1e-10 between device and oracle)."""
+    from test_oracle import CPMG8Z_T0_ROW
+    cfg = write(tmp_path, "cpmg8z.toml", CPMG8Z)
+    r = spinsim("run", cfg, "--realizations", "20", "--threads", "4")
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[0].startswith("time,corr_x_re_mean,corr_x_re_sd,corr_x_re_stderr,")
+    assert lines[1].startswith(CPMG8Z_T0_ROW + ",")
+    header, rows = read_csv(r.stdout)
+    p = O.Program().measure().repeat(20).evolve(0.05, 1)
+    p.evolve(0.000314159265, 1, fields=np.tile([10700.0, 0.0, 0.0], (10, 1))).evolve(0.05, 1).measure().end()
+    cfg_o = O.Config(L=10, program=p, seed=20240726, field_disorder=100.0,
+                     observables=((O.OBS_CORR_X, 0), (O.OBS_NORM, 0)), reference_axis=0, epsilon=1e-7)
+    times, _, mean, sd, se = O.run_ensemble(cfg_o, 20, threads=8)
+    assert rows.shape == (21, 1 + 3 * mean.shape[1])
+    assert np.allclose(rows[:, 0], times, rtol=1e-12, atol=1e-15)
+    for k, want in enumerate((mean, sd, se)):
+        assert np.allclose(rows[:, 1 + k::3], want, rtol=1e-9, atol=1e-10), k

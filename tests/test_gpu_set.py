@@ -176,6 +176,23 @@ def test_same_hamiltonian_twice_and_argument_errors(ctx):
     foreign = spx.StateVector(ctx2, L, O.random_state(L, 4))
     with pytest.raises(spx.InvalidArgument, match="another context"):
         prop.evolve_set([h], spx.REAL, [foreign], 0.05, 1)
+    # single-state evolves and prebuilt steps check the same, for the state
+    # and for the Hamiltonian
+    with pytest.raises(spx.InvalidArgument, match="another context"):
+        prop.evolve(h, spx.REAL, foreign, 0.05, 1)
+    coeffs, eff = spx.cheb_build(spx.REAL, h.spectral_bound(), 0.05, 1e-12)
+    with pytest.raises(spx.InvalidArgument, match="another context"):
+        prop.steps(h, spx.REAL, foreign, coeffs, eff, 1)
+    h2 = device_ham(ctx2, L, fields, c, 2)
+    with pytest.raises(spx.InvalidArgument, match="another context"):
+        prop.evolve(h2, spx.REAL, a, 0.05, 1)
+    with pytest.raises(spx.InvalidArgument, match="another context"):
+        prop.evolve_many(h2, spx.REAL, [a, b], 0.05, 1)
+    # the rejected calls left the states alone
+    for p, s in ((p0, a), (p1, b)):
+        one = spx.StateVector(ctx, L, p)
+        prop.evolve(h, spx.REAL, one, 0.05, 2)
+        assert np.array_equal(one.download(), s.download())
 
 
 def test_steps_set_over_prebuilt_expansions_matches_evolve_set(ctx):
