@@ -41,6 +41,12 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {("k_dense<K_ORDERN, 18, 10>", 1, 1): 42417 * 1024,
                     ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8, 8): 384722 * 1024,
                     ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8, 4): 385260 * 1024}
+# where each figure can be reproduced: the kernel-trace summary whose
+# per-dispatch means add up to `us_per_launch`, and the PMC runs behind
+# `traffic` (scaled FETCH_SIZE + WRITE_SIZE, not measured by this run)
+TRACE_PROFILE = {"k_dense<K_ORDERN, 18, 10>": "profiles/k_dense_L18_order.md",
+                 "k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>": "profiles/k_wht_L18_8states_order.md"}
+TRAFFIC_PROFILE = TRACE_PROFILE
 BYTES_PER_AMP_ORDER = 88          # read T_{n-1}, T_{n-2}, diag; write T_n; read+write psi
 BYTES_FIRST, BYTES_SECOND = 40, 72  # order 1 (psi -> T_1), order 2 (fused psi update)
 
@@ -249,7 +255,7 @@ def batched_leg(spx, ctx, count=4096, seed=20240726):
     if not np.isfinite(v).all():
         raise SystemExit("non-finite batched ensemble output")
     return {"workload": f"{count} realisations, L uniform in 8..12, XY-8 (finite pulses, Omega=50, tau=0.5) x 16, "
-                        "corr:x + norm per block, one persistent launch",
+                        "corr:x + norm per block, one persistent launch per L on concurrent streams",
             "realisations": count, "s": round(dt, 4), "realisations_per_s": round(count / dt, 1),
             "mean_L": round(float(np.mean(Ls)), 2)}
 
@@ -361,11 +367,13 @@ def main():
     W = (S + 1) // 2 if args.streams == 2 and S >= 4 else S
     tune_report = None
     if not args.no_tune:
-        # the schedule for launches of W states; the other realisations adopt
-        # it from the tuner's cache (same shape and width)
-        tune_report = hams[0].autotune(effs[0], nvec=W)
+        # a set evolve settles each Hamiltonian's path at the set's
+        # one-stream width S before it narrows to halves of W, so the
+        # schedule is tuned at width S; the other realisations adopt it from
+        # the tuner's cache (same shape and width)
+        tune_report = hams[0].autotune(effs[0], nvec=S)
         spx.autotune_mode(1)
-        log(f"[bench] autotune (width {W}): {tune_report[:300]}")
+        log(f"[bench] autotune (width {S}): {tune_report[:300]}")
     prop = spx.Propagator(ctx, L, args.epsilon)
     prop.set_graphs(args.graphs)
     prop.set_streams(args.streams)
@@ -373,7 +381,8 @@ def main():
 
     prop.steps_set(hams, spx.REAL, states, coeffs, effs, args.warmup)
     ctx.synchronize()
-    log(f"[bench] warmup {args.warmup} steps done")
+    kernels = [h.kernel_name() for h in hams]
+    log(f"[bench] warmup {args.warmup} steps done; kernels {kernels}")
 
     ctx.record(0)
     prop.steps_set(hams, spx.REAL, states, coeffs, effs, args.steps)
@@ -392,25 +401,12 @@ def main():
     reps = max(20, args.steps // 4)
     m = min(orders)
     sh = max(3, m // 2)
-    ctx.record(2)
-    prop.steps_set(hams, spx.REAL, states, [c[: m + 1] for c in coeffs], effs, reps)
-    ctx.record(3)
-    prop.steps_set(hams, spx.REAL, states, [c[: sh + 1] for c in coeffs], effs, reps)
-    ctx.record(4)
-    ctx.synchronize()
-    t_long, t_short = ctx.elapsed_ms(2, 3), ctx.elapsed_ms(3, 4)
-    per_order_ms = (t_long - t_short) / (reps * (m - sh))
-    kernel_bytes = BYTES_PER_AMP_ORDER * n * S
-    kernel_name = hams[0].kernel_name()
-    achieved = kernel_bytes / (per_order_ms * 1e-3) / 1e9
-    log(f"[bench] general order over {S} states: {per_order_ms * 1e3:.2f} us -> {achieved:.1f} GB/s")
-    # the same difference with every launch on one stream: the figure that a
-    # kernel trace's per-dispatch durations add up to (overlapping dispatches
-    # of two streams do not)
-    one_stream_us = None
-    if W < S:
-        prop.set_streams(1)
-        prop.steps_set(hams, spx.REAL, states, coeffs, effs, 2)   # untimed: launch width 8
+
+    def order_us(streams):
+        """Time of one general order over the set, by difference of two runs
+        cut to m and sh < m orders, with the set's launches on `streams`."""
+        prop.set_streams(streams)
+        prop.steps_set(hams, spx.REAL, states, coeffs, effs, 2)   # untimed: settles the launch width
         ctx.record(2)
         prop.steps_set(hams, spx.REAL, states, [c[: m + 1] for c in coeffs], effs, reps)
         ctx.record(3)
@@ -418,8 +414,26 @@ def main():
         ctx.record(4)
         ctx.synchronize()
         prop.set_streams(args.streams)
-        one_stream_us = (ctx.elapsed_ms(2, 3) - ctx.elapsed_ms(3, 4)) / (reps * (m - sh)) * 1e3
-        log(f"[bench] the same order with one stream: {one_stream_us:.2f} us")
+        return (ctx.elapsed_ms(2, 3) - ctx.elapsed_ms(3, 4)) / (reps * (m - sh)) * 1e3
+
+    kernel_bytes = BYTES_PER_AMP_ORDER * n * S
+    kernel_name = hams[0].kernel_name()
+    # roofline: every launch of the order on one stream, the figure that a
+    # kernel trace's per-dispatch durations add up to (the committed profile
+    # reproduces it); with two streams the halves' dispatches overlap, and
+    # that live figure is reported beside it
+    per_order_us = order_us(1)
+    achieved = kernel_bytes / (per_order_us * 1e-6) / 1e9
+    log(f"[bench] general order over {S} states, one stream: {per_order_us:.2f} us -> {achieved:.1f} GB/s")
+    two = None
+    if W < S:
+        two_us = order_us(2)
+        two_gbs = kernel_bytes / (two_us * 1e-6) / 1e9
+        two = {"us_per_launch": round(two_us, 3), "achieved": round(two_gbs, 2),
+               "frac": round(two_gbs / HBM_PEAK_GBS, 4),
+               "arrangement": f"two halves of {W} states, one launch sequence each, on two streams "
+                              "(the timed steps' arrangement); events on the propagator's stream"}
+        log(f"[bench] the same order as two halves on two streams: {two_us:.2f} us -> {two_gbs:.1f} GB/s")
     spx.autotune_mode(0)
     # the achievable streaming rate beside the datasheet peak: the in-place
     # scale kernel over 2^24 amplitudes (256 MiB each way, past every cache)
@@ -455,17 +469,18 @@ def main():
                                    else "one launch per order for the set"),
                    "nspins": L, "pairs": int(lo.size), "realisations": S, "orders": [int(o) for o in orders],
                    "spectral_bounds": [round(h.spectral_bound(), 4) for h in hams],
-                   "schedule": tune_report.split(";")[0] if tune_report else "default"},
+                   "schedule": tune_report.split(";")[0] if tune_report else "default",
+                   "kernels": sorted(set(kernels))},
         "hpsi_per_s": round(args.steps * sum(orders) / (ms * 1e-3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((kernel_name, S, W)),
-                     "kernel": kernel_name + (f" (general Chebyshev order over {S} states: two halves of {W}, "
-                                              "one launch sequence each, on two streams)" if W < S else
-                                              f" (general Chebyshev order, {S} states per launch)"),
-                     "bytes_per_launch": kernel_bytes, "us_per_launch": round(per_order_ms * 1e3, 3),
+                     "traffic": MEASURED_TRAFFIC.get((kernel_name, S, S)),
+                     "traffic_from": TRAFFIC_PROFILE.get(kernel_name),
+                     "kernel": kernel_name + f" (general Chebyshev order, {S} states per launch, one stream)",
+                     "bytes_per_launch": kernel_bytes, "us_per_launch": round(per_order_us, 3),
+                     "profile": TRACE_PROFILE.get(kernel_name),
                      "copy_GBs": copy["GB/s"], "frac_of_copy": round(achieved / copy["GB/s"], 4),
-                     "us_per_launch_one_stream": round(one_stream_us, 3) if one_stream_us else None},
+                     "two_stream": two},
     }
     if single:
         line["single_state"] = single

@@ -284,7 +284,9 @@ int spx_ham_destroy(spx_ham* h) {
 
 int spx_ham_set_field(spx_ham* h, size_t spin, double x, double y, double z) {
   if (spin >= h->nspins) return fail(SPX_ERANGE, "spin index out of range");
-  if (h->hx[spin] != x || h->hy[spin] != y) h->terms_valid = false;
+  // the Hadamard path's factor tables hold hz, the other paths' tables the
+  // transverse fields
+  if (h->hx[spin] != x || h->hy[spin] != y || h->hz[spin] != z) h->terms_valid = false;
   h->hx[spin] = x;
   h->hy[spin] = y;
   h->hz[spin] = z;
@@ -356,7 +358,8 @@ int spx_ham_apply(spx_ham* h, int kind, double bound, const spx_state* in, spx_s
 }
 
 int spx_ham_diagonal(spx_ham* h, double* host) {
-  SPX_TRY(ham_prepare(h, h->terms_valid ? h->terms_scale : 1.0));
+  SPX_TRY(set_device(h->ctx));
+  SPX_TRY(ensure_diag(h));
   const size_t n = static_cast<size_t>(1) << h->nspins;
   SPX_HIP(hipMemcpyAsync(host, h->d_diag, n * sizeof(double), hipMemcpyDeviceToHost, h->ctx->stream));
   SPX_HIP(hipStreamSynchronize(h->ctx->stream));
@@ -761,7 +764,8 @@ int evolve_set_impl(spx_prop* p, spx_ham* const* hams, int mode, spx_state* cons
   // to the gathers would only rebuild every Hamiltonian's tables twice per
   // call and fall back: such sets keep one stream
   for (size_t i : idx)
-    split = split && (hams[i]->sched_set || default_dense_path(hams[i]->nspins, narrow, true) == 2);
+    split = split && (hams[i]->sched_set ||
+                      default_dense_path(hams[i]->nspins, narrow, ham_flipflop_only(hams[i])) == 2);
   if (split) {
     if (narrow != width) {
       SPX_TRY(prepare_set(hams, idx, bounds, narrow));

@@ -247,6 +247,8 @@ int dense_tile_bits(const spx_ham* h);
 // evolve's width and invalidates the prepared tables when that changes the
 // choice.
 int default_dense_path(size_t L, int width, bool flipflop_only);
+bool ham_flipflop_only(const spx_ham* h);
+int ensure_diag(spx_ham* h);
 void set_launch_width(spx_ham* h, int width);
 // the autotuner's process-wide mode and its hook in the propagator
 int autotune_before_evolve(spx_ham* h, double bound, int nvec);

@@ -1016,9 +1016,22 @@ int default_dense_path(size_t L, int width, bool flipflop_only) {
   return 1;
 }
 
+// Whether the Hamiltonian has flip-flop couplings only (no transverse field,
+// jx = jy on every pair): the case in which the default path depends on the
+// launch width. A heuristic on the raw values; the path itself is settled in
+// prepare_dense after the skip rule.
+bool ham_flipflop_only(const spx_ham* h) {
+  for (size_t s = 0; s < h->nspins; ++s)
+    if (h->hx[s] != 0.0 || h->hy[s] != 0.0) return false;
+  for (size_t p = 0; p < h->lo.size(); ++p)
+    if (h->jx[p] != h->jy[p]) return false;
+  return true;
+}
+
 void set_launch_width(spx_ham* h, int width) {
   if (h->width == width) return;
-  const bool changes = default_dense_path(h->nspins, h->width, true) != default_dense_path(h->nspins, width, true);
+  const bool ff = ham_flipflop_only(h);
+  const bool changes = default_dense_path(h->nspins, h->width, ff) != default_dense_path(h->nspins, width, ff);
   h->width = width;
   if (changes) h->terms_valid = false;
 }
@@ -1079,7 +1092,11 @@ static int prepare_dense(spx_ham* h, double scale) {
   return SPX_OK;
 }
 
-int ham_prepare(spx_ham* h, double scale) {
+// The cached sigma_z diagonal (rebuildDiagonal, src/core/Hamiltonian.cpp:54-80),
+// rebuilt when hz or the couplings changed since it was built. Only the
+// gather and table kernels read it (the Hadamard path forms the diagonal
+// from small factor tables), so it is built on demand.
+int ensure_diag(spx_ham* h) {
   spx_ctx* c = h->ctx;
   const size_t L = h->nspins, n = static_cast<size_t>(1) << L;
   if (!h->diag_valid || h->diag_hz != h->hz || h->diag_jz != h->jz || h->diag_lo != h->lo ||
@@ -1118,8 +1135,24 @@ int ham_prepare(spx_ham* h, double scale) {
     h->diag_jz = h->jz;
     h->diag_lo = h->lo;
     h->diag_hi = h->hi;
-    h->terms_valid = false;
   }
+  return SPX_OK;
+}
+
+static int prepare_terms(spx_ham* h, double scale);
+
+int ham_prepare(spx_ham* h, double scale) {
+  SPX_TRY(prepare_terms(h, scale));
+  if (!h->wht_ok) SPX_TRY(ensure_diag(h));
+  return SPX_OK;
+}
+
+// The launch tables of the order kernels at `scale` (field, pair and
+// diagonal weights after the skip rule), rebuilt when any field or coupling
+// or the scale changed.
+static int prepare_terms(spx_ham* h, double scale) {
+  spx_ctx* c = h->ctx;
+  const size_t L = h->nspins;
   if (h->terms_valid && h->terms_scale == scale) return SPX_OK;
   SPX_TRY(prepare_dense(h, scale));
 

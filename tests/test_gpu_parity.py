@@ -428,6 +428,47 @@ def test_imaginary_time_relaxes_to_the_ground_state(ctx):
     assert overlap > 1.0 - 1e-9
 
 
+def test_benchmarked_arrangement_matches_the_oracle(ctx):
+    """The headline workload exactly as bench.py runs it: realisations
+    (20240726, 0..7) at L = 18 from bench.synthetic_system, expansions built
+    at dt = 0.05 and eps = 1e-12, the schedule the tuner picks at the set's
+    one-stream width and adopted through the tuner's cache, two halves of
+    four on two streams, three chained steps through steps_set. Every
+    realisation agrees with the oracle's three steps amplitude by amplitude
+    to 1e-12 of its largest amplitude, and the kernels are the Hadamard
+    path's."""
+    import bench
+    L = 18
+    banner("benchmarked arrangement: oracle three steps of eight L=18 realisations (CPU, about 15 s)")
+    systems = [bench.synthetic_system(L, index=i) for i in range(8)]
+    hams, states, wants = [], [], []
+    for f, lo, hi, jx, jy, jz, p0 in systems:
+        x = p0.copy()
+        rc, _ = O.Ham(L, f, O.Couplings(lo, hi, jx, jy, jz)).evolve(x, 0.05, 3, eps=1e-12)
+        assert rc == 0
+        wants.append(x)
+        h = spx.Hamiltonian(ctx, L)
+        h.set_fields(f)
+        h.set_couplings(lo, hi, jx, jy, jz)
+        hams.append(h)
+        states.append(spx.StateVector(ctx, L, p0))
+    built = [spx.cheb_build(spx.REAL, h.spectral_bound(), 0.05, 1e-12) for h in hams]
+    banner("benchmarked arrangement: tune, then three steps on the device")
+    hams[0].autotune(built[0][1], nvec=8)
+    spx.autotune_mode(1)
+    try:
+        prop = spx.Propagator(ctx, L, 1e-12)
+        prop.set_streams(2)
+        prop.steps_set(hams, spx.REAL, states, [c for c, _ in built], [e for _, e in built], 3)
+    finally:
+        spx.autotune_mode(0)
+    for i, (s, w) in enumerate(zip(states, wants)):
+        err = rel_err(s.download(), w)
+        banner(f"realisation {i}: {hams[i].kernel_name()} rel err {err:.2e}")
+        assert err < 1e-12, (i, err)
+        assert hams[i].kernel_name().startswith("k_wht_"), hams[i].kernel_name()
+
+
 def test_l18_drift_over_1e4_steps_is_the_expansions_own(ctx):
     """The single-large-state drift check at full length: eight L = 18
     realisations (indices 0..7, two halves of four on two streams), 1e4 real

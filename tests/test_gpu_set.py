@@ -239,3 +239,28 @@ def test_default_path_follows_size_and_launch_width(ctx):
     assert h.kernel_name().startswith("k_wht_"), h.kernel_name()
     prop.evolve(h, spx.REAL, s, 0.05, 1)
     assert h.kernel_name().startswith("k_dense"), h.kernel_name()
+
+
+@pytest.mark.parametrize("L,count,transverse", [(16, 8, False), (17, 5, False), (16, 8, True)])
+def test_two_stream_guard_without_a_schedule(ctx, L, count, transverse):
+    """Sets without a tuned schedule: flip-flop sets at L = 16 (8 states)
+    and L = 17 (5 states) take the Hadamard path at their one-stream width,
+    whose halves would fall to the gathers, so they keep one stream; a set
+    with transverse fields stays on the Hadamard path at every width and
+    splits. Either way every state is on the Hadamard path and bitwise equal
+    to the same set on one stream."""
+    banner(f"two-stream guard L={L} x{count} transverse={transverse}")
+    specs = [realisation(L, i, 1.0, transverse) for i in range(count)]
+    psis = [O.random_state(L, 700 + k) for k in range(count)]
+    out = []
+    for streams in (2, 1):
+        hams = [device_ham(ctx, L, f, c, -1) for f, c in specs]
+        prop = spx.Propagator(ctx, L, 1e-12)
+        prop.set_streams(streams)
+        states = [spx.StateVector(ctx, L, p) for p in psis]
+        prop.evolve_set(hams, spx.REAL, states, 0.05, 3)
+        names = [h.kernel_name() for h in hams]
+        assert all(n.startswith("k_wht_") for n in names), names
+        out.append([s.download() for s in states])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
