@@ -248,6 +248,14 @@ int spx_batch_destroy(spx_batch* b);
 int spx_batch_add(spx_batch* b, size_t nspins, const double* fields3L, size_t npairs,
                   const uint32_t* lo, const uint32_t* hi, const double* jx, const double* jy,
                   const double* jz, const double* psi0);
+/* Adds realisations first .. first+count-1 of the dipolar ensemble as
+ * runRealization builds them (src/io/Experiment.cpp:10-55) for a run with
+ * zero base fields, dipolar geometry, Gaussian Sz disorder and a random
+ * initial state: the lattice, the disorder and the typical state drawn from
+ * Rng::forRealization(seed, index) in that order (spx_dipolar_realisation,
+ * spx_dipolar_state), on a pool of host threads; slots follow index order. */
+int spx_batch_add_dipolar(spx_batch* b, uint64_t seed, size_t first, size_t count, size_t nspins,
+                          double scale, double min_separation, int angular, double disorder);
 /* Runs every added realisation; times [points], values [n][points][width]. */
 int spx_batch_run(spx_batch* b, double* times, double* values);
 int spx_batch_width(const spx_batch* b, size_t* width);
@@ -265,6 +273,13 @@ int spx_batch_clear(spx_batch* b);
 int spx_dipolar_realisation(uint64_t seed, size_t index, size_t nspins, double scale,
                             double min_separation, int angular, double disorder, uint32_t* lo,
                             uint32_t* hi, double* jx, double* jy, double* jz, double* hz);
+/* The typical initial state of realisation (seed, index): the draws of
+ * spx_dipolar_realisation, then prepareInitialState's independent Gaussian
+ * real and imaginary parts per amplitude on the same stream, normalised
+ * (src/ensemble/Ensemble.cpp:20-45, src/io/Experiment.cpp:17-30);
+ * psi: 2*2^L doubles. */
+int spx_dipolar_state(uint64_t seed, size_t index, size_t nspins, double scale, double min_separation,
+                      int angular, double disorder, double* psi);
 /* Realisations first .. first+count-1 as dense symmetric J [count][L][L]
  * (the secular jz of each pair) and fields h [count][L], drawn on a pool
  * of host threads; the result does not depend on the pool size. */

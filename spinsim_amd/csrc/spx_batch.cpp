@@ -2,6 +2,7 @@
 // (src/ensemble/EnsembleRunner.cpp:27-88) with each realisation's program
 // walked as Sequence::run does (src/sequence/Sequence.cpp:35-106).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "spx_batch.hpp"
+#include "spx_ensemble.hpp"
 
 using namespace spx;
 
@@ -622,6 +624,62 @@ int spx_batch_add(spx_batch* b, size_t nspins, const double* fields3L, size_t np
   r.jz.assign(jz, jz + npairs);
   r.psi0.assign(psi0, psi0 + (static_cast<size_t>(2) << nspins));
   b->items.push_back(std::move(r));
+  return SPX_OK;
+}
+
+int spx_batch_add_dipolar(spx_batch* b, uint64_t seed, size_t first, size_t count, size_t nspins, double scale,
+                          double min_separation, int angular, double disorder) {
+  if (!b) return fail(SPX_EINVAL, "null batch");
+  const DipolarSpec spec{seed, nspins, scale, min_separation, angular != 0, disorder};
+  SPX_TRY(check_spec(spec));
+  for (size_t o = 0; o < b->obs_codes.size(); ++o)
+    if (b->obs_codes[o] >= 3 && b->obs_codes[o] <= 5 &&
+        (b->obs_spins[o] < 0 || static_cast<size_t>(b->obs_spins[o]) >= nspins))
+      return fail(SPX_ERANGE, "observable names a spin outside the system");
+  for (const ProgOp& op : b->program.ops) {
+    if (op.kind == OP_EVOLVE && !op.fields.empty() && op.fields.size() != 3 * nspins)
+      return fail(SPX_EINVAL, "step field table does not match the number of spins");
+    if (op.kind == OP_PULSE)
+      for (uint32_t t : op.targets)
+        if (t >= nspins) return fail(SPX_ERANGE, "rotation names a spin outside the system");
+  }
+  const size_t np = nspins * (nspins - 1) / 2;
+  std::vector<Realisation> drawn(count);
+  std::atomic<size_t> next{0};
+  std::atomic<int> status{SPX_OK};
+  auto work = [&] {
+    std::vector<double> hz(nspins);
+    for (;;) {
+      const size_t i = next.fetch_add(1);
+      if (i >= count || status.load() != SPX_OK) return;
+      Realisation& r = drawn[i];
+      r.nspins = nspins;
+      r.lo.resize(np);
+      r.hi.resize(np);
+      r.jz.resize(np);
+      r.psi0.resize(static_cast<size_t>(2) << nspins);
+      const int rc = draw_realisation(spec, first + i, r.lo.data(), r.hi.data(), r.jz.data(), hz.data(),
+                                      r.psi0.data());
+      if (rc != SPX_OK) {
+        status.store(rc);
+        return;
+      }
+      r.jx.resize(np);
+      r.jy.resize(np);
+      for (size_t p = 0; p < np; ++p) r.jx[p] = r.jy[p] = -0.5 * r.jz[p];
+      r.fields.assign(3 * nspins, 0.0);
+      for (size_t q = 0; q < nspins; ++q) r.fields[3 * q + 2] = hz[q];
+    }
+  };
+  const size_t threads = std::min<size_t>(
+      std::max<size_t>(1, count / 4), std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency())));
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < threads; ++t) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  if (status.load() == SPX_EDOMAIN) return fail(SPX_EDOMAIN, "cannot normalize a zero state");
+  if (status.load() != SPX_OK) return fail(status.load(), "could not place the spins at the minimum separation");
+  for (Realisation& r : drawn) b->items.push_back(std::move(r));
   return SPX_OK;
 }
 

@@ -17,6 +17,11 @@
 #include "spx_common.hpp"
 #include "spx_ensemble.hpp"
 
+// The reference builds with -ffp-contract=off (CMakeLists.txt:16-22); the
+// draws and the normalisation here must round as it does, so no multiply-add
+// is fused in this file.
+#pragma clang fp contract(off)
+
 namespace spx {
 namespace {
 
@@ -49,17 +54,25 @@ struct Stream {
     const double v = uniform();
     const double radius = std::sqrt(-2.0 * std::log(u));
     const double phase = 2.0 * std::numbers::pi * v;
-    spare = radius * std::sin(phase);
+    // one sincos call for both, as GCC compiles the reference's separate
+    // std::sin and std::cos of one argument; glibc's sincos and its sin / cos
+    // can differ in the last bit
+    double sn = 0.0, cs = 0.0;
+    ::sincos(phase, &sn, &cs);
+    spare = radius * sn;
     has_spare = true;
-    return radius * std::cos(phase);
+    return radius * cs;
   }
 };
 
 }  // namespace
 
-int draw_dipolar(const DipolarSpec& s, uint64_t index, uint32_t* lo, uint32_t* hi, double* jz, double* hz) {
+namespace {
+
+// generateDipolarLattice2D then applyFieldDisorder on one stream
+// (src/io/Experiment.cpp:17-27 draw order).
+int draw_lattice(const DipolarSpec& s, Stream& r, uint32_t* lo, uint32_t* hi, double* jz, double* hz) {
   const size_t L = s.nspins;
-  Stream r(s.seed, index);
   const double half = 0.5 * std::sqrt(static_cast<double>(L));
   double x[kMaxSpins] = {}, y[kMaxSpins] = {};
   size_t p = 0;
@@ -87,6 +100,33 @@ int draw_dipolar(const DipolarSpec& s, uint64_t index, uint32_t* lo, uint32_t* h
     }
   }
   for (size_t i = 0; i < L; ++i) hz[i] = s.disorder != 0.0 ? s.disorder * r.gaussian() : 0.0;
+  return SPX_OK;
+}
+
+}  // namespace
+
+int draw_dipolar(const DipolarSpec& s, uint64_t index, uint32_t* lo, uint32_t* hi, double* jz, double* hz) {
+  Stream r(s.seed, index);
+  return draw_lattice(s, r, lo, hi, jz, hz);
+}
+
+// The same draws, then prepareInitialState's typical state on the same
+// stream (src/ensemble/Ensemble.cpp:20-45): a Gaussian real then imaginary
+// part per amplitude, normalised as StateVector::normalize does (one serial
+// sum of squares over the interleaved values, then a scale by 1 / norm,
+// src/core/StateVector.cpp:43-56).
+int draw_realisation(const DipolarSpec& s, uint64_t index, uint32_t* lo, uint32_t* hi, double* jz, double* hz,
+                     double* psi) {
+  Stream r(s.seed, index);
+  SPX_TRY(draw_lattice(s, r, lo, hi, jz, hz));
+  const size_t n = static_cast<size_t>(2) << s.nspins;
+  for (size_t k = 0; k < n; ++k) psi[k] = r.gaussian();
+  double sum = 0.0;
+  for (size_t k = 0; k < n; ++k) sum += psi[k] * psi[k];
+  const double norm = std::sqrt(sum);
+  if (norm == 0.0) return SPX_EDOMAIN;
+  const double f = 1.0 / norm;
+  for (size_t k = 0; k < n; ++k) psi[k] *= f;
   return SPX_OK;
 }
 
@@ -160,6 +200,20 @@ int spx_dipolar_realisation(uint64_t seed, size_t index, size_t nspins, double s
   const int rc = draw_dipolar(s, index, lo, hi, jz, hz);
   if (rc != SPX_OK) return fail(rc, "could not place the spins at the minimum separation");
   for (size_t p = 0; p < nspins * (nspins - 1) / 2; ++p) jx[p] = jy[p] = -0.5 * jz[p];
+  return SPX_OK;
+}
+
+int spx_dipolar_state(uint64_t seed, size_t index, size_t nspins, double scale, double min_separation,
+                      int angular, double disorder, double* psi) {
+  const DipolarSpec s{seed, nspins, scale, min_separation, angular != 0, disorder};
+  SPX_TRY(check_spec(s));
+  if (!psi) return fail(SPX_EINVAL, "null output");
+  const size_t np = nspins * (nspins - 1) / 2;
+  std::vector<uint32_t> lo(np + 1), hi(np + 1);
+  std::vector<double> jz(np + 1), hz(nspins);
+  const int rc = draw_realisation(s, index, lo.data(), hi.data(), jz.data(), hz.data(), psi);
+  if (rc == SPX_EDOMAIN) return fail(rc, "cannot normalize a zero state");
+  if (rc != SPX_OK) return fail(rc, "could not place the spins at the minimum separation");
   return SPX_OK;
 }
 

@@ -20,6 +20,11 @@ int check_spec(const DipolarSpec& s);
 // One realisation: pairs (lo < hi, in generation order) with the secular
 // coupling jz (jx = jy = -jz / 2), then the Sz disorder hz.
 int draw_dipolar(const DipolarSpec& s, uint64_t index, uint32_t* lo, uint32_t* hi, double* jz, double* hz);
+// The same draws followed by the realisation's typical initial state
+// (psi: 2 * 2^L doubles, interleaved, normalised); SPX_EDOMAIN for a zero
+// state.
+int draw_realisation(const DipolarSpec& s, uint64_t index, uint32_t* lo, uint32_t* hi, double* jz, double* hz,
+                     double* psi);
 // Realisations first .. first + count - 1 as dense symmetric J [count][L][L]
 // and fields h [count][L], in FP64 and/or FP32 (null outputs are skipped).
 // threads <= 0 picks the pool size.

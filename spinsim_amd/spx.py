@@ -150,6 +150,8 @@ SIGNATURES = {
     "spx_batch_destroy": (C.c_int, [_vp]),
     "spx_batch_add": (C.c_int, [_vp, C.c_size_t, _dp, C.c_size_t, _u32p, _u32p, _dp, _dp, _dp,
                                 _dp]),
+    "spx_batch_add_dipolar": (C.c_int, [_vp, C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double,
+                                        C.c_double, C.c_int, C.c_double]),
     "spx_batch_run": (C.c_int, [_vp, _dp, _dp]),
     "spx_batch_width": (C.c_int, [_vp, _szp]),
     "spx_batch_clear": (C.c_int, [_vp]),
@@ -167,6 +169,8 @@ SIGNATURES = {
                                           C.c_double, _u32p, _u32p, _dp, _dp, _dp, _dp]),
     "spx_dipolar_graphs": (C.c_int, [C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double, C.c_double,
                                      C.c_int, C.c_double, _dp, _dp]),
+    "spx_dipolar_state": (C.c_int, [C.c_uint64, C.c_size_t, C.c_size_t, C.c_double, C.c_double, C.c_int,
+                                    C.c_double, _dp]),
     "spx_gnn_gradient": (C.c_int, [_vp, C.c_size_t, C.c_size_t, _dp, _dp, _fp, _fp, _fp]),
     "spx_gnn_train_step": (C.c_int, [_vp, C.c_size_t, C.c_size_t, _dp, _dp, _fp, C.c_float, _fp]),
 }
@@ -616,6 +620,14 @@ class Batch:
                                   hi.ctypes.data_as(_u32p), _d(jx), _d(jy), _d(jz), _d(p)))
         self.count += 1
 
+    def add_dipolar(self, seed: int, first: int, count: int, nspins: int, scale: float = 1.0,
+                    min_separation: float = 0.5, angular: bool = True, disorder: float = 0.5):
+        """Realisations first .. first+count-1 of the dipolar ensemble with
+        their random initial states, drawn inside libspx (runRealization order)."""
+        check(lib().spx_batch_add_dipolar(self.h, seed, first, count, nspins, scale, min_separation,
+                                          int(angular), disorder))
+        self.count += count
+
     def run(self):
         points = self.program.measurements()
         w = self.width()
@@ -648,6 +660,14 @@ def dipolar_realisation(seed: int, index: int, nspins: int, scale: float = 1.0, 
                                         lo.ctypes.data_as(_u32p), hi.ctypes.data_as(_u32p), _d(jx), _d(jy),
                                         _d(jz), _d(hz)))
     return Dipolar(lo, hi, jx, jy, jz, hz)
+
+
+def dipolar_state(seed: int, index: int, nspins: int, scale: float = 1.0, min_separation: float = 0.5,
+                  angular: bool = True, disorder: float = 0.5) -> np.ndarray:
+    """The random initial state of realisation (seed, index), interleaved."""
+    psi = np.zeros(2 << nspins)
+    check(lib().spx_dipolar_state(seed, index, nspins, scale, min_separation, int(angular), disorder, _d(psi)))
+    return psi
 
 
 def dipolar_graphs(seed: int, first: int, count: int, nspins: int, scale: float = 1.0,

@@ -111,3 +111,18 @@ def test_dipolar_draw_errors():
         spx.dipolar_realisation(1, 0, 31)
     with pytest.raises(spx.ConvergenceError):   # 9 spins cannot sit 2 apart in a 3 x 3 box
         spx.dipolar_realisation(1, 0, 9, min_separation=2.0)
+
+
+def test_dipolar_initial_state_matches_oracle_bitwise():
+    """The realisation's typical initial state drawn inside libspx (after the
+    lattice and the disorder, on the same stream) equals the oracle's
+    prepareInitialState bit for bit; the oracle's state is pinned by the
+    documented demo row (tests/test_oracle.py)."""
+    for L, angular, disorder in ((3, False, 0.0), (10, True, 0.5), (10, False, 100.0), (14, True, 0.5)):
+        for index in (0, 19, 4321):
+            _, _, r = O.dipolar_lattice(L, 20240726, index, scale=1.0, min_sep=0.5, angular=angular,
+                                        disorder=disorder)
+            want = np.zeros(2 << L)
+            assert O.lib().so_prepare_initial_state(O._d(want), L, 0, O.C.byref(r), 0) == 0
+            got = spx.dipolar_state(20240726, index, L, 1.0, 0.5, angular, disorder)
+            assert np.array_equal(got, want), (L, index)
