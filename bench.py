@@ -326,7 +326,7 @@ def main():
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule")
     ap.add_argument("--set", type=int, default=8, metavar="S",
-                    help="realisations stepped together, one launch per order (1..8; 1 = a single state)")
+                    help="realisations stepped together, one launch per order (1..16; 1 = a single state)")
     ap.add_argument("--graphs", action="store_true", help="replay each step as a captured HIP graph")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="2: a set of 4..8 runs as two halves on two streams (the default); 1: one stream")
@@ -346,8 +346,8 @@ def main():
     L = args.nspins
     n = 1 << L
     S = args.set
-    if not 1 <= S <= 8:
-        raise SystemExit("--set takes 1..8 realisations per launch")
+    if not 1 <= S <= 16:
+        raise SystemExit("--set takes 1..16 realisations per launch")
     systems = [synthetic_system(L, index=i) for i in range(S)]
     fields, lo, hi, jx, jy, jz, _ = systems[0]
     log(f"[bench] L={L} pairs={lo.size} realisations={S} building device objects")

@@ -118,7 +118,7 @@ int spx_ham_schedule(const spx_ham* h, char* buf, size_t len);
  * and caches it per problem shape (L, whether the couplings are flip-flop
  * only, flip-term count) for the process; `report` receives the timings. */
 int spx_ham_autotune(spx_ham* h, double bound, char* report, size_t len);
-/* The same with `nvec` states (1..8) sharing every timed launch, the width of
+/* The same with `nvec` states (1..16) sharing every timed launch, the width of
  * a batched evolve (spx_prop_evolve_many / spx_prop_evolve_set); winners are
  * cached per width as well. */
 int spx_ham_autotune_batch(spx_ham* h, double bound, int nvec, char* report, size_t len);
@@ -147,7 +147,7 @@ int spx_prop_evolve_many(spx_prop* p, spx_ham* h, int mode, spx_state* const* st
                          size_t nstates, double tau, size_t nsteps, int autonormalize);
 /* State i evolved under hams[i] (a realisation each, as in LocalRunner::run,
  * src/ensemble/EnsembleRunner.cpp:27-88, which evolves them one per thread):
- * each with its own spectral bound and expansion order, up to 8 states per
+ * each with its own spectral bound and expansion order, up to 16 states per
  * order launch where their Hamiltonians take the Hadamard or all-pairs path
  * together (else one by one). Every state's result is bitwise that of
  * spx_prop_evolve on it alone; last_order reports the longest expansion. */

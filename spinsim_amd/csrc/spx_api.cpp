@@ -276,7 +276,12 @@ int spx_ham_destroy(spx_ham* h) {
   if (h->d_tab) (void)hipFree(h->d_tab);
   if (h->d_scratch) (void)hipFree(h->d_scratch);
   if (h->d_coef) (void)hipFree(h->d_coef);
-  if (h->d_wht_e) (void)hipFree(h->d_wht_e);
+  for (auto& t : h->wht_cache) {
+    if (t.copied) (void)hipEventSynchronize(t.copied);
+    if (t.copied) (void)hipEventDestroy(t.copied);
+    if (t.dev) (void)hipFree(t.dev);
+    if (t.host) (void)hipHostFree(t.host);
+  }
   if (h->d_wht) (void)hipFree(h->d_wht);
   delete h;
   return SPX_OK;

@@ -249,7 +249,7 @@ int batch_run_streaming(spx_batch* b, size_t first, size_t count, double* times,
   const size_t points = program_measurements(b->program, 0, b->program.ops.size());
   static const int env_workers = [] {
     const char* e = std::getenv("SPX_BATCH_WORKERS");
-    return e ? std::atoi(e) : 4;
+    return e ? std::atoi(e) : 8;
   }();
   std::vector<size_t> idx(count);
   for (size_t i = 0; i < count; ++i) idx[i] = first + i;

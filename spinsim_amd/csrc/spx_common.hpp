@@ -17,7 +17,7 @@ namespace spx {
 constexpr double kNegligible = 1.0e-14;
 constexpr size_t kMaxSpins = 30;          // include/spinsim/core/Bits.hpp:13
 constexpr int kMaxEvents = 16;
-constexpr int kMaxMany = 8;               // states per batched evolve launch
+constexpr int kMaxMany = 16;              // states per batched evolve launch
 constexpr int kReduceBlocks = 1024;       // fixed reduction fan-in (determinism)
 constexpr int kReduceThreads = 256;
 
@@ -113,8 +113,22 @@ struct spx_ham {
   bool dense_ok = false;         // specialised all-pairs kernel usable
   double* d_coef = nullptr;      // its weights
   bool wht_ok = false;           // Hadamard-sandwich flip-flop path selected
-  double* d_wht_e = nullptr;     // its Hadamard-basis diagonals A then B, factored (spx_hadamard.hip)
+  double* d_wht_e = nullptr;     // its Hadamard-basis diagonals A then B, factored (spx_hadamard.hip);
+                                 // points into the table cache below
   size_t wht_tab = 0;            // doubles per factor table (B starts there)
+  // Factor tables of the field/coupling/scale sets this Hamiltonian has
+  // seen (a pulse program alternates between a few), so that returning to
+  // one costs no host work and no upload. Owned here; freed on destroy.
+  struct WhtTables {
+    std::vector<double> key;     // the weights the tables were built from
+    double* host = nullptr;      // pinned copy of the tables (the upload's source)
+    double* dev = nullptr;
+    size_t count = 0;            // doubles in the tables
+    hipEvent_t copied = nullptr; // after the upload from `host`
+    uint64_t used = 0;           // last-use stamp
+  };
+  std::vector<WhtTables> wht_cache;
+  uint64_t wht_clock = 0;
   double2* d_wht = nullptr;      // its transform scratch: wht_slots pairs of 2^L vectors
   int wht_slots = 0;
   double terms_scale = 0.0;

@@ -46,6 +46,7 @@
 // The transforms are exact sums and differences, the phases i^k exact swaps
 // and sign changes; results agree with the gather kernels to rounding (a few
 // ulps of the amplitude scale).
+#include <algorithm>
 #include <cmath>
 #include <utility>
 
@@ -476,6 +477,17 @@ int hadamard_prepare(spx_ham* h, const std::vector<double>& anti, const std::vec
   const size_t L = h->nspins;
   if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard path needs 14 <= L <= 20");
   const size_t T = size_t(1) << kTB, G = L - kTB, R = size_t(1) << G, TS = T + R + G * T;
+  // tables already built for these weights: point at them
+  std::vector<double> key;
+  key.reserve(2 * anti.size() + 2 * L + L + dpair.size());
+  for (const auto* v : {&anti, &aligned, &cx, &cy, &dfield, &dpair}) key.insert(key.end(), v->begin(), v->end());
+  for (auto& t : h->wht_cache)
+    if (t.key == key) {
+      t.used = ++h->wht_clock;
+      h->d_wht_e = t.dev;
+      h->wht_tab = TS;
+      return hadamard_reserve(h, 1);
+    }
   const int sc = -static_cast<int>(L);
   auto q_of = [&](size_t lo, size_t hi) { return lo * L - lo * (lo + 1) / 2 + (hi - lo - 1); };
   auto z = [](size_t k, size_t bit) { return ((k >> bit) & 1u) ? -1.0 : 1.0; };
@@ -535,14 +547,42 @@ int hadamard_prepare(spx_ham* h, const std::vector<double>& anti, const std::vec
       }
     }
   }
+  // A cache slot: a new one up to kWhtCache, else the least recently used.
+  // The upload is ordered on the context's stream after every launch that
+  // may still read the slot's old tables (a second stream of a split set is
+  // joined back into it before the set evolve returns), so no host wait is
+  // needed; only the pinned source is reused after its previous upload.
+  constexpr size_t kWhtCache = 8;
   spx_ctx* c = h->ctx;
-  constexpr size_t kMaxTS = (size_t(1) << kTB) + (size_t(1) << (kHadamardMaxSpins - kTB)) +
-                            (kHadamardMaxSpins - kTB) * (size_t(1) << kTB);
-  constexpr size_t kMaxDS = (size_t(1) << kTB) + (size_t(1) << (kHadamardMaxSpins - kTB)) * (kTB + 1);
-  if (!h->d_wht_e) SPX_HIP(hipMalloc(&h->d_wht_e, (2 * kMaxTS + kMaxDS) * sizeof(double)));
+  spx_ham::WhtTables* slot = nullptr;
+  if (h->wht_cache.size() < kWhtCache) {
+    h->wht_cache.emplace_back();
+    slot = &h->wht_cache.back();
+  } else {
+    slot = &h->wht_cache[0];
+    for (auto& t : h->wht_cache)
+      if (t.used < slot->used) slot = &t;
+  }
+  if (slot->count < tab.size()) {
+    if (slot->copied) SPX_HIP(hipEventSynchronize(slot->copied));
+    if (slot->dev) SPX_HIP(hipFree(slot->dev));
+    if (slot->host) SPX_HIP(hipHostFree(slot->host));
+    slot->dev = nullptr;
+    slot->host = nullptr;
+    slot->count = 0;
+    SPX_HIP(hipMalloc(&slot->dev, tab.size() * sizeof(double)));
+    SPX_HIP(hipHostMalloc(&slot->host, tab.size() * sizeof(double), hipHostMallocDefault));
+    slot->count = tab.size();
+  }
+  if (!slot->copied) SPX_HIP(hipEventCreateWithFlags(&slot->copied, hipEventDisableTiming));
+  SPX_HIP(hipEventSynchronize(slot->copied));   // the pinned source is free again
+  std::copy(tab.begin(), tab.end(), slot->host);
+  SPX_HIP(hipMemcpyAsync(slot->dev, slot->host, tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  SPX_HIP(hipEventRecord(slot->copied, c->stream));
+  slot->key = std::move(key);
+  slot->used = ++h->wht_clock;
+  h->d_wht_e = slot->dev;
   h->wht_tab = TS;
-  SPX_HIP(hipMemcpyAsync(h->d_wht_e, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  SPX_HIP(hipStreamSynchronize(c->stream));  // `tab` is pageable and local
   return hadamard_reserve(h, 1);
 }
 

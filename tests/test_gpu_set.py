@@ -1,6 +1,6 @@
 """Propagator.evolve_set: realisations (one Hamiltonian each, as in
 LocalRunner::run, src/ensemble/EnsembleRunner.cpp:27-88) stepped together,
-up to eight per order launch, each with its own spectral bound and
+up to sixteen per order launch, each with its own spectral bound and
 expansion order. Every state must come out bitwise as when evolved alone."""
 import numpy as np
 import pytest
@@ -218,7 +218,7 @@ def test_batched_autotune_reports_the_hadamard_path(ctx):
     banner(report[:300])
     assert report.startswith("chosen ") and "hadamard: " in report
     with pytest.raises(spx.InvalidArgument):
-        h.autotune(h.spectral_bound(), nvec=9)
+        h.autotune(h.spectral_bound(), nvec=17)
 
 
 def test_default_path_follows_size_and_launch_width(ctx):
