@@ -18,6 +18,20 @@ using namespace spx;
 
 namespace spx {
 
+int Realisation::initial_state(double* out) const {
+  if (!drawn_state) {
+    std::copy(psi0.begin(), psi0.end(), out);
+    return SPX_OK;
+  }
+  const size_t np = nspins * (nspins - 1) / 2;
+  std::vector<uint32_t> l(np + 1), h(np + 1);
+  std::vector<double> z(np + 1), f(nspins);
+  const int rc = draw_realisation(spec, index, l.data(), h.data(), z.data(), f.data(), out);
+  if (rc == SPX_EDOMAIN) return fail(rc, "cannot normalize a zero state");
+  if (rc != SPX_OK) return fail(rc, "could not place the spins at the minimum separation");
+  return SPX_OK;
+}
+
 size_t program_measurements(const spx_program& p, size_t b, size_t e) {
   size_t total = 0;
   for (size_t i = b; i < e; ++i) {
@@ -168,6 +182,7 @@ struct WorkerObjects {
   spx_state* state[kGroup] = {};
   spx_state* ref[kGroup] = {};
   spx_prop* prop = nullptr;
+  std::vector<double> psi;   // host staging of an initial state
 
   void release() {
     spx_prop_destroy(prop);
@@ -230,7 +245,9 @@ static int run_group(spx_batch* b, WorkerObjects& o, const size_t* idx, size_t n
       SPX_TRY(spx_ham_set_field(o.h[k], s, r.fields[3 * s], r.fields[3 * s + 1], r.fields[3 * s + 2]));
     for (size_t p = 0; p < r.lo.size(); ++p)
       SPX_TRY(spx_ham_set_coupling(o.h[k], r.lo[p], r.hi[p], r.jx[p], r.jy[p], r.jz[p]));
-    SPX_TRY(spx_state_upload(o.state[k], r.psi0.data()));
+    o.psi.resize(static_cast<size_t>(2) << r.nspins);
+    SPX_TRY(r.initial_state(o.psi.data()));
+    SPX_TRY(spx_state_upload(o.state[k], o.psi.data()));
     SPX_TRY(spx_apply_total_spin(b->reference_axis, o.state[k], o.ref[k]));
     m.push_back(Member{&r, o.h[k], o.state[k], o.ref[k], idx[k] == timed ? times : nullptr, values_of(idx[k])});
   }
@@ -434,10 +451,35 @@ int batch_run_resident(spx_batch* b, double* times, double* values) {
     R.measure_off = static_cast<int>(evo.size());
     evo.push_back(me);
     R.psi_off = psi_off;
-    psi.insert(psi.end(), r.psi0.begin(), r.psi0.end());
-    psi_off += static_cast<long long>(r.psi0.size());
+    psi_off += static_cast<long long>(static_cast<size_t>(2) << r.nspins);
     R.out_off = static_cast<long long>(i * points * b->width);
     cost[i] *= static_cast<double>(1u << r.nspins);
+  }
+  // initial states into one buffer, drawn or copied on host threads
+  psi.resize(static_cast<size_t>(psi_off));
+  {
+    std::atomic<size_t> next{0};
+    std::atomic<int> status{SPX_OK};
+    std::string msg;
+    auto work = [&] {
+      for (;;) {
+        const size_t i = next.fetch_add(1);
+        if (i >= n || status.load() != SPX_OK) return;
+        const int rc = b->items[i].initial_state(psi.data() + reals[i].psi_off);
+        if (rc != SPX_OK) {
+          int ok = SPX_OK;
+          if (status.compare_exchange_strong(ok, rc)) msg = spx_last_error();
+          return;
+        }
+      }
+    };
+    const size_t threads = std::min<size_t>(std::max<size_t>(1, n / 64),
+                                            std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (status.load() != SPX_OK) return fail(status.load(), msg);
   }
   if (w.size() > static_cast<size_t>(0x7fffffff)) return fail(SPX_ENOMEM, "batch tables too large");
   // realisations grouped by L (one persistent launch per group, queue
@@ -657,13 +699,14 @@ int spx_batch_add_dipolar(spx_batch* b, uint64_t seed, size_t first, size_t coun
       r.lo.resize(np);
       r.hi.resize(np);
       r.jz.resize(np);
-      r.psi0.resize(static_cast<size_t>(2) << nspins);
-      const int rc = draw_realisation(spec, first + i, r.lo.data(), r.hi.data(), r.jz.data(), hz.data(),
-                                      r.psi0.data());
+      const int rc = draw_dipolar(spec, first + i, r.lo.data(), r.hi.data(), r.jz.data(), hz.data());
       if (rc != SPX_OK) {
         status.store(rc);
         return;
       }
+      r.drawn_state = true;
+      r.spec = spec;
+      r.index = first + i;
       r.jx.resize(np);
       r.jy.resize(np);
       for (size_t p = 0; p < np; ++p) r.jx[p] = r.jy[p] = -0.5 * r.jz[p];
@@ -677,7 +720,6 @@ int spx_batch_add_dipolar(spx_batch* b, uint64_t seed, size_t first, size_t coun
   for (size_t t = 1; t < threads; ++t) pool.emplace_back(work);
   work();
   for (auto& t : pool) t.join();
-  if (status.load() == SPX_EDOMAIN) return fail(SPX_EDOMAIN, "cannot normalize a zero state");
   if (status.load() != SPX_OK) return fail(status.load(), "could not place the spins at the minimum separation");
   for (Realisation& r : drawn) b->items.push_back(std::move(r));
   return SPX_OK;

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "spx_common.hpp"
+#include "spx_ensemble.hpp"
 
 namespace spx {
 
@@ -48,7 +49,15 @@ struct Realisation {
   std::vector<double> fields;     // [3L]
   std::vector<uint32_t> lo, hi;
   std::vector<double> jx, jy, jz;
-  std::vector<double> psi0;       // [2 * 2^L]
+  std::vector<double> psi0;       // [2 * 2^L]; empty when drawn on use (below)
+  // Realisations added by (seed, index) keep only their couplings and fields
+  // and draw their initial state when a run needs it (a state of L = 16 is
+  // 1 MiB; a plain Monte-Carlo arm holds tens of thousands).
+  bool drawn_state = false;
+  DipolarSpec spec{};
+  uint64_t index = 0;
+  // The initial state into `out` (2 * 2^L doubles).
+  int initial_state(double* out) const;
 };
 
 }  // namespace spx
