@@ -85,7 +85,7 @@ def cpu_baseline(L, fields, lo, hi, jx, jy, jz, order_bytes, target_s=15.0):
     + addScaled (src/core/Propagator.cpp:58-72), one independent state per
     thread, like LocalRunner's realisation pool. Bounded sample."""
     from oracle import oracle as O
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = max(1, len(os.sched_getaffinity(0)))
     f = np.ascontiguousarray(fields)
     probe_orders = 2
     if O.ref_available():
@@ -122,96 +122,66 @@ def cpu_baseline(L, fields, lo, hi, jx, jy, jz, order_bytes, target_s=15.0):
 
 # ------------------------------------------------ control-variate leg
 
-CV_SEED = 20240726
+def control_variate_leg(spx, ctx):
+    """Seconds to a +-1e-3 error bar on every point of the mean XY-16 echo at
+    L = 16: plain batched Monte-Carlo against the control variate (the
+    surrogate on 1e6 random geometries plus the exact correction), both arms
+    run at the sample sizes the held-out variances prescribe; the one-shot
+    cost with the surrogate's training counted is reported beside it
+    (spinsim_amd/ensembles.py)."""
+    from spinsim_amd import ensembles as E
+    return E.control_variate(ctx, log=lambda m: log(f"[bench] {m}")).line
 
 
-def cpmg_program(spx, L, cycles=32, steps=5):
-    """pi/2_x on every spin, then `cycles` x [tau, pi_y, tau, measure],
-    tau = steps x 0.05."""
-    P = spx.Program()
-    allspins = list(range(L))
-    P.pulse((1, 0, 0), math.pi / 2, allspins)
-    P.repeat(cycles)
-    P.evolve(0.05, steps)
-    P.pulse((0, 1, 0), math.pi, allspins)
-    P.evolve(0.05, steps)
-    P.measure()
-    P.end()
-    return P
-
-
-def exact_curves(spx, ctx, L, first, count):
-    """Echo curves (mean <sigma_y> over spins, from the all-down state) of
-    realisations first .. first+count-1, by the resident batch runner; the
-    run's wall time (the host-side adds are not timed)."""
-    obs = [(spx.OBS_MAG_Y, s) for s in range(L)]
-    b = spx.Batch(ctx, cpmg_program(spx, L), obs, epsilon=1e-7, reference_axis=0)
-    psi0 = np.zeros(2 << L)
-    psi0[0] = 1.0
-    f = np.zeros((L, 3))
-    for i in range(first, first + count):
-        d = spx.dipolar_realisation(CV_SEED, i, L)
-        f[:, 2] = d.hz
-        b.add(L, f, d.lo, d.hi, d.jx, d.jy, d.jz, psi0)
+def config0_leg(spx, ctx, count=20):
+    """The plain CPMG echo configuration on the device: L = 10, pi/2_x then
+    8 x [10 x 0.05, ideal pi_y, 10 x 0.05, measure], corr:x and norm,
+    `count` realisations in one batch (the LDS-resident launch)."""
+    from spinsim_amd import ensembles as E
+    b = spx.Batch(ctx, E.cpmg_program(10, 8), [(spx.OBS_CORR_X, 0), (spx.OBS_NORM, 0)], epsilon=1e-7,
+                  reference_axis=0)
+    b.add_dipolar(E.SEED, 0, count, 10, **E.DRAW)
+    b.run()   # warm: module load and first-launch costs
     ctx.synchronize()
     t0 = time.perf_counter()
-    _, v = b.run()
-    return v.mean(axis=2), time.perf_counter() - t0
+    b.run()
+    dt = time.perf_counter() - t0
+    return {"workload": f"L=10 CPMG, 8 echoes, {count} realisations", "s": round(dt, 4),
+            "realisations_per_s": round(count / dt, 1)}
 
 
-def control_variate_leg(spx, ctx, L=10, target=1e-3):
-    """Device seconds to a +-target error bar on every point of the mean echo
-    curve: plain batched Monte-Carlo against the control variate (exact
-    correction + surrogate draws), both actually run at the sample sizes the
-    held-out variances prescribe. Training cost is reported beside it."""
-    from spinsim_amd import surrogate as S
-    # 24576 training curves: the held-out worst-point variance reduction is
-    # 8.0 against 6.3 with 12288 (same 60 epochs); training stays a one-off
-    n_tr, n_va = 24576, 2048
-    log(f"[bench] control variate: training set of {n_tr} + {n_va} exact L={L} curves")
-    Ytr, t_ytr = exact_curves(spx, ctx, L, 0, n_tr)
-    Yva, t_yva = exact_curves(spx, ctx, L, n_tr, n_va)
-    Jtr, htr = spx.dipolar_graphs(CV_SEED, 0, n_tr, L)
-    Jva, hva = spx.dipolar_graphs(CV_SEED, n_tr, n_va, L)
-    model = S.Model(ctx, Jtr, htr, Ytr, seed=11)
-    t0 = time.perf_counter()
-    model.train(Jtr, htr, Ytr, epochs=60, batch=128, lr=2e-3, seed=1, decay=0.97)
-    t_train = time.perf_counter() - t0
-    gva = model.predict(Jva, hva)
-    beta = S.fit_beta(Yva, gva)
-    var_y = Yva.var(axis=0, ddof=1)
-    var_r = (Yva - beta * gva).var(axis=0, ddof=1)
-    var_g = beta * beta * gva.var(axis=0, ddof=1)
-    se2 = target * target
-    n_p = int(math.ceil(var_y.max() / se2))
-    n_c = int(math.ceil(var_r.max() / (0.9 * se2)))
-    m_c = int(math.ceil(var_g.max() / (0.1 * se2)))
-    log(f"[bench] plain: {n_p} exact runs")
-    Yp, t_plain = exact_curves(spx, ctx, L, 1_000_000, n_p)
-    plain = S.plain_estimate(Yp)
-    log(f"[bench] control variate: {n_c} exact runs + {m_c} surrogate draws")
-    Yc, t_exact_c = exact_curves(spx, ctx, L, 2_000_000, n_c)
-    Jc, hc = spx.dipolar_graphs(CV_SEED, 2_000_000, n_c, L)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    gc = model.predict(Jc, hc)
-    gx = model.predict_dipolar(CV_SEED, 3_000_000, m_c, L)
-    t_sur = time.perf_counter() - t0
-    cv = S.cv_estimate(Yc, gc, gx, beta)
-    t_cv = t_exact_c + t_sur
-    z = float(np.max(np.abs(cv.mean - plain.mean) / np.sqrt(cv.stderr ** 2 + plain.stderr ** 2)))
-    return {
-        "workload": f"L={L} dipolar XXZ (min spacing 0.5, Hz sigma 0.5), CPMG pi/2_x + 32 x [tau=0.25, pi_y, "
-                    f"tau], mean <sigma_y> per echo from the all-down state; +-{target:g} on every point",
-        "plain": {"s": round(t_plain, 3), "exact_runs": n_p, "max_stderr": float(plain.stderr.max())},
-        "control_variate": {"s": round(t_cv, 3), "exact_runs": n_c, "surrogate_draws": m_c,
-                            "max_stderr": float(cv.stderr.max()), "surrogate_s": round(t_sur, 3)},
-        "speedup": round(t_plain / t_cv, 2),
-        "worst_point_variance_reduction": round(float(var_y.max() / var_r.max()), 2),
-        "max_abs_z_cv_vs_plain": round(z, 2),
-        "training": {"exact_runs": n_tr + n_va, "exact_s": round(t_ytr + t_yva, 3), "train_s": round(t_train, 3),
-                     "note": "one-off; reused by every later estimate of this ensemble"},
-    }
+def cpu_realisation_rates(threads, target_s=8.0):
+    """Realisations per second of the configurations' exact runs on the
+    host's cores: the oracle's runRealization (the reference's algorithm and
+    flags), one realisation per thread at a time as LocalRunner runs them.
+    Bounded samples; best and median of repeats where a sample is short."""
+    from oracle import oracle as O
+    out = {}
+
+    def timed(cfg, n):
+        t0 = time.perf_counter()
+        O.run_ensemble(cfg, n, threads=threads)
+        return time.perf_counter() - t0
+
+    # plain CPMG echo, L = 10, 20 realisations
+    c0 = O.dipolar_config(10, O.cpmg_program(10, 8))
+    reps = [timed(c0, 20)]
+    while sum(reps) < target_s and len(reps) < 9:
+        reps.append(timed(c0, 20))
+    out["cpmg_L10"] = {"value": round(20 / min(reps), 2), "median": round(20 / float(np.median(reps)), 2),
+                       "unit": "realisations/s", "sample": f"20 realisations x {len(reps)} repeats, best and median"}
+    # uneven L in 8..12, XY-8 x 16: one realisation per thread at each size
+    tot, n = 0.0, 0
+    for L in range(8, 13):
+        tot += timed(O.dipolar_config(L, O.xy8_program(L)), threads)
+        n += threads
+    out["xy8_L8_12"] = {"value": round(n / tot, 3), "unit": "realisations/s",
+                        "sample": f"{threads} realisations at each L in 8..12 ({n}), {tot:.1f} s"}
+    # L = 16 XY-16 with finite pulses: one realisation per thread
+    t4 = timed(O.dipolar_config(16, O.xy16_program(16)), threads)
+    out["xy16_L16"] = {"value": round(threads / t4, 3), "unit": "realisations/s",
+                       "sample": f"{threads} realisations, {t4:.1f} s"}
+    return out
 
 
 def xy8_program(spx, blocks=16, omega=50.0, free_steps=10):
@@ -321,7 +291,7 @@ def main():
     ap.add_argument("--nspins", type=int, default=18)
     ap.add_argument("--epsilon", type=float, default=1e-12)
     ap.add_argument("--dt", type=float, default=0.05)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule")
@@ -485,13 +455,21 @@ def main():
     if single:
         line["single_state"] = single
     if not args.no_cv:
+        log("[bench] control variate at L=16 ...")
         line["control_variate"] = control_variate_leg(spx, ctx)
         log("[bench] batched ensemble leg ...")
         line["batched_ensemble"] = batched_leg(spx, ctx)
+        line["cpmg_L10"] = config0_leg(spx, ctx)
+        cv = line["control_variate"]
+        line["realisations_per_s"] = {
+            "cpmg_L10": line["cpmg_L10"]["realisations_per_s"],
+            "xy8_L8_12": line["batched_ensemble"]["realisations_per_s"],
+            "xy16_L16": round(cv["plain"]["exact_runs"] / cv["plain"]["s"], 1)}
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(L, fields, lo, hi, jx, jy, jz, BYTES_PER_AMP_ORDER * n,
                                             args.cpu_seconds)
+        line["cpu_baseline"]["realisations_per_s"] = cpu_realisation_rates(line["cpu_baseline"]["cores"])
     print(json.dumps(line), flush=True)
 
 

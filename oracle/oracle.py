@@ -8,6 +8,7 @@ as the path being measured or shipped.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
@@ -419,3 +420,54 @@ def ref_test_state(L: int, seed: int) -> np.ndarray:
     out = np.zeros(2 << L)
     _std.std_random_state(L, seed, _d(out))
     return out
+
+
+# ------------------------------------------------------- experiment programs
+# The pulse programs of the ensemble workloads, restated for the oracle (the
+# device path builds its own in spinsim_amd.ensembles and bench.py).
+
+def cpmg_program(L: int, points: int, steps: int = 10, dt: float = 0.05) -> Program:
+    """pi/2_x on every spin, then points x [steps x dt, ideal pi_y, steps x dt, measure]."""
+    every = list(range(L))
+    p = Program().pulse((1, 0, 0), math.pi / 2, every).repeat(points)
+    return p.evolve(dt, steps).pulse((0, 1, 0), math.pi, every).evolve(dt, steps).measure().end()
+
+
+def pulse_fields(L: int, axis: str, field: float) -> np.ndarray:
+    f = np.zeros((L, 3))
+    f[:, 0 if axis == "x" else 1] = field
+    return f
+
+
+XY8_AXES = "xyxyyxyx"
+
+
+def xy8_program(L: int, blocks: int = 16, omega: float = 50.0, free_steps: int = 10) -> Program:
+    """A measure, then blocks x [XY-8 with finite pulses (pi / omega under
+    hx or hy = omega, hz 0), each after free_steps x 0.05, measure]."""
+    p = Program().measure().repeat(blocks)
+    for axis in XY8_AXES:
+        p.evolve(0.05, free_steps).evolve(math.pi / omega, 1, fields=pulse_fields(L, axis, omega))
+    return p.measure().end()
+
+
+def xy16_program(L: int, omega: float = 50.0, free: float = 0.05, per_measure: int = 4) -> Program:
+    """XY-16 (XY-8 and its phase-inverted copy), one free step of `free`
+    before each finite pulse, a measure after every `per_measure` pulses."""
+    seq = [(a, 1.0) for a in XY8_AXES] + [(a, -1.0) for a in XY8_AXES]
+    p = Program().repeat(1)
+    for k, (axis, sign) in enumerate(seq):
+        p.evolve(free, 1).evolve(math.pi / omega, 1, fields=pulse_fields(L, axis, sign * omega))
+        if (k + 1) % per_measure == 0:
+            p.measure()
+    return p.end()
+
+
+def dipolar_config(L: int, program: Program, observables=((OBS_CORR_X, 0), (OBS_NORM, 0)),
+                   seed: int = 20240726) -> "Config":
+    """runRealization for the dipolar ensemble of the workloads: geometry at
+    min spacing 0.5 with the secular angle, Sz disorder 0.5, a random initial
+    state, reference axis x, epsilon 1e-7."""
+    return Config(L=L, program=program, observables=observables, seed=seed, initial_state=0,
+                  generate_geometry=True, dipolar_scale=1.0, min_separation=0.5, angular=True,
+                  field_disorder=0.5, reference_axis=0, epsilon=1e-7)
