@@ -159,6 +159,19 @@ int spx_prop_steps_set(spx_prop* p, spx_ham* const* hams, int mode, spx_state* c
                        size_t count, const double* const* coeffs, const size_t* orders,
                        const double* bounds, size_t nsteps, int autonormalize);
 int spx_prop_last_order(const spx_prop* p, size_t* order);                      /* lastOrder */
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops). spx_prop_evolve on
+ * one state with no Propagator object and no device allocation, no host
+ * synchronisation and no copy: the Chebyshev scratch vectors and the
+ * diagonal live in a static device array for at most 16 spins (2.5 MiB per
+ * device, held until the process ends), and the Hamiltonian's fields and
+ * couplings are read from its host side. L > 16 is rejected with
+ * SPX_ERANGE; there is no autonormalisation. Every call on a device shares
+ * that device's array: calls that overlap on one device race (calls on one
+ * context's stream do not overlap). `order` (may be NULL) receives the
+ * expansion order. Results agree with spx_prop_evolve to rounding. */
+int spx_prop_evolve_noalloc(spx_ham* h, int mode, spx_state* psi, double tau, size_t nsteps, double epsilon,
+                            size_t* order);
 /* No reference counterpart: with enable = 1, evolve calls of at least 4
  * steps without autonormalisation record one step as a HIP graph and replay
  * it; with enable = 0 (the default) every order is launched from the host.

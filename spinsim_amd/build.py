@@ -15,7 +15,7 @@ LIBDIR = os.path.join(PKG, "lib")
 INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["spx_kernels.hip", "spx_hadamard.hip", "spx_resident.hip", "spx_gnn.hip", "spx_ensemble.cpp", "spx_tune.cpp", "spx_host.cpp", "spx_api.cpp", "spx_batch.cpp"]
+SOURCES = ["spx_kernels.hip", "spx_hadamard.hip", "spx_resident.hip", "spx_gnn.hip", "spx_noalloc.hip", "spx_ensemble.cpp", "spx_tune.cpp", "spx_host.cpp", "spx_api.cpp", "spx_batch.cpp"]
 
 # Packed-FP32 instruction selection is switched off for every device pass;
 # all arithmetic here is FP64 anyway.

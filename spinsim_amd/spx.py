@@ -153,6 +153,7 @@ SIGNATURES = {
     "spx_batch_add_dipolar": (C.c_int, [_vp, C.c_uint64, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double,
                                         C.c_double, C.c_int, C.c_double]),
     "spx_batch_run": (C.c_int, [_vp, _dp, _dp]),
+    "spx_prop_evolve_noalloc": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_size_t, C.c_double, _szp]),
     "spx_batch_width": (C.c_int, [_vp, _szp]),
     "spx_batch_clear": (C.c_int, [_vp]),
     "spx_gnn_create": (C.c_int, [_vp, C.c_size_t, C.c_uint64, C.c_double, C.c_double, C.POINTER(_vp)]),
@@ -660,6 +661,16 @@ def dipolar_realisation(seed: int, index: int, nspins: int, scale: float = 1.0, 
                                         lo.ctypes.data_as(_u32p), hi.ctypes.data_as(_u32p), _d(jx), _d(jy),
                                         _d(jz), _d(hz)))
     return Dipolar(lo, hi, jx, jy, jz, hz)
+
+
+def evolve_noalloc(ham: "Hamiltonian", mode: int, psi: "StateVector", tau: float, nsteps: int = 1,
+                   epsilon: float = 1e-7) -> int:
+    """spx_prop_evolve_noalloc: Propagator::evolve on one state from the
+    library's static scratch (no device allocation; at most 16 spins).
+    Returns the expansion order."""
+    order = C.c_size_t()
+    check(lib().spx_prop_evolve_noalloc(ham.h, mode, psi.h, tau, nsteps, epsilon, C.byref(order)))
+    return order.value
 
 
 def dipolar_state(seed: int, index: int, nspins: int, scale: float = 1.0, min_separation: float = 0.5,
