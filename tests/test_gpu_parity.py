@@ -401,6 +401,35 @@ def test_resident_batch_uneven_sizes_finite_pulses_match_oracle(ctx):
     assert np.array_equal(gv, gv2)
 
 
+def test_resident_batch_full_xy8_program_matches_oracle(ctx):
+    """The uneven-size batched workload's whole pulse program: XY-8 with
+    finite pulses repeated 16 times (128 pulses), a measure before and after
+    every block, on two realisations of each L = 8..12 drawn inside libspx
+    (spx_batch_add_dipolar: lattice, disorder and the random initial state
+    from the realisation's stream); every row within 1e-10 of the oracle's
+    runRealization."""
+    from concurrent.futures import ThreadPoolExecutor
+    banner("resident batch, XY-8 x 16 at L = 8..12: oracle (CPU, about 20 s)")
+    Ls = list(range(8, 13))
+    obs = ((O.OBS_CORR_X, 0), (O.OBS_NORM, 0))
+
+    def oracle_rows(L):
+        return O.run_ensemble(O.dipolar_config(L, O.xy8_program(L), observables=obs), 2, threads=2)[1]
+
+    with ThreadPoolExecutor(len(Ls)) as pool:
+        want = list(pool.map(oracle_rows, Ls))
+    b = spx.Batch(ctx, xy8(spx.Program(), 0, 16), obs, epsilon=1e-7, reference_axis=0)
+    for L in Ls:
+        b.add_dipolar(20240726, 0, 2, L)
+    _, gv = b.run()
+    assert gv.shape[0] == 2 * len(Ls) and gv.shape[1] == 17
+    for g, L in enumerate(Ls):
+        for i in range(2):
+            err = float(np.max(np.abs(gv[2 * g + i] - want[g][i])))
+            assert err < 1e-10, (L, i, err)
+    banner("resident batch, XY-8 x 16: every row within 1e-10")
+
+
 def test_imaginary_time_relaxes_to_the_ground_state(ctx):
     """Imaginary-time propagation of -H with autonormalisation (the ground-state
     run of the single-large-state workload) reaches the lowest eigenvalue of H
