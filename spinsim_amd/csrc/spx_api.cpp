@@ -949,11 +949,9 @@ int spx_rotate(spx_state* s, double ax, double ay, double az, double angle, cons
   const double nx = ax / length, ny = ay / length, nz = az / length;
   const double c = std::cos(0.5 * angle), sn = std::sin(0.5 * angle);
   const double u[8] = {c, sn * nz, sn * ny, -sn * nx, -sn * ny, -sn * nx, c, -sn * nz};
-  for (size_t t = 0; t < ntargets; ++t) {
+  for (size_t t = 0; t < ntargets; ++t)
     if (targets[t] >= s->nspins) return fail(SPX_ERANGE, "rotation names a spin outside the system");
-    SPX_TRY(launch_rotate(s->ctx, s->d, s->n, targets[t], u));
-  }
-  return SPX_OK;
+  return launch_rotate_targets(s->ctx, s->d, s->nspins, targets, ntargets, u);
 }
 
 int spx_two_spin_gate(spx_state* s, const double* u16, size_t a, size_t b) {

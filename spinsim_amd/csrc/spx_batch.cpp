@@ -69,34 +69,67 @@ struct Member {
   size_t row = 0;
 };
 
+// Per measured value, four doubles of raw sums on the device: (re, im) of
+// the observable's fixed-order reduction, and for norm the second vector's.
+constexpr size_t kRawPerObs = 4;
+
 struct Walker {
   spx_batch* b;
   std::vector<Member>& m;
   spx_prop* prop;
+  double* d_raw;        // [member][row][observable][kRawPerObs]
+  size_t points;
   double time = 0.0;
 
-  int measure(Member& x) {
-    double* out = x.values + x.row * b->width;
-    size_t at = 0;
-    for (size_t o = 0; o < b->obs_codes.size(); ++o) {
+  // The observables' reductions are left on the device (no host wait per
+  // measurement); finish() reads them all once the group is done.
+  int measure(size_t k) {
+    Member& x = m[k];
+    const size_t nobs = b->obs_codes.size();
+    double* raw = d_raw + (k * points + x.row) * nobs * kRawPerObs;
+    const size_t L = x.state->nspins;
+    spx_ctx* c = x.state->ctx;
+    for (size_t o = 0; o < nobs; ++o) {
       const int code = b->obs_codes[o];
+      double* dst = raw + o * kRawPerObs;
       if (code <= 2) {
-        SPX_TRY(spx_measure_corr(code, x.state, x.ref, &out[at], &out[at + 1]));
-        at += 2;
+        SPX_TRY(reduce_corr_dev(c, code, x.state->d, x.ref->d, L, dst));
       } else if (code <= 5) {
-        SPX_TRY(spx_measure_mag(code - 3, static_cast<size_t>(b->obs_spins[o]), x.state, &out[at]));
-        at += 1;
+        SPX_TRY(reduce_mag_dev(c, code - 3, static_cast<uint32_t>(b->obs_spins[o]), x.state->d, L, dst));
       } else if (code == 6) {
-        SPX_TRY(spx_state_norm(x.state, &out[at]));
-        SPX_TRY(spx_state_norm(x.ref, &out[at + 1]));
-        at += 2;
+        SPX_TRY(reduce_norm2_dev(c, x.state->d, x.state->n, dst));
+        SPX_TRY(reduce_norm2_dev(c, x.ref->d, x.ref->n, dst + 2));
       } else {
-        SPX_TRY(spx_measure_energy(x.h, x.state, &out[at]));
-        at += 1;
+        SPX_TRY(reduce_energy_dev(x.h, x.state->d, dst));
       }
     }
     if (x.times) x.times[x.row] = time;
     ++x.row;
+    return SPX_OK;
+  }
+
+  // Rows from the raw sums, as the single-state entry points finish them:
+  // corr (re, im), mag re, norm sqrt of each sum, energy re.
+  int finish(const std::vector<double>& raw) {
+    const size_t nobs = b->obs_codes.size();
+    for (size_t k = 0; k < m.size(); ++k)
+      for (size_t row = 0; row < m[k].row; ++row) {
+        double* out = m[k].values + row * b->width;
+        const double* r = raw.data() + (k * points + row) * nobs * kRawPerObs;
+        size_t at = 0;
+        for (size_t o = 0; o < nobs; ++o, r += kRawPerObs) {
+          const int code = b->obs_codes[o];
+          if (code <= 2) {
+            out[at++] = r[0];
+            out[at++] = r[1];
+          } else if (code == 6) {
+            out[at++] = std::sqrt(r[0]);
+            out[at++] = std::sqrt(r[2]);
+          } else {
+            out[at++] = r[0];
+          }
+        }
+      }
     return SPX_OK;
   }
 
@@ -156,7 +189,7 @@ struct Walker {
           SPX_TRY(spx_rotate(x.ref, op.axis[0], op.axis[1], op.axis[2], op.angle, t, nt));
         }
       } else if (op.kind == OP_MEASURE) {
-        for (Member& x : m) SPX_TRY(measure(x));
+        for (size_t k = 0; k < m.size(); ++k) SPX_TRY(measure(k));
       } else if (op.kind == OP_REPEAT) {
         for (size_t k = 0; k < op.count; ++k) SPX_TRY(run(i + 1, op.match));
         i = op.match;
@@ -182,9 +215,20 @@ struct WorkerObjects {
   spx_state* state[kGroup] = {};
   spx_state* ref[kGroup] = {};
   spx_prop* prop = nullptr;
-  std::vector<double> psi;   // host staging of an initial state
+  double* psi_pin[kGroup] = {};   // pinned staging of the members' initial states
+  double* d_raw = nullptr;        // the group's raw measurement sums
+  size_t raw_cap = 0;             // doubles
+  std::vector<double> raw;
 
   void release() {
+    if (c) (void)hipStreamSynchronize(c->stream);
+    for (double*& p : psi_pin) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+    }
+    if (d_raw) (void)hipFree(d_raw);
+    d_raw = nullptr;
+    raw_cap = 0;
     spx_prop_destroy(prop);
     prop = nullptr;
     for (size_t k = 0; k < kGroup; ++k) {
@@ -226,7 +270,21 @@ struct WorkerObjects {
     // the workers' own streams already overlap their groups; a second stream
     // per worker measured slower (L = 14, 16 realisations: 559 -> 448 /s)
     SPX_TRY(spx_prop_set_streams(prop, 1));
+    for (double*& p : psi_pin)
+      SPX_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), (size_t(2) << nspins) * sizeof(double),
+                            hipHostMallocDefault));
     L = nspins;
+    return SPX_OK;
+  }
+
+  int reserve_raw(size_t doubles) {
+    if (doubles <= raw_cap) return SPX_OK;
+    if (c) SPX_HIP(hipStreamSynchronize(c->stream));
+    if (d_raw) SPX_HIP(hipFree(d_raw));
+    d_raw = nullptr;
+    raw_cap = 0;
+    SPX_HIP(hipMalloc(&d_raw, doubles * sizeof(double)));
+    raw_cap = doubles;
     return SPX_OK;
   }
 };
@@ -238,6 +296,8 @@ template <typename Rows>
 static int run_group(spx_batch* b, WorkerObjects& o, const size_t* idx, size_t n, size_t timed, double* times,
                      Rows values_of) {
   SPX_TRY(o.prepare(b->items[idx[0]].nspins, b->epsilon));
+  const size_t points = program_measurements(b->program, 0, b->program.ops.size());
+  SPX_TRY(o.reserve_raw(std::max<size_t>(1, n * points * b->obs_codes.size() * kRawPerObs)));
   std::vector<Member> m;
   for (size_t k = 0; k < n; ++k) {
     const Realisation& r = b->items[idx[k]];
@@ -245,15 +305,21 @@ static int run_group(spx_batch* b, WorkerObjects& o, const size_t* idx, size_t n
       SPX_TRY(spx_ham_set_field(o.h[k], s, r.fields[3 * s], r.fields[3 * s + 1], r.fields[3 * s + 2]));
     for (size_t p = 0; p < r.lo.size(); ++p)
       SPX_TRY(spx_ham_set_coupling(o.h[k], r.lo[p], r.hi[p], r.jx[p], r.jy[p], r.jz[p]));
-    o.psi.resize(static_cast<size_t>(2) << r.nspins);
-    SPX_TRY(r.initial_state(o.psi.data()));
-    SPX_TRY(spx_state_upload(o.state[k], o.psi.data()));
+    // the pinned staging of member k was last read by the previous group's
+    // upload, which the end of that group waited for
+    SPX_TRY(r.initial_state(o.psi_pin[k]));
+    SPX_HIP(hipMemcpyAsync(o.state[k]->d, o.psi_pin[k], o.state[k]->n * sizeof(double2), hipMemcpyHostToDevice,
+                           o.c->stream));
     SPX_TRY(spx_apply_total_spin(b->reference_axis, o.state[k], o.ref[k]));
     m.push_back(Member{&r, o.h[k], o.state[k], o.ref[k], idx[k] == timed ? times : nullptr, values_of(idx[k])});
   }
-  Walker w{b, m, o.prop};
+  Walker w{b, m, o.prop, o.d_raw, points};
   SPX_TRY(w.run(0, b->program.ops.size()));
-  return spx_ctx_synchronize(o.c);
+  const size_t nraw = n * points * b->obs_codes.size() * kRawPerObs;
+  o.raw.resize(std::max<size_t>(1, nraw));
+  if (nraw) SPX_HIP(hipMemcpyAsync(o.raw.data(), o.d_raw, nraw * sizeof(double), hipMemcpyDeviceToHost, o.c->stream));
+  SPX_TRY(spx_ctx_synchronize(o.c));
+  return w.finish(o.raw);
 }
 
 // Realisations in lockstep groups of up to kGroup of one size (by size, in

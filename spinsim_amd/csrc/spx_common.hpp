@@ -248,6 +248,12 @@ int reduce_corr(spx_ctx* c, int axis, const double2* state, const double2* ref, 
                 double* re, double* im);
 int reduce_mag(spx_ctx* c, int axis, uint32_t spin, const double2* state, size_t L, double* out);
 int reduce_energy(spx_ham* h, const double2* state, double* out);
+int reduce_norm2_dev(spx_ctx* c, const double2* x, size_t n, double* dst);
+int reduce_corr_dev(spx_ctx* c, int axis, const double2* state, const double2* ref, size_t L, double* dst);
+int reduce_mag_dev(spx_ctx* c, int axis, uint32_t spin, const double2* state, size_t L, double* dst);
+int reduce_energy_dev(spx_ham* h, const double2* state, double* dst);
+int launch_rotate_targets(spx_ctx* c, double2* s, size_t L, const uint32_t* targets, size_t nt,
+                          const double* u8);
 
 // tunables: the Hamiltonian's schedule, else the environment (for sweeps),
 // else the defaults

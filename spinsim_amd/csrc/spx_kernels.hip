@@ -811,6 +811,67 @@ __global__ __launch_bounds__(kBlock) void k_rotate(double2* __restrict__ s, size
                       (hlr * a.y + hli * a.x) + (hhr * b.y + hhi * b.x));
 }
 
+// The rotation of one pair (l, h) of a spin: the arithmetic of k_rotate.
+__device__ __forceinline__ void rotate_pair(double2& a, double2& b, const Rot& u) {
+#pragma clang fp contract(off)
+  const double llr = u.m[0], lli = u.m[1], lhr = u.m[2], lhi = u.m[3];
+  const double hlr = u.m[4], hli = u.m[5], hhr = u.m[6], hhi = u.m[7];
+  const double2 x = a, y = b;
+  a = make_double2((llr * x.x - lli * x.y) + (lhr * y.x - lhi * y.y),
+                   (llr * x.y + lli * x.x) + (lhr * y.y + lhi * y.x));
+  b = make_double2((hlr * x.x - hli * x.y) + (hhr * y.x - hhi * y.y),
+                   (hlr * x.y + hli * x.x) + (hhr * y.y + hhi * y.x));
+}
+
+// The same rotation on every target spin below tb (mask: target bits), in
+// ascending order, on a tile of 2^tb contiguous amplitudes in LDS: one pass
+// over the state instead of one per target. Each pair sees exactly the
+// arithmetic and the order of per-target k_rotate launches.
+__global__ __launch_bounds__(256) void k_rotate_lo(double2* __restrict__ s, uint32_t tb, uint32_t mask, Rot u) {
+  __shared__ double2 t[1024];
+  const uint32_t T = 1u << tb;
+  const size_t base = static_cast<size_t>(blockIdx.x) << tb;
+  for (uint32_t i = threadIdx.x; i < T; i += 256) t[i] = s[base + i];
+  __syncthreads();
+  for (uint32_t b = 0; b < tb; ++b) {
+    if (!((mask >> b) & 1u)) continue;   // uniform
+    for (uint32_t q = threadIdx.x; q < T / 2; q += 256) {
+      const uint32_t l = static_cast<uint32_t>(insert_zero(q, b)), h = l | (1u << b);
+      rotate_pair(t[l], t[h], u);
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < T; i += 256) s[base + i] = t[i];
+}
+
+// The target spins from tb up (mask: target bits above tb, shifted down by
+// tb; g = L - tb of them at most, 2^g <= 1024 rows), in ascending order: a
+// workgroup takes 1024 / 2^g consecutive low indices (columns) and all 2^g
+// rows through LDS.
+__global__ __launch_bounds__(256) void k_rotate_hi(double2* __restrict__ s, uint32_t tb, uint32_t g, uint32_t mask,
+                                                   Rot u) {
+  __shared__ double2 t[1024];
+  const uint32_t R = 1u << g, C = 1024u >> g;
+  const size_t c0 = static_cast<size_t>(blockIdx.x) * C;
+  for (uint32_t i = threadIdx.x; i < R * C; i += 256) {
+    const uint32_t r = i / C, j = i % C;
+    t[i] = s[(static_cast<size_t>(r) << tb) + c0 + j];
+  }
+  __syncthreads();
+  for (uint32_t b = 0; b < g; ++b) {
+    if (!((mask >> b) & 1u)) continue;   // uniform
+    for (uint32_t q = threadIdx.x; q < R * C / 2; q += 256) {
+      const uint32_t rl = static_cast<uint32_t>(insert_zero(q / C, b)), j = q % C;
+      rotate_pair(t[rl * C + j], t[(rl | (1u << b)) * C + j], u);
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < R * C; i += 256) {
+    const uint32_t r = i / C, j = i % C;
+    s[(static_cast<size_t>(r) << tb) + c0 + j] = t[i];
+  }
+}
+
 struct Gate4 {
   double m[32];
 };
@@ -940,15 +1001,23 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce2(const double2* parti
 
 inline unsigned blocks_for(size_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
+// The fixed-order reduction of op over n terms into dst[0..1] on the
+// device (stream-ordered, no host wait).
 template <class Op>
-int reduce(spx_ctx* c, Op op, size_t n, double* re, double* im) {
+int reduce_dev(spx_ctx* c, Op op, size_t n, double* dst) {
   size_t nb = (n + kReduceThreads - 1) / kReduceThreads;
   if (nb > static_cast<size_t>(kReduceBlocks)) nb = kReduceBlocks;
   if (nb == 0) nb = 1;
   double2* partials = reinterpret_cast<double2*>(c->d_partials);
   k_reduce1<Op><<<static_cast<unsigned>(nb), kReduceThreads, 0, c->stream>>>(op, n, partials);
-  k_reduce2<<<1, kReduceThreads, 0, c->stream>>>(partials, static_cast<int>(nb), c->d_result);
+  k_reduce2<<<1, kReduceThreads, 0, c->stream>>>(partials, static_cast<int>(nb), dst);
   SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+template <class Op>
+int reduce(spx_ctx* c, Op op, size_t n, double* re, double* im) {
+  SPX_TRY(reduce_dev(c, op, n, c->d_result));
   SPX_HIP(hipMemcpyAsync(c->h_result, c->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
                          c->stream));
   SPX_HIP(hipStreamSynchronize(c->stream));
@@ -1155,6 +1224,12 @@ static int prepare_terms(spx_ham* h, double scale) {
   const size_t L = h->nspins;
   if (h->terms_valid && h->terms_scale == scale) return SPX_OK;
   SPX_TRY(prepare_dense(h, scale));
+  if (h->wht_ok) {
+    // the Hadamard path reads its factor tables only: no flip table
+    h->terms_scale = scale;
+    h->terms_valid = true;
+    return SPX_OK;
+  }
 
   const int tb = tile_bits_for(h);
   const int g = static_cast<int>(L) - tb;
@@ -1621,6 +1696,39 @@ int launch_rotate(spx_ctx* c, double2* s, size_t n, uint32_t spin, const double*
   return SPX_OK;
 }
 
+// SpinRotation::apply over a target list (src/core/Gate.cpp:60-85): targets
+// in ascending order without repeats, L <= 20, run as one LDS pass over the
+// low 10 bits and one over the rest; any other list, one launch per target
+// in list order.
+int launch_rotate_targets(spx_ctx* c, double2* s, size_t L, const uint32_t* targets, size_t nt,
+                          const double* u8) {
+  bool ascending = L <= 20;
+  for (size_t i = 1; i < nt && ascending; ++i) ascending = targets[i] > targets[i - 1];
+  if (!ascending || nt < 2) {
+    for (size_t i = 0; i < nt; ++i) SPX_TRY(launch_rotate(c, s, size_t(1) << L, targets[i], u8));
+    return SPX_OK;
+  }
+  Rot u;
+  for (int i = 0; i < 8; ++i) u.m[i] = u8[i];
+  const uint32_t tb = L < 10 ? static_cast<uint32_t>(L) : 10u;
+  uint32_t lo = 0, hi = 0;
+  for (size_t i = 0; i < nt; ++i) {
+    if (targets[i] < tb) lo |= 1u << targets[i];
+    else hi |= 1u << (targets[i] - tb);
+  }
+  if (lo) {
+    k_rotate_lo<<<static_cast<unsigned>((size_t(1) << L) >> tb), 256, 0, c->stream>>>(s, tb, lo, u);
+    SPX_HIP(hipGetLastError());
+  }
+  if (hi) {
+    const uint32_t g = static_cast<uint32_t>(L) - tb;
+    const uint32_t C = 1024u >> g;
+    k_rotate_hi<<<static_cast<unsigned>((size_t(1) << tb) / C), 256, 0, c->stream>>>(s, tb, g, hi, u);
+    SPX_HIP(hipGetLastError());
+  }
+  return SPX_OK;
+}
+
 int launch_two_spin(spx_ctx* c, double2* s, size_t n, uint32_t lo, uint32_t hi, const double* u32) {
   Gate4 u;
   for (int i = 0; i < 32; ++i) u.m[i] = u32[i];
@@ -1649,6 +1757,21 @@ int reduce_mag(spx_ctx* c, int axis, uint32_t spin, const double2* state, size_t
   return reduce(c, OpMag{axis, spin, state}, static_cast<size_t>(1) << L, out, nullptr);
 }
 
+// The same sums left on the device at dst (two doubles each: re, im; the
+// norm's sum of squares, not its square root), for callers that read many
+// at once.
+int reduce_norm2_dev(spx_ctx* c, const double2* x, size_t n, double* dst) {
+  return reduce_dev(c, OpNorm{x}, n, dst);
+}
+
+int reduce_corr_dev(spx_ctx* c, int axis, const double2* state, const double2* ref, size_t L, double* dst) {
+  return reduce_dev(c, OpCorr{axis, static_cast<uint32_t>(L), state, ref}, static_cast<size_t>(1) << L, dst);
+}
+
+int reduce_mag_dev(spx_ctx* c, int axis, uint32_t spin, const double2* state, size_t L, double* dst) {
+  return reduce_dev(c, OpMag{axis, spin, state}, static_cast<size_t>(1) << L, dst);
+}
+
 // Energy::measure (src/core/Observable.cpp:112-120): H psi into the
 // Hamiltonian's scratch vector, then Re <H psi|psi> by the fixed-order
 // reduction.
@@ -1657,6 +1780,13 @@ int reduce_energy(spx_ham* h, const double2* state, double* out) {
   if (!h->d_scratch) SPX_HIP(hipMalloc(&h->d_scratch, n * sizeof(double2)));
   SPX_TRY(launch_apply(h, 1.0, state, h->d_scratch, false));
   return reduce(h->ctx, OpInner{state, h->d_scratch}, n, out, nullptr);
+}
+
+int reduce_energy_dev(spx_ham* h, const double2* state, double* dst) {
+  const size_t n = static_cast<size_t>(1) << h->nspins;
+  if (!h->d_scratch) SPX_HIP(hipMalloc(&h->d_scratch, n * sizeof(double2)));
+  SPX_TRY(launch_apply(h, 1.0, state, h->d_scratch, false));
+  return reduce_dev(h->ctx, OpInner{state, h->d_scratch}, n, dst);
 }
 
 }  // namespace spx

@@ -165,6 +165,26 @@ def test_sigma_rotation_and_observables_match_oracle(ctx, L):
     assert abs(e - O.lib().so_energy(ho.h, O._d(psi), O._d(np.zeros_like(psi)))) < 1e-12 * max(1, abs(e))
 
 
+@pytest.mark.parametrize("L", [5, 10, 13, 16, 18])
+def test_rotation_of_many_spins_matches_oracle(ctx, L):
+    """Rotations over ascending target lists (every spin, as a pulse step
+    gives them, and a sorted subset) run as two LDS passes instead of one
+    launch per target; each pair sees the per-target arithmetic and order,
+    so the result is the oracle's sequential rotation to rounding."""
+    banner(f"rotation of many spins L={L}")
+    psi = O.random_state(L, 60 + L)
+    sp = spx.StateVector(ctx, L, psi)
+    for axis, angle, targets in (((0, 1, 0), math.pi, list(range(L))),
+                                 ((1, 0, 0), math.pi / 2, list(range(L))),
+                                 ((0.3, -0.5, 0.8), 1.7, sorted({0, 2, L // 2, L - 2, L - 1}))):
+        sp.upload(psi)
+        spx.rotate(sp, axis, angle, targets)
+        want = psi.copy()
+        t = np.array(targets, np.uint32)
+        O.lib().so_rotate(O._d(want), L, *axis, angle, O._u32(t), t.size)
+        assert rel_err(sp.download(), want) < 1e-14, (axis, targets)
+
+
 def test_two_spin_gate_matches_oracle(ctx):
     L = 6
     g = np.random.default_rng(5)
