@@ -45,38 +45,130 @@ struct GemmArgs {
   int accumulate;
 };
 
-// C = A B (+ C) (+ bias), optional ReLU; A is M x K, B is K x N, bf16.
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m0 = blockIdx.y * 32 + (wave >> 1) * 16;
-  const int n0 = blockIdx.x * 32 + (wave & 1) * 16;
-  const int r = m0 + (lane & 15);
-  const int cidx = n0 + (lane & 15);
-  const int kq = (lane >> 4) * 4;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    bf16x4 av, bv;
+// C = A B (+ C) (+ bias), optional ReLU; A is M x K, B is K x N, bf16,
+// FP32 accumulation. A workgroup of four wavefronts owns a 64 x 64 tile of
+// C (each wavefront 32 x 32, four 16 x 16 MFMA accumulators) and walks its
+// K range 32 at a time through LDS: A as [m][k], B transposed as [n][k], so
+// every MFMA operand fragment is four consecutive bf16 of one LDS row. An
+// operand whose K (or M / N) index is contiguous in memory is loaded eight
+// elements (16 bytes) per thread; other strides element by element. When
+// the output has few tiles and K is long (the weight gradients reduce over
+// every row of the batch), K is split across workgroups into partial tiles
+// that k_gemm_reduce sums in a fixed order, so results do not depend on
+// scheduling.
+constexpr int kBM = 64, kBN = 64, kBK = 32, kLP = kBK + 8;   // LDS row pitch (bf16)
+
+__device__ __forceinline__ void gemm_store(const GemmArgs& g, int row, int col, float v) {
+  if (g.bias) v += g.bias[col];
+  float* dst = g.c + static_cast<long>(row) * g.scm + static_cast<long>(col) * g.scn;
+  if (g.accumulate) v += *dst;
+  if (g.preact) g.preact[static_cast<long>(row) * g.scm + static_cast<long>(col) * g.scn] = v;
+  if (g.relu) v = v > 0.f ? v : 0.f;
+  *dst = v;
+}
+
+// One 64 x 32 operand tile into LDS as t[r][k] (r: row of A or column of B),
+// element (r, k) at src[(r0 + r) * sr + (k0 + k) * sk], zero outside
+// [0, R) x [k0, k1).
+__device__ __forceinline__ void load_tile(uint16_t (*t)[kLP], const uint16_t* __restrict__ src, long sr, long sk,
+                                          int r0, int R, int k0, int k1) {
+  const int tid = threadIdx.x;
+  if (sk == 1) {
+    // K contiguous: eight consecutive k of one row per thread
+    const int r = tid >> 2, kc = (tid & 3) * 8;
+    const int gr = r0 + r, gk = k0 + kc;
+    const uint16_t* p = src + static_cast<long>(gr) * sr + gk;
+    if (gr < R && gk + 8 <= k1 && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+      const uint4 v = *reinterpret_cast<const uint4*>(p);
+      *reinterpret_cast<uint4*>(&t[r][kc]) = v;
+    } else {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int k = k0 + kq + i;
-      av[i] = (r < g.M && k < g.K) ? static_cast<short>(g.a[r * g.sam + k * g.sak]) : short(0);
-      bv[i] = (cidx < g.N && k < g.K) ? static_cast<short>(g.b[k * g.sbk + cidx * g.sbn]) : short(0);
+      for (int e = 0; e < 8; ++e) t[r][kc + e] = (gr < R && gk + e < k1) ? p[e] : uint16_t(0);
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-  }
-  if (cidx >= g.N) return;
+  } else if (sr == 1) {
+    // rows contiguous: eight consecutive rows of one k per thread
+    const int k = tid >> 3, rc = (tid & 7) * 8;
+    const int gk = k0 + k, gr = r0 + rc;
+    const uint16_t* p = src + static_cast<long>(gk) * sk + gr;
+    if (gk < k1 && gr + 8 <= R && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+      const uint4 v = *reinterpret_cast<const uint4*>(p);
+      const uint16_t* e8 = reinterpret_cast<const uint16_t*>(&v);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int row = m0 + kq + i;
-    if (row >= g.M) continue;
-    float v = acc[i];
-    if (g.bias) v += g.bias[cidx];
-    float* dst = g.c + row * g.scm + cidx * g.scn;
-    if (g.accumulate) v += *dst;
-    if (g.preact) g.preact[row * g.scm + cidx * g.scn] = v;
-    if (g.relu) v = v > 0.f ? v : 0.f;
-    *dst = v;
+      for (int e = 0; e < 8; ++e) t[rc + e][k] = e8[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) t[rc + e][k] = (gk < k1 && gr + e < R) ? p[e] : uint16_t(0);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int i = tid * 8 + e, r = i / kBK, k = i % kBK;
+      const int gr = r0 + r, gk = k0 + k;
+      t[r][k] = (gr < R && gk < k1) ? src[static_cast<long>(gr) * sr + static_cast<long>(gk) * sk] : uint16_t(0);
+    }
   }
+}
+
+// part: null for a single K range (the epilogue is applied here), else
+// [gridDim.z][M][N] partial sums of K ranges of kchunk each.
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g, float* __restrict__ part, int kchunk) {
+  __shared__ __align__(16) uint16_t As[kBM][kLP];
+  __shared__ __align__(16) uint16_t Bs[kBN][kLP];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int bm = blockIdx.y * kBM, bn = blockIdx.x * kBN;
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(g.K, kbeg + kchunk);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int fr = lane & 15, fk = (lane >> 4) * 4;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+    load_tile(As, g.a, g.sam, g.sak, bm, g.M, k0, kend);
+    load_tile(Bs, g.b, g.sbn, g.sbk, bn, g.N, k0, kend);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 16) {
+      bf16x4 af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x4*>(&As[wm + i * 16 + fr][kk + fk]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = *reinterpret_cast<const bf16x4*>(&Bs[wn + j * 16 + fr][kk + fk]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = bn + wn + j * 16 + fr;
+      if (col >= g.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = bm + wm + i * 16 + fk + r;
+        if (row >= g.M) continue;
+        if (part)
+          part[(static_cast<long>(blockIdx.z) * g.M + row) * g.N + col] = acc[i][j][r];
+        else
+          gemm_store(g, row, col, acc[i][j][r]);
+      }
+    }
+}
+
+// Sum of the split-K partials in K-range order, then the epilogue.
+__global__ __launch_bounds__(256) void k_gemm_reduce(GemmArgs g, const float* __restrict__ part, int splits) {
+  const long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  const long mn = static_cast<long>(g.M) * g.N;
+  if (i >= mn) return;
+  float v = 0.f;
+  for (int s = 0; s < splits; ++s) v += part[s * mn + i];
+  gemm_store(g, static_cast<int>(i / g.N), static_cast<int>(i % g.N), v);
 }
 
 __global__ void k_cast(const float* __restrict__ x, uint16_t* __restrict__ y, long n) {
@@ -115,26 +207,30 @@ __global__ __launch_bounds__(32) void k_graph_inputs(const float* __restrict__ J
   for (int j = 0; j < L; ++j) Ar[j] = Jr[j] * inv_sj / sqrtf((1.f + d) * (1.f + deg[j]));
 }
 
-// out[g,i,:] (+)= sum_j A[g,i,j] x[g,j,:]  (fp32)
-__global__ void k_mix(const float* __restrict__ J, const float* __restrict__ x, float* __restrict__ out,
-                      int L, int accumulate) {
+// [x | sum_j A_ij x_j] in bf16, rows of 2H: the message sum in FP32, rounded
+// once with the node's own features.
+__global__ void k_mix_cat(const float* __restrict__ J, const float* __restrict__ x, uint16_t* __restrict__ cat,
+                          int L) {
   const int g = blockIdx.x, i = blockIdx.y, c = threadIdx.x;
   const float* Jr = J + (static_cast<long>(g) * L + i) * L;
   const float* xg = x + static_cast<long>(g) * L * H;
   float s = 0.f;
   for (int j = 0; j < L; ++j) s += Jr[j] * xg[j * H + c];
-  float* o = out + (static_cast<long>(g) * L + i) * H + c;
-  *o = accumulate ? *o + s : s;
+  uint16_t* row = cat + (static_cast<long>(g) * L + i) * 2 * H;
+  row[c] = to_bf16(xg[i * H + c]);
+  row[H + c] = to_bf16(s);
 }
 
-// [x | m] in bf16, rows of 2H
-__global__ void k_concat(const float* __restrict__ x, const float* __restrict__ m, uint16_t* __restrict__ cat,
-                         long rows) {
-  const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= rows * 2 * H) return;
-  const long row = t / (2 * H);
-  const int col = static_cast<int>(t % (2 * H));
-  cat[t] = to_bf16(col < H ? x[row * H + col] : m[row * H + col - H]);
+// dx = d[x | m][:, :H] + A^T d[x | m][:, H:] (A symmetric): the backward of
+// k_mix_cat, reading both halves of the product's rows in place.
+__global__ void k_mix_bwd(const float* __restrict__ J, const float* __restrict__ dcat, float* __restrict__ dx,
+                          int L) {
+  const int g = blockIdx.x, i = blockIdx.y, c = threadIdx.x;
+  const float* Jr = J + (static_cast<long>(g) * L + i) * L;
+  const float* dg = dcat + static_cast<long>(g) * L * 2 * H;
+  float s = 0.f;
+  for (int j = 0; j < L; ++j) s += Jr[j] * dg[j * 2 * H + H + c];
+  dx[(static_cast<long>(g) * L + i) * H + c] = dg[i * 2 * H + c] + s;
 }
 
 __global__ void k_pool(const float* __restrict__ x, float* __restrict__ p, int L) {
@@ -151,18 +247,24 @@ __global__ void k_pool_bwd(const float* __restrict__ dp, float* __restrict__ dx,
 
 // d (MSE) / dy and the loss terms (summed on the host in a fixed order)
 __global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, float* __restrict__ dy,
-                      float* __restrict__ sq, long n, float scale) {
+                      uint16_t* __restrict__ dybf, float* __restrict__ sq, long n, float scale) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float d = y[i] - t[i];
-  dy[i] = 2.f * d * scale;
+  const float v = 2.f * d * scale;
+  dy[i] = v;
+  dybf[i] = to_bf16(v);
   sq[i] = d * d;
 }
 
+// dz = dx where the pre-activation is positive, in FP32 and in bf16
 __global__ void k_relu_bwd(const float* __restrict__ dx, const float* __restrict__ z, float* __restrict__ dz,
-                           long n) {
+                           uint16_t* __restrict__ dzbf, long n) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) dz[i] = z[i] > 0.f ? dx[i] : 0.f;
+  if (i >= n) return;
+  const float v = z[i] > 0.f ? dx[i] : 0.f;
+  dz[i] = v;
+  dzbf[i] = to_bf16(v);
 }
 
 // Column sums of a rows x cols matrix: a block owns 64 columns; its 16 row
@@ -249,11 +351,11 @@ struct spx_gnn {
   long cap_rows = 0, cap_graphs = 0, cap_L = 0;
   float *Jin = nullptr, *hin = nullptr, *J = nullptr;  // raw couplings/fields (FP32), edge weights
   uint16_t* feat = nullptr;
-  float *z[NLAYER + 1] = {}, *x[NLAYER + 1] = {}, *mix[NLAYER + 1] = {};
+  float *z[NLAYER + 1] = {}, *x[NLAYER + 1] = {};
   uint16_t* cat[NLAYER + 1] = {};
   float *pool = nullptr, *y = nullptr, *tgt = nullptr, *dy = nullptr, *sq = nullptr, *dpool = nullptr;
   uint16_t *poolbf = nullptr, *dybf = nullptr, *dzbf = nullptr;
-  float *dx = nullptr, *dz = nullptr, *dcat = nullptr, *dm = nullptr;
+  float *dx = nullptr, *dz = nullptr, *dcat = nullptr;
   // resident training set
   float *set_J = nullptr, *set_h = nullptr, *set_t = nullptr;
   uint32_t* set_idx = nullptr;
@@ -263,6 +365,9 @@ struct spx_gnn {
   size_t pin_graphs = 0, pin_L = 0;
   float *pin_J[2] = {}, *pin_h[2] = {}, *pin_y[2] = {};
   hipEvent_t ev[2] = {};
+  // split-K partial tiles of the products
+  float* gemm_part = nullptr;
+  size_t gemm_part_cap = 0;
 };
 
 namespace {
@@ -275,11 +380,38 @@ size_t wsize(const spx_gnn* g, int l) {
 
 size_t bsize(const spx_gnn* g, int l) { return l <= NLAYER ? H : g->P; }
 
-int gemm(spx_ctx* c, const uint16_t* a, long sam, long sak, const uint16_t* b, long sbk, long sbn, float* out,
+int gemm(spx_gnn* net, const uint16_t* a, long sam, long sak, const uint16_t* b, long sbk, long sbn, float* out,
          long scm, long scn, int M, int N, int K, const float* bias, float* preact, int relu, int acc) {
+  spx_ctx* c = net->ctx;
   GemmArgs g{a, sam, sak, b, sbk, sbn, out, scm, scn, M, N, K, bias, preact, relu, acc};
-  const dim3 grid((N + 31) / 32, (M + 31) / 32);
-  k_gemm<<<grid, 256, 0, c->stream>>>(g);
+  const int tiles = ((N + kBN - 1) / kBN) * ((M + kBM - 1) / kBM);
+  // Split K when the output alone would leave most CUs idle and K is long:
+  // that is the weight gradients, whose K is the batch's rows. The forward
+  // products (K = 16, 128, 256) never split, so a prediction does not
+  // depend on the batch it is computed in.
+  int splits = 1;
+  if (tiles < 128 && K >= 512) splits = std::min({32, K / (2 * kBK), std::max(1, 256 / tiles)});
+  int kchunk = (K + splits - 1) / splits;
+  kchunk = (kchunk + kBK - 1) / kBK * kBK;
+  splits = (K + kchunk - 1) / kchunk;
+  if (splits <= 1) {
+    k_gemm<<<dim3(static_cast<unsigned>((N + kBN - 1) / kBN), static_cast<unsigned>((M + kBM - 1) / kBM)), 256, 0,
+             c->stream>>>(g, nullptr, K > 0 ? K : 1);
+    SPX_HIP(hipGetLastError());
+    return SPX_OK;
+  }
+  const size_t need = static_cast<size_t>(splits) * M * N;
+  if (need > net->gemm_part_cap) {
+    SPX_HIP(hipStreamSynchronize(c->stream));
+    if (net->gemm_part) SPX_HIP(hipFree(net->gemm_part));
+    net->gemm_part = nullptr;
+    net->gemm_part_cap = 0;
+    SPX_HIP(hipMalloc(&net->gemm_part, need * sizeof(float)));
+    net->gemm_part_cap = need;
+  }
+  k_gemm<<<dim3(static_cast<unsigned>((N + kBN - 1) / kBN), static_cast<unsigned>((M + kBM - 1) / kBM),
+                static_cast<unsigned>(splits)), 256, 0, c->stream>>>(g, net->gemm_part, kchunk);
+  k_gemm_reduce<<<nb(static_cast<long>(M) * N), 256, 0, c->stream>>>(g, net->gemm_part, splits);
   SPX_HIP(hipGetLastError());
   return SPX_OK;
 }
@@ -303,7 +435,6 @@ int reserve(spx_gnn* g, long G, long L) {
   for (int l = 0; l <= NLAYER; ++l) {
     SPX_TRY(dalloc(&g->z[l], R * H));
     SPX_TRY(dalloc(&g->x[l], R * H));
-    SPX_TRY(dalloc(&g->mix[l], R * H));
     SPX_TRY(dalloc(&g->cat[l], R * 2 * H));
   }
   SPX_TRY(dalloc(&g->pool, GG * H));
@@ -318,7 +449,6 @@ int reserve(spx_gnn* g, long G, long L) {
   SPX_TRY(dalloc(&g->dz, R * H));
   SPX_TRY(dalloc(&g->dzbf, R * H));
   SPX_TRY(dalloc(&g->dcat, R * 2 * H));
-  SPX_TRY(dalloc(&g->dm, R * H));
   g->cap_rows = R;
   g->cap_graphs = GG;
   g->cap_L = JJ;
@@ -356,18 +486,17 @@ int forward(spx_gnn* g, long G, long L) {
                                                                   static_cast<float>(1.0 / g->sh));
   k_cast<<<nb(static_cast<long>(g->nparam)), 256, 0, c->stream>>>(g->p, g->pbf, static_cast<long>(g->nparam));
   // x0 = relu(feat W0^T + b0)
-  SPX_TRY(gemm(c, g->feat, F, 1, g->pbf + g->off_w[0], 1, F, g->x[0], H, 1, static_cast<int>(R), H, F,
+  SPX_TRY(gemm(g, g->feat, F, 1, g->pbf + g->off_w[0], 1, F, g->x[0], H, 1, static_cast<int>(R), H, F,
                g->p + g->off_b[0], g->z[0], 1, 0));
   for (int l = 1; l <= NLAYER; ++l) {
-    k_mix<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(
-        g->J, g->x[l - 1], g->mix[l], static_cast<int>(L), 0);
-    k_concat<<<nb(R * 2 * H), 256, 0, c->stream>>>(g->x[l - 1], g->mix[l], g->cat[l], R);
-    SPX_TRY(gemm(c, g->cat[l], 2 * H, 1, g->pbf + g->off_w[l], 1, 2 * H, g->x[l], H, 1, static_cast<int>(R),
+    k_mix_cat<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(
+        g->J, g->x[l - 1], g->cat[l], static_cast<int>(L));
+    SPX_TRY(gemm(g, g->cat[l], 2 * H, 1, g->pbf + g->off_w[l], 1, 2 * H, g->x[l], H, 1, static_cast<int>(R),
                  H, 2 * H, g->p + g->off_b[l], g->z[l], 1, 0));
   }
   k_pool<<<static_cast<unsigned>(G), H, 0, c->stream>>>(g->x[NLAYER], g->pool, static_cast<int>(L));
   k_cast<<<nb(G * H), 256, 0, c->stream>>>(g->pool, g->poolbf, G * H);
-  SPX_TRY(gemm(c, g->poolbf, H, 1, g->pbf + g->off_w[NLAYER + 1], 1, H, g->y, g->P, 1, static_cast<int>(G),
+  SPX_TRY(gemm(g, g->poolbf, H, 1, g->pbf + g->off_w[NLAYER + 1], 1, H, g->y, g->P, 1, static_cast<int>(G),
                g->P, H, g->p + g->off_b[NLAYER + 1], nullptr, 0, 0));
   SPX_HIP(hipGetLastError());
   return SPX_OK;
@@ -377,37 +506,32 @@ int backward(spx_gnn* g, long G, long L, float* loss) {
   spx_ctx* c = g->ctx;
   const long R = G * L;
   const int P = g->P;
-  k_mse<<<nb(G * P), 256, 0, c->stream>>>(g->y, g->tgt, g->dy, g->sq, G * P, 1.f / static_cast<float>(G * P));
-  k_cast<<<nb(G * P), 256, 0, c->stream>>>(g->dy, g->dybf, G * P);
+  k_mse<<<nb(G * P), 256, 0, c->stream>>>(g->y, g->tgt, g->dy, g->dybf, g->sq, G * P,
+                                          1.f / static_cast<float>(G * P));
   float* gr = g->grad;
   // readout
-  SPX_TRY(gemm(c, g->dybf, 1, P, g->poolbf, H, 1, gr + g->off_w[NLAYER + 1], H, 1, P, H, static_cast<int>(G),
+  SPX_TRY(gemm(g, g->dybf, 1, P, g->poolbf, H, 1, gr + g->off_w[NLAYER + 1], H, 1, P, H, static_cast<int>(G),
                nullptr, nullptr, 0, 0));
   k_colsum<<<nb(P, 64), 1024, 0, c->stream>>>(g->dy, gr + g->off_b[NLAYER + 1], G, P);
-  SPX_TRY(gemm(c, g->dybf, P, 1, g->pbf + g->off_w[NLAYER + 1], H, 1, g->dpool, H, 1, static_cast<int>(G), H, P,
+  SPX_TRY(gemm(g, g->dybf, P, 1, g->pbf + g->off_w[NLAYER + 1], H, 1, g->dpool, H, 1, static_cast<int>(G), H, P,
                nullptr, nullptr, 0, 0));
   k_pool_bwd<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(g->dpool, g->dx,
                                                                                            static_cast<int>(L));
   for (int l = NLAYER; l >= 0; --l) {
-    k_relu_bwd<<<nb(R * H), 256, 0, c->stream>>>(g->dx, g->z[l], g->dz, R * H);
-    k_cast<<<nb(R * H), 256, 0, c->stream>>>(g->dz, g->dzbf, R * H);
+    k_relu_bwd<<<nb(R * H), 256, 0, c->stream>>>(g->dx, g->z[l], g->dz, g->dzbf, R * H);
     k_colsum<<<nb(H, 64), 1024, 0, c->stream>>>(g->dz, gr + g->off_b[l], R, H);
     if (l == 0) {
-      SPX_TRY(gemm(c, g->dzbf, 1, H, g->feat, F, 1, gr + g->off_w[0], F, 1, H, F, static_cast<int>(R), nullptr,
+      SPX_TRY(gemm(g, g->dzbf, 1, H, g->feat, F, 1, gr + g->off_w[0], F, 1, H, F, static_cast<int>(R), nullptr,
                    nullptr, 0, 0));
       break;
     }
-    SPX_TRY(gemm(c, g->dzbf, 1, H, g->cat[l], 2 * H, 1, gr + g->off_w[l], 2 * H, 1, H, 2 * H, static_cast<int>(R),
+    SPX_TRY(gemm(g, g->dzbf, 1, H, g->cat[l], 2 * H, 1, gr + g->off_w[l], 2 * H, 1, H, 2 * H, static_cast<int>(R),
                  nullptr, nullptr, 0, 0));
     // d[x | m] = dz W_l; dx_{l-1} = first half + J^T (second half), J symmetric
-    SPX_TRY(gemm(c, g->dzbf, H, 1, g->pbf + g->off_w[l], 2 * H, 1, g->dcat, 2 * H, 1, static_cast<int>(R), 2 * H,
+    SPX_TRY(gemm(g, g->dzbf, H, 1, g->pbf + g->off_w[l], 2 * H, 1, g->dcat, 2 * H, 1, static_cast<int>(R), 2 * H,
                  H, nullptr, nullptr, 0, 0));
-    SPX_HIP(hipMemcpy2DAsync(g->dx, H * sizeof(float), g->dcat, 2 * H * sizeof(float), H * sizeof(float), R,
-                             hipMemcpyDeviceToDevice, c->stream));
-    SPX_HIP(hipMemcpy2DAsync(g->dm, H * sizeof(float), g->dcat + H, 2 * H * sizeof(float), H * sizeof(float), R,
-                             hipMemcpyDeviceToDevice, c->stream));
-    k_mix<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(g->J, g->dm, g->dx,
-                                                                                       static_cast<int>(L), 1);
+    k_mix_bwd<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(
+        g->J, g->dcat, g->dx, static_cast<int>(L));
   }
   SPX_HIP(hipGetLastError());
   if (loss) {
@@ -505,11 +629,10 @@ int spx_gnn_destroy(spx_gnn* g) {
                   static_cast<void*>(g->dy), static_cast<void*>(g->sq), static_cast<void*>(g->dpool),
                   static_cast<void*>(g->poolbf), static_cast<void*>(g->dybf), static_cast<void*>(g->dzbf),
                   static_cast<void*>(g->dx), static_cast<void*>(g->dz), static_cast<void*>(g->dcat),
-                  static_cast<void*>(g->dm)})
+                  static_cast<void*>(g->gemm_part)})
     if (q) (void)hipFree(q);
   for (int l = 0; l <= NLAYER; ++l)
-    for (void* q : {static_cast<void*>(g->z[l]), static_cast<void*>(g->x[l]), static_cast<void*>(g->mix[l]),
-                    static_cast<void*>(g->cat[l])})
+    for (void* q : {static_cast<void*>(g->z[l]), static_cast<void*>(g->x[l]), static_cast<void*>(g->cat[l])})
       if (q) (void)hipFree(q);
   for (int b = 0; b < 2; ++b) {
     for (float* q : {g->pin_J[b], g->pin_h[b], g->pin_y[b]})
