@@ -300,16 +300,40 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   p[i] -= lr * mh / (sqrtf(vh) + 1e-8f);
 }
 
-// Minibatch rows of the resident training set: graph i of the batch is
-// graph idx[i] of the set (couplings, fields, targets).
-__global__ void k_gather(const float* __restrict__ sJ, const float* __restrict__ sh, const float* __restrict__ st,
-                         const uint32_t* __restrict__ idx, int L, int P, float* __restrict__ J, float* __restrict__ h,
-                         float* __restrict__ t) {
+// Minibatch rows of the resident training set (graph i of step s's batch is
+// graph order[s batch + i] of the set: couplings, fields, targets), in the
+// replayable form of an epoch's step (one HIP graph for every step but the
+// first): the step index lives on the device, so the minibatch rows and
+// Adam's bias corrections of step s are read at launch, not baked into the
+// launch arguments.
+__global__ void k_gather_step(const float* __restrict__ sJ, const float* __restrict__ sh,
+                              const float* __restrict__ st, const uint32_t* __restrict__ order,
+                              const int* __restrict__ step, int batch, int L, int P, float* __restrict__ J,
+                              float* __restrict__ h, float* __restrict__ t) {
   const int i = blockIdx.x;
-  const long src = idx[i];
+  const long src = order[static_cast<long>(*step) * batch + i];
   for (int q = threadIdx.x; q < L * L; q += blockDim.x) J[static_cast<long>(i) * L * L + q] = sJ[src * L * L + q];
   for (int q = threadIdx.x; q < L; q += blockDim.x) h[static_cast<long>(i) * L + q] = sh[src * L + q];
   for (int q = threadIdx.x; q < P; q += blockDim.x) t[static_cast<long>(i) * P + q] = st[src * P + q];
+}
+
+// k_adam with this step's (b1^t, b2^t) from bt[2 step], bt[2 step + 1]
+__global__ void k_adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, long n, float lr, const float* __restrict__ bt,
+                            const int* __restrict__ step) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = *step;
+  const float b1t = bt[2 * s], b2t = bt[2 * s + 1];
+  const float gi = g[i];
+  m[i] = 0.9f * m[i] + 0.1f * gi;
+  v[i] = 0.999f * v[i] + 0.001f * gi * gi;
+  const float mh = m[i] / (1.f - b1t), vh = v[i] / (1.f - b2t);
+  p[i] -= lr * mh / (sqrtf(vh) + 1e-8f);
+}
+
+__global__ void k_step_next(int* step) {
+  if (threadIdx.x == 0) *step += 1;
 }
 
 // acc += sum(sq[0..n)) in a fixed order (one block, strided partial sums,
@@ -359,6 +383,9 @@ struct spx_gnn {
   // resident training set
   float *set_J = nullptr, *set_h = nullptr, *set_t = nullptr;
   uint32_t* set_idx = nullptr;
+  int* d_step = nullptr;          // step of the running epoch (graph replays read it)
+  float* d_bt = nullptr;          // per step (b1^t, b2^t)
+  size_t bt_cap = 0;
   double* loss_acc = nullptr;
   size_t set_n = 0, set_L = 0;
   // pinned double buffers of the bulk prediction
@@ -629,7 +656,7 @@ int spx_gnn_destroy(spx_gnn* g) {
                   static_cast<void*>(g->dy), static_cast<void*>(g->sq), static_cast<void*>(g->dpool),
                   static_cast<void*>(g->poolbf), static_cast<void*>(g->dybf), static_cast<void*>(g->dzbf),
                   static_cast<void*>(g->dx), static_cast<void*>(g->dz), static_cast<void*>(g->dcat),
-                  static_cast<void*>(g->gemm_part)})
+                  static_cast<void*>(g->gemm_part), static_cast<void*>(g->d_step), static_cast<void*>(g->d_bt)})
     if (q) (void)hipFree(q);
   for (int l = 0; l <= NLAYER; ++l)
     for (void* q : {static_cast<void*>(g->z[l]), static_cast<void*>(g->x[l]), static_cast<void*>(g->cat[l])})
@@ -777,18 +804,61 @@ int spx_gnn_train_epoch(spx_gnn* g, size_t batch, const uint32_t* order, float l
   SPX_HIP(hipMemcpyAsync(g->set_idx, order, g->set_n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   SPX_HIP(hipMemsetAsync(g->loss_acc, 0, sizeof(double), c->stream));
   const size_t steps = g->set_n / batch;
+  // Adam's bias corrections of every step of the epoch, as train_step
+  // computes them, and the device step counter
+  std::vector<float> bt(2 * steps);
   for (size_t st = 0; st < steps; ++st) {
-    k_gather<<<static_cast<unsigned>(G), 128, 0, c->stream>>>(g->set_J, g->set_h, g->set_t, g->set_idx + st * batch,
-                                                              static_cast<int>(L), g->P, g->Jin, g->hin, g->tgt);
+    const double t = static_cast<double>(g->step + 1 + st);
+    bt[2 * st] = static_cast<float>(std::pow(0.9, t));
+    bt[2 * st + 1] = static_cast<float>(std::pow(0.999, t));
+  }
+  if (2 * steps > g->bt_cap) {
+    SPX_HIP(hipStreamSynchronize(c->stream));
+    if (g->d_bt) SPX_HIP(hipFree(g->d_bt));
+    g->d_bt = nullptr;
+    g->bt_cap = 0;
+    SPX_HIP(hipMalloc(&g->d_bt, 2 * steps * sizeof(float)));
+    g->bt_cap = 2 * steps;
+  }
+  if (!g->d_step) SPX_HIP(hipMalloc(&g->d_step, sizeof(int)));
+  SPX_HIP(hipMemcpyAsync(g->d_bt, bt.data(), bt.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  SPX_HIP(hipMemsetAsync(g->d_step, 0, sizeof(int), c->stream));
+  auto one_step = [&]() -> int {
+    k_gather_step<<<static_cast<unsigned>(G), 128, 0, c->stream>>>(
+        g->set_J, g->set_h, g->set_t, g->set_idx, g->d_step, static_cast<int>(G), static_cast<int>(L), g->P, g->Jin,
+        g->hin, g->tgt);
     SPX_TRY(forward(g, G, L));
     SPX_TRY(backward(g, G, L, nullptr));
     k_loss_acc<<<1, 256, 0, c->stream>>>(g->sq, G * g->P, g->loss_acc);
-    ++g->step;
-    const float b1t = static_cast<float>(std::pow(0.9, static_cast<double>(g->step)));
-    const float b2t = static_cast<float>(std::pow(0.999, static_cast<double>(g->step)));
-    k_adam<<<nb(static_cast<long>(g->nparam)), 256, 0, c->stream>>>(g->p, g->grad, g->m, g->v,
-                                                                    static_cast<long>(g->nparam), lr, b1t, b2t);
+    k_adam_step<<<nb(static_cast<long>(g->nparam)), 256, 0, c->stream>>>(
+        g->p, g->grad, g->m, g->v, static_cast<long>(g->nparam), lr, g->d_bt, g->d_step);
+    k_step_next<<<1, 64, 0, c->stream>>>(g->d_step);
+    SPX_HIP(hipGetLastError());
+    return SPX_OK;
+  };
+  // the first step eagerly (it sizes every buffer), the others as replays
+  // of one captured step; a capture the runtime refuses leaves them eager
+  SPX_TRY(one_step());
+  size_t done = 1;
+  if (steps > 1 && hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    const int rc = one_step();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (rc == SPX_OK && ec == hipSuccess && graph &&
+        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+      hipError_t e = hipSuccess;
+      for (; done < steps && e == hipSuccess; ++done) e = hipGraphLaunch(exec, c->stream);
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(SPX_EDEVICE, std::string("replaying a training step: ") + hipGetErrorString(e));
+    } else {
+      if (graph) (void)hipGraphDestroy(graph);
+      (void)hipGetLastError();
+    }
   }
+  for (; done < steps; ++done) SPX_TRY(one_step());
+  g->step += static_cast<long>(steps);
   SPX_HIP(hipGetLastError());
   double acc = 0.0;
   SPX_HIP(hipMemcpyAsync(&acc, g->loss_acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
