@@ -127,7 +127,7 @@ int spx_ham_autotune_batch(spx_ham* h, double bound, int nvec, char* report, siz
  * tuning it first if absent. Schedules reorder floating-point additions only. */
 int spx_autotune_mode(int mode);
 /* Fix a schedule by hand; 0 / -1 fields keep the defaults. `dense`: -1
- * default, 0 table kernels only, 1 the all-pairs gather kernel (L = 14..18,
+ * default, 0 table kernels only, 1 the all-pairs gather kernel (L = 13..18,
  * flip-flop couplings only: no transverse field and jx = jy, else the table
  * kernels), 2 the Walsh-Hadamard path (L = 14..20, any couplings and
  * fields). */

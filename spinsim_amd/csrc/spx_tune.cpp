@@ -70,7 +70,7 @@ std::vector<Schedule> candidates(const spx_ham* h, bool flipflop) {
     s.dense = 2;
     c.push_back(s);
   }
-  if (flipflop && L >= 14 && L <= 18)
+  if (flipflop && L >= 13 && L <= 18)
     for (int tb : {10, 9})
       for (int xcd : {0, 1}) {
         Schedule s;
@@ -115,7 +115,7 @@ struct Scratch {
 // nvec states share every timed launch (the width of a batched evolve).
 int tune(spx_ham* h, double bound, int nvec, std::string* report) {
   if (!(bound > 0.0)) return fail(SPX_EDOMAIN, "spectral bound must be positive");
-  if (nvec < 1 || nvec > kMaxMany) return fail(SPX_EINVAL, "states per launch must be in [1, 8]");
+  if (nvec < 1 || nvec > kMaxMany) return fail(SPX_EINVAL, "states per launch must be in [1, 16]");
   const double scale = -2.0 / bound;
   const Key key = shape_of(h, scale, nvec);
   spx_ctx* c = h->ctx;
