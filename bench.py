@@ -125,9 +125,9 @@ def cpu_baseline(L, fields, lo, hi, jx, jy, jz, order_bytes, target_s=15.0):
 def control_variate_leg(spx, ctx):
     """Seconds to a +-1e-3 error bar on every point of the mean XY-16 echo at
     L = 16: plain batched Monte-Carlo against the control variate (the
-    surrogate on 1e6 random geometries plus the exact correction), both arms
-    run at the sample sizes the held-out variances prescribe; the one-shot
-    cost with the surrogate's training counted is reported beside it
+    surrogate on 1e6 random geometries plus the exact correction), each arm
+    run until its own standard error meets the bar; the one-shot cost with
+    the surrogate's training counted is reported beside it
     (spinsim_amd/ensembles.py)."""
     from spinsim_amd import ensembles as E
     return E.control_variate(ctx, log=lambda m: log(f"[bench] {m}")).line

@@ -171,18 +171,21 @@ class CVResult:
 
 
 def control_variate(ctx: spx.Context, L: int = 16, target: float = 1e-3, draws: int = 1_000_000,
-                    L_train: int = 12, n_train: int = 16384, n_val: int = 1024, epochs: int = 40,
+                    L_train: int = 12, n_train: int = 16384, n_val: int = 256, epochs: int = 40,
                     seed: int = SEED, log=lambda m: None) -> CVResult:
     """Seconds to a +-target error bar on every point of the mean XY-16 echo
     at L: plain batched Monte-Carlo against the control variate (the
     surrogate on `draws` random geometries plus the exact correction), both
-    arms actually run at the sample sizes the held-out variances prescribe.
+    arms actually run to the bar.
 
     The surrogate is trained on exact curves of the same sequence at the
     smaller L_train (the LDS-resident runner: seconds for 16384 curves); the
     network is size-agnostic and the short-time echo is set by local
-    geometry. beta is fitted on n_val held-out exact curves at L, which also
-    size both arms; whatever the surrogate's quality the estimate is
+    geometry. beta is fitted on n_val held-out exact curves at L (its error
+    adds a fraction of about 1 / n_val to the residual variance), whose
+    variances also give both arms their first sample sizes; each arm then
+    runs further realisations until its own standard error is at most
+    `target` on every point. Whatever the surrogate's quality the estimate is
     unbiased, and plain Monte-Carlo runs beside it as the judge."""
     P = xy16_program()
     # disjoint index ranges: training, validation, plain arm, exact side of
@@ -202,24 +205,54 @@ def control_variate(ctx: spx.Context, L: int = 16, target: float = 1e-3, draws: 
     beta = S.fit_beta(Yva, gva)
     var_y = Yva.var(axis=0, ddof=1)
     var_r = (Yva - beta * gva).var(axis=0, ddof=1)
-    var_g = beta * beta * gva.var(axis=0, ddof=1)
     se2 = target * target
-    n_plain = int(math.ceil(var_y.max() / se2))
-    n_cv = int(math.ceil(var_r.max() / (0.9 * se2)))
-    if (var_g / draws).max() > 0.1 * se2:
-        raise RuntimeError("the surrogate-only draws cannot carry their share of the error bar")
-    log(f"plain: {n_plain} exact L={L} runs")
-    Yp, t_plain = exact_curves_chunked(ctx, P, L, base_plain, n_plain, seed, chunk=4096, log=log)
-    plain = S.plain_estimate(Yp)
-    log(f"control variate: {n_cv} exact runs + {draws} surrogate draws")
-    Yc, t_exact = exact_curves(ctx, P, L, base_cv, n_cv, seed)
+
+    def more(var, share, n):
+        """Runs still needed by a per-point variance at a per-point share of se^2."""
+        return max(64, int(math.ceil(float((var / share).max()) * 1.01)) - n)
+
+    log(f"plain: from {int(math.ceil(var_y.max() / se2))} exact L={L} runs")
+    parts, t_plain, n_plain = [], 0.0, 0
+    todo = int(math.ceil(var_y.max() / se2))
+    for _ in range(4):
+        y, dt = exact_curves_chunked(ctx, P, L, base_plain + n_plain, todo, seed, chunk=4096, log=log)
+        parts.append(y)
+        t_plain += dt
+        n_plain += todo
+        Yp = np.concatenate(parts)
+        plain = S.plain_estimate(Yp)
+        if plain.stderr.max() <= target:
+            break
+        todo = more(Yp.var(axis=0, ddof=1), se2, n_plain)
+        log(f"  plain stderr {plain.stderr.max():.6f}: {todo} more")
+
     ctx.synchronize()
     t0 = time.perf_counter()
-    Jc, hc = graphs(L, base_cv, n_cv, seed)
-    gc = model.predict(Jc, hc)
     gx = model.predict_dipolar(seed, base_draws, draws, L, **DRAW)
     t_sur = time.perf_counter() - t0
-    cv = S.cv_estimate(Yc, gc, gx, beta)
+    var_g = beta * beta * gx.var(axis=0, ddof=1)
+    if (var_g / draws).max() > 0.1 * se2:
+        raise RuntimeError("the surrogate-only draws cannot carry their share of the error bar")
+    share = se2 - var_g / draws
+    log(f"control variate: from {int(math.ceil((var_r / share).max()))} exact runs + {draws} surrogate draws")
+    ys, gs, t_exact, n_cv = [], [], 0.0, 0
+    todo = int(math.ceil(float((var_r / share).max())))
+    for _ in range(4):
+        y, dt = exact_curves(ctx, P, L, base_cv + n_cv, todo, seed)
+        t_exact += dt
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        J, h = graphs(L, base_cv + n_cv, todo, seed)
+        gs.append(model.predict(J, h))
+        t_sur += time.perf_counter() - t0
+        ys.append(y)
+        n_cv += todo
+        Yc, gc = np.concatenate(ys), np.concatenate(gs)
+        cv = S.cv_estimate(Yc, gc, gx, beta)
+        if cv.stderr.max() <= target:
+            break
+        todo = more((Yc - beta * gc).var(axis=0, ddof=1), share, n_cv)
+        log(f"  control variate stderr {cv.stderr.max():.6f}: {todo} more")
     t_cv = t_exact + t_sur
     t_oneshot = t_cv + t_ytr + t_train + t_va
     z = float(np.max(np.abs(cv.mean - plain.mean) / np.sqrt(cv.stderr ** 2 + plain.stderr ** 2)))

@@ -100,15 +100,16 @@ def test_control_variate_at_l16_against_plain_monte_carlo(ctx):
     finite pulses, the surrogate (trained on L = 12 curves of the same
     sequence) on 1e6 random geometries plus the exact correction, against
     plain batched Monte-Carlo, both run to +-5e-3 on every point. The two
-    means agree within their error bars, both reach the bar, the surrogate
-    removes most of the worst point's variance, and the control variate
-    takes less device time."""
-    res = E.control_variate(ctx, L=16, target=5e-3, draws=1_000_000, n_train=8192, n_val=512, epochs=30,
+    means agree within their error bars, both reach the bar (each arm tops
+    itself up until its own standard error does), the surrogate removes most
+    of the worst point's variance, and the control variate takes less device
+    time."""
+    res = E.control_variate(ctx, L=16, target=5e-3, draws=1_000_000, n_train=8192, n_val=256, epochs=30,
                             log=banner)
     line = res.line
     banner(f"plain {line['plain']}; cv {line['control_variate']}; speedup {line['speedup']}; "
            f"VR {line['worst_point_variance_reduction']}; |z| {line['max_abs_z_cv_vs_plain']}")
     assert line["max_abs_z_cv_vs_plain"] < 4.5
-    assert res.plain.stderr.max() < 5e-3 * 1.15 and res.cv.stderr.max() < 5e-3 * 1.15
+    assert res.plain.stderr.max() <= 5e-3 and res.cv.stderr.max() <= 5e-3
     assert line["worst_point_variance_reduction"] > 4.0
     assert line["speedup"] > 2.0
