@@ -1078,12 +1078,16 @@ int dense_tile_bits(const spx_ham* h) {
 // exists. At L = 13 the all-pairs kernel replaces the table kernels, whose
 // sixteen-state launches fill only 128 workgroups there: 64-echo CPMG
 // ensembles of 128 realisations ran 9.6x faster (13.5 -> 130 per second).
+// Sixteen states per launch (lockstep ensemble groups, 64-echo CPMG),
+// realisations per second, gathers / Hadamard: L = 14, 82.7 / 61.8;
+// L = 15, 29.2 / 40.1.
 int default_dense_path(size_t L, int width, bool flipflop_only) {
   if (!flipflop_only) return hadamard_eligible(L) ? 2 : 0;
   if (L >= 19) return 2;
   if (L == 18) return width >= 2 ? 2 : 1;
   if (L == 17) return width >= 4 ? 2 : 1;
   if (L == 16) return width >= 8 ? 2 : 1;
+  if (L == 15) return width >= 16 ? 2 : 1;
   return 1;
 }
 

@@ -649,3 +649,29 @@ def test_streaming_batch_mid_size_matches_oracle_and_repeats(ctx):
         assert err < 1e-10, (i, Ls[i], err)
     _, gv2 = b.run()
     assert np.array_equal(gv, gv2)
+
+
+def test_streaming_batch_l15_group_of_eight_matches_oracle(ctx):
+    """Eight L = 15 realisations walk as one lockstep group, sixteen vectors
+    per launch, which at L = 15 takes the Hadamard path for the flip-flop
+    couplings between pulses (the gather kernel below that width): the first
+    two realisations match the oracle within 1e-10, and a second run is
+    bitwise identical."""
+    L, n = 15, 8
+    banner(f"streaming batch L={L} x {n}")
+    obs = ((O.OBS_CORR_X, 0), (O.OBS_NORM, 0))
+    b = spx.Batch(ctx, cpmg(spx.Program(), L, cycles=2), obs, epsilon=1e-7, reference_axis=0)
+    for i in range(n):
+        fields, c, psi0 = realisation_inputs(L, 20240726, 900 + i)
+        b.add(L, fields, c.lo, c.hi, c.jx, c.jy, c.jz, psi0)
+    gt, gv = b.run()
+    cfg = O.Config(L=L, program=cpmg(O.Program(), L, cycles=2), observables=obs, field_disorder=0.5,
+                   min_separation=0.5, angular=True)
+    for i in range(2):
+        t, v = O.run_realization(cfg, 900 + i)
+        assert np.allclose(gt, t, rtol=0, atol=1e-12)
+        err = float(np.max(np.abs(gv[i] - v)))
+        banner(f"L={L} realisation {i}: max |device - oracle| {err:.2e}")
+        assert err < 1e-10, (i, err)
+    _, gv2 = b.run()
+    assert np.array_equal(gv, gv2)
