@@ -181,9 +181,9 @@ int spx_prop_set_graphs(spx_prop* p, int enable);
 /* No reference counterpart: with streams = 2 (the default), a set evolve
  * (spx_prop_evolve_set, spx_prop_steps_set) on the Hadamard path, without
  * autonormalisation, uses the context's stream and a second stream of the
- * context, so that launches overlap: 4..8 states that split into two parts
+ * context, so that launches overlap: 4..16 states that split into two parts
  * of two or more states with no Hamiltonian in both run as those halves;
- * more than 8 states with no Hamiltonian in two launch groups alternate
+ * more than 16 states with no Hamiltonian in two launch groups alternate
  * their groups between the streams. The second stream is joined back before
  * return.
  * streams = 1 issues every launch on the context's stream. Results are

@@ -452,7 +452,7 @@ class Propagator:
 
     def evolve_set(self, hams: Sequence[Hamiltonian], mode: int, states: Sequence[StateVector], tau: float,
                    nsteps: int = 1, autonormalize: bool = False):
-        """State i under hams[i], each with its own expansion; up to eight
+        """State i under hams[i], each with its own expansion; up to sixteen
         states share every order launch where their Hamiltonians allow."""
         if len(hams) != len(states):
             raise InvalidArgument(EINVAL, "one Hamiltonian per state")
