@@ -36,7 +36,52 @@ int set_device(spx_ctx* c) {
   return SPX_OK;
 }
 
+int reserve_bounce(spx_ctx* c, size_t bytes) {
+  const size_t want = bytes < kBounceBytes ? bytes : kBounceBytes;
+  if (want <= c->bounce_cap) return SPX_OK;
+  SPX_HIP(hipStreamSynchronize(c->stream));
+  if (c->bounce) SPX_HIP(hipHostFree(c->bounce));
+  c->bounce = nullptr;
+  c->bounce_cap = 0;
+  SPX_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->bounce), want, hipHostMallocDefault));
+  c->bounce_cap = want;
+  return SPX_OK;
+}
+
 }  // namespace
+
+namespace spx {
+
+int copy_to_device(spx_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return SPX_OK;
+  SPX_TRY(reserve_bounce(c, bytes));
+  auto* d = static_cast<unsigned char*>(dst);
+  const auto* h = static_cast<const unsigned char*>(src);
+  for (size_t off = 0; off < bytes; off += c->bounce_cap) {
+    const size_t n = bytes - off < c->bounce_cap ? bytes - off : c->bounce_cap;
+    // the previous piece's copy has completed: the bounce buffer is free
+    std::memcpy(c->bounce, h + off, n);
+    SPX_HIP(hipMemcpyAsync(d + off, c->bounce, n, hipMemcpyHostToDevice, c->stream));
+    SPX_HIP(hipStreamSynchronize(c->stream));
+  }
+  return SPX_OK;
+}
+
+int copy_to_host(spx_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return SPX_OK;
+  SPX_TRY(reserve_bounce(c, bytes));
+  auto* h = static_cast<unsigned char*>(dst);
+  const auto* d = static_cast<const unsigned char*>(src);
+  for (size_t off = 0; off < bytes; off += c->bounce_cap) {
+    const size_t n = bytes - off < c->bounce_cap ? bytes - off : c->bounce_cap;
+    SPX_HIP(hipMemcpyAsync(c->bounce, d + off, n, hipMemcpyDeviceToHost, c->stream));
+    SPX_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(h + off, c->bounce, n);
+  }
+  return SPX_OK;
+}
+
+}  // namespace spx
 
 extern "C" {
 
@@ -92,6 +137,7 @@ int spx_ctx_destroy(spx_ctx* c) {
   (void)hipFree(c->d_partials);
   (void)hipFree(c->d_result);
   (void)hipHostFree(c->h_result);
+  if (c->bounce) (void)hipHostFree(c->bounce);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return SPX_OK;
@@ -163,16 +209,14 @@ int spx_state_nspins(const spx_state* s, size_t* nspins) {
 
 int spx_state_upload(spx_state* s, const double* host) {
   SPX_TRY(check_state(s));
-  SPX_HIP(hipMemcpyAsync(s->d, host, s->n * sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
-  SPX_HIP(hipStreamSynchronize(s->ctx->stream));
-  return SPX_OK;
+  if (!host) return fail(SPX_EINVAL, "null host buffer");
+  return copy_to_device(s->ctx, s->d, host, s->n * sizeof(double2));
 }
 
 int spx_state_download(const spx_state* s, double* host) {
   SPX_TRY(check_state(s));
-  SPX_HIP(hipMemcpyAsync(host, s->d, s->n * sizeof(double2), hipMemcpyDeviceToHost, s->ctx->stream));
-  SPX_HIP(hipStreamSynchronize(s->ctx->stream));
-  return SPX_OK;
+  if (!host) return fail(SPX_EINVAL, "null host buffer");
+  return copy_to_host(s->ctx, host, s->d, s->n * sizeof(double2));
 }
 
 int spx_state_copy(spx_state* dst, const spx_state* src) {
@@ -193,9 +237,7 @@ int spx_state_set_basis(spx_state* s, size_t index) {
   if (index >= s->n) return fail(SPX_ERANGE, "basis state index outside the 2^L state space");
   SPX_TRY(launch_fill(s->ctx, s->d, s->n));
   static const double one[2] = {1.0, 0.0};
-  SPX_HIP(hipMemcpyAsync(s->d + index, one, sizeof(double2), hipMemcpyHostToDevice, s->ctx->stream));
-  SPX_HIP(hipStreamSynchronize(s->ctx->stream));
-  return SPX_OK;
+  return copy_to_device(s->ctx, s->d + index, one, sizeof(double2));
 }
 
 int spx_state_scale(spx_state* s, double f) {
@@ -366,9 +408,7 @@ int spx_ham_diagonal(spx_ham* h, double* host) {
   SPX_TRY(set_device(h->ctx));
   SPX_TRY(ensure_diag(h));
   const size_t n = static_cast<size_t>(1) << h->nspins;
-  SPX_HIP(hipMemcpyAsync(host, h->d_diag, n * sizeof(double), hipMemcpyDeviceToHost, h->ctx->stream));
-  SPX_HIP(hipStreamSynchronize(h->ctx->stream));
-  return SPX_OK;
+  return copy_to_host(h->ctx, host, h->d_diag, n * sizeof(double));
 }
 
 int spx_ham_schedule(const spx_ham* h, char* buf, size_t len) {

@@ -317,8 +317,7 @@ static int run_group(spx_batch* b, WorkerObjects& o, const size_t* idx, size_t n
   SPX_TRY(w.run(0, b->program.ops.size()));
   const size_t nraw = n * points * b->obs_codes.size() * kRawPerObs;
   o.raw.resize(std::max<size_t>(1, nraw));
-  if (nraw) SPX_HIP(hipMemcpyAsync(o.raw.data(), o.d_raw, nraw * sizeof(double), hipMemcpyDeviceToHost, o.c->stream));
-  SPX_TRY(spx_ctx_synchronize(o.c));
+  SPX_TRY(copy_to_host(o.c, o.raw.data(), o.d_raw, nraw * sizeof(double)));
   return w.finish(o.raw);
 }
 
@@ -436,8 +435,7 @@ int upload(spx_ctx* c, T** d, const std::vector<T>& h) {
   const size_t bytes = (h.size() ? h.size() : 1) * sizeof(T);
   hipError_t e = hipMalloc(d, bytes);
   if (e != hipSuccess) return fail(SPX_ENOMEM, "hipMalloc for the batch tables failed");
-  if (!h.empty()) SPX_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
-  return SPX_OK;
+  return copy_to_device(c, *d, h.data(), h.size() * sizeof(T));
 }
 
 bool resident_ok(const spx_batch* b) {
@@ -627,14 +625,9 @@ int batch_run_resident(spx_batch* b, double* times, double* values) {
     }
   }
   std::vector<int> flags(queue0.size(), 0);
-  if (rc == SPX_OK) {
-    hipError_t e1 = hipMemcpyAsync(values, d_out, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    hipError_t e2 = hipMemcpyAsync(flags.data(), d_queue, flags.size() * sizeof(int), hipMemcpyDeviceToHost,
-                                   c->stream);
-    hipError_t e3 = hipStreamSynchronize(c->stream);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
-      rc = fail(SPX_EDEVICE, std::string("resident batch: ") + hipGetErrorString(e3));
-  }
+  // the copies are ordered after the per-size streams' joins
+  if (rc == SPX_OK) rc = copy_to_host(c, values, d_out, nout * sizeof(double));
+  if (rc == SPX_OK) rc = copy_to_host(c, flags.data(), d_queue, flags.size() * sizeof(int));
   // every launch has finished (or none was made) before the buffers go
   for (hipStream_t st : extra) {
     (void)hipStreamSynchronize(st);

@@ -81,6 +81,9 @@ struct spx_ctx {
   // the events that order it after, and join it back into, `stream`
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // pinned bounce buffer of copy_to_device / copy_to_host (grown on use)
+  unsigned char* bounce = nullptr;
+  size_t bounce_cap = 0;
 };
 
 struct spx_state {
@@ -269,6 +272,16 @@ int dense_tile_bits(const spx_ham* h);
 int default_dense_path(size_t L, int width, bool flipflop_only);
 bool ham_flipflop_only(const spx_ham* h);
 int ensure_diag(spx_ham* h);
+// Host <-> device copies of pageable host memory through the context's
+// pinned bounce buffer (at most kBounceBytes, allocated on first use), in
+// pieces, each ordered on the context's stream; both return when the copy
+// is complete. An asynchronous copy from or to pageable memory makes the
+// runtime copy on the CPU into its own staging memory; these never ask it
+// to. The context's stream is synchronised, so the caller's earlier work on
+// it has finished too.
+constexpr size_t kBounceBytes = size_t(4) << 20;
+int copy_to_device(spx_ctx* c, void* dst, const void* src, size_t bytes);
+int copy_to_host(spx_ctx* c, void* dst, const void* src, size_t bytes);
 void set_launch_width(spx_ham* h, int width);
 // the autotuner's process-wide mode and its hook in the propagator
 int autotune_before_evolve(spx_ham* h, double bound, int nvec);

@@ -497,11 +497,9 @@ int load_batch(spx_gnn* g, long G, long L, const double* J, const double* h, con
   for (long q = 0; q < G * L * L; ++q) Jf[q] = static_cast<float>(J[q]);
   for (long q = 0; q < G * L; ++q) hf[q] = static_cast<float>(h[q]);
   spx_ctx* c = g->ctx;
-  SPX_HIP(hipMemcpyAsync(g->Jin, Jf.data(), Jf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  SPX_HIP(hipMemcpyAsync(g->hin, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  if (target)
-    SPX_HIP(hipMemcpyAsync(g->tgt, target, G * g->P * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  SPX_HIP(hipStreamSynchronize(c->stream));  // pageable sources
+  SPX_TRY(copy_to_device(c, g->Jin, Jf.data(), Jf.size() * sizeof(float)));
+  SPX_TRY(copy_to_device(c, g->hin, hf.data(), hf.size() * sizeof(float)));
+  if (target) SPX_TRY(copy_to_device(c, g->tgt, target, G * g->P * sizeof(float)));
   return SPX_OK;
 }
 
@@ -563,8 +561,7 @@ int backward(spx_gnn* g, long G, long L, float* loss) {
   SPX_HIP(hipGetLastError());
   if (loss) {
     std::vector<float> sq(G * P);
-    SPX_HIP(hipMemcpyAsync(sq.data(), g->sq, sq.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    SPX_HIP(hipStreamSynchronize(c->stream));
+    SPX_TRY(copy_to_host(c, sq.data(), g->sq, sq.size() * sizeof(float)));
     double s = 0.0;
     for (float v : sq) s += v;
     *loss = static_cast<float>(s / static_cast<double>(G * P));
@@ -632,7 +629,7 @@ int spx_gnn_create(spx_ctx* ctx, size_t outputs, uint64_t seed, double scale_h, 
       hipMalloc(&g->m, off * sizeof(float)) != hipSuccess || hipMalloc(&g->v, off * sizeof(float)) != hipSuccess ||
       hipMalloc(&g->pbf, off * sizeof(uint16_t)) != hipSuccess)
     rc = fail(SPX_ENOMEM, "hipMalloc of network parameters failed");
-  else if (hipMemcpy(g->p, p.data(), off * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+  else if (copy_to_device(g->ctx, g->p, p.data(), off * sizeof(float)) != SPX_OK ||
            hipMemset(g->m, 0, off * sizeof(float)) != hipSuccess ||
            hipMemset(g->v, 0, off * sizeof(float)) != hipSuccess)
     rc = fail(SPX_EDEVICE, "initialising network parameters failed");
@@ -681,14 +678,12 @@ int spx_gnn_num_params(const spx_gnn* g, size_t* n) {
 
 int spx_gnn_get_params(const spx_gnn* g, float* host) {
   if (!g || !host) return fail(SPX_EINVAL, "null argument");
-  SPX_HIP(hipMemcpy(host, g->p, g->nparam * sizeof(float), hipMemcpyDeviceToHost));
-  return SPX_OK;
+  return copy_to_host(g->ctx, host, g->p, g->nparam * sizeof(float));
 }
 
 int spx_gnn_set_params(spx_gnn* g, const float* host) {
   if (!g || !host) return fail(SPX_EINVAL, "null argument");
-  SPX_HIP(hipMemcpy(g->p, host, g->nparam * sizeof(float), hipMemcpyHostToDevice));
-  return SPX_OK;
+  return copy_to_device(g->ctx, g->p, host, g->nparam * sizeof(float));
 }
 
 int spx_gnn_predict(spx_gnn* g, size_t G, size_t L, const double* J, const double* h, float* pred) {
@@ -696,9 +691,7 @@ int spx_gnn_predict(spx_gnn* g, size_t G, size_t L, const double* J, const doubl
   if (!pred) return fail(SPX_EINVAL, "null prediction buffer");
   SPX_TRY(load_batch(g, static_cast<long>(G), static_cast<long>(L), J, h, nullptr));
   SPX_TRY(forward(g, static_cast<long>(G), static_cast<long>(L)));
-  SPX_HIP(hipMemcpyAsync(pred, g->y, G * g->P * sizeof(float), hipMemcpyDeviceToHost, g->ctx->stream));
-  SPX_HIP(hipStreamSynchronize(g->ctx->stream));
-  return SPX_OK;
+  return copy_to_host(g->ctx, pred, g->y, G * g->P * sizeof(float));
 }
 
 int spx_gnn_predict_dipolar(spx_gnn* g, uint64_t seed, size_t first, size_t count, size_t nspins, double scale,
@@ -742,10 +735,7 @@ int spx_gnn_gradient(spx_gnn* g, size_t G, size_t L, const double* J, const doub
   SPX_TRY(load_batch(g, static_cast<long>(G), static_cast<long>(L), J, h, target));
   SPX_TRY(forward(g, static_cast<long>(G), static_cast<long>(L)));
   SPX_TRY(backward(g, static_cast<long>(G), static_cast<long>(L), loss));
-  if (grad) {
-    SPX_HIP(hipMemcpyAsync(grad, g->grad, g->nparam * sizeof(float), hipMemcpyDeviceToHost, g->ctx->stream));
-    SPX_HIP(hipStreamSynchronize(g->ctx->stream));
-  }
+  if (grad) SPX_TRY(copy_to_host(g->ctx, grad, g->grad, g->nparam * sizeof(float)));
   return SPX_OK;
 }
 
@@ -784,9 +774,9 @@ int spx_gnn_set_training_set(spx_gnn* g, size_t n, size_t L, const double* J, co
   SPX_HIP(hipMalloc(&g->set_t, n * g->P * sizeof(float)));
   SPX_HIP(hipMalloc(&g->set_idx, n * sizeof(uint32_t)));
   if (!g->loss_acc) SPX_HIP(hipMalloc(&g->loss_acc, sizeof(double)));
-  SPX_HIP(hipMemcpy(g->set_J, Jf.data(), Jf.size() * sizeof(float), hipMemcpyHostToDevice));
-  SPX_HIP(hipMemcpy(g->set_h, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice));
-  SPX_HIP(hipMemcpy(g->set_t, target, n * g->P * sizeof(float), hipMemcpyHostToDevice));
+  SPX_TRY(copy_to_device(g->ctx, g->set_J, Jf.data(), Jf.size() * sizeof(float)));
+  SPX_TRY(copy_to_device(g->ctx, g->set_h, hf.data(), hf.size() * sizeof(float)));
+  SPX_TRY(copy_to_device(g->ctx, g->set_t, target, n * g->P * sizeof(float)));
   g->set_n = n;
   g->set_L = L;
   return SPX_OK;
@@ -801,7 +791,7 @@ int spx_gnn_train_epoch(spx_gnn* g, size_t batch, const uint32_t* order, float l
   const long G = static_cast<long>(batch), L = static_cast<long>(g->set_L);
   SPX_TRY(reserve(g, G, L));
   spx_ctx* c = g->ctx;
-  SPX_HIP(hipMemcpyAsync(g->set_idx, order, g->set_n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  SPX_TRY(copy_to_device(c, g->set_idx, order, g->set_n * sizeof(uint32_t)));
   SPX_HIP(hipMemsetAsync(g->loss_acc, 0, sizeof(double), c->stream));
   const size_t steps = g->set_n / batch;
   // Adam's bias corrections of every step of the epoch, as train_step
@@ -821,7 +811,7 @@ int spx_gnn_train_epoch(spx_gnn* g, size_t batch, const uint32_t* order, float l
     g->bt_cap = 2 * steps;
   }
   if (!g->d_step) SPX_HIP(hipMalloc(&g->d_step, sizeof(int)));
-  SPX_HIP(hipMemcpyAsync(g->d_bt, bt.data(), bt.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  SPX_TRY(copy_to_device(c, g->d_bt, bt.data(), bt.size() * sizeof(float)));
   SPX_HIP(hipMemsetAsync(g->d_step, 0, sizeof(int), c->stream));
   auto one_step = [&]() -> int {
     k_gather_step<<<static_cast<unsigned>(G), 128, 0, c->stream>>>(
@@ -861,8 +851,7 @@ int spx_gnn_train_epoch(spx_gnn* g, size_t batch, const uint32_t* order, float l
   g->step += static_cast<long>(steps);
   SPX_HIP(hipGetLastError());
   double acc = 0.0;
-  SPX_HIP(hipMemcpyAsync(&acc, g->loss_acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  SPX_HIP(hipStreamSynchronize(c->stream));
+  SPX_TRY(copy_to_host(c, &acc, g->loss_acc, sizeof(double)));
   if (mean_loss) *mean_loss = static_cast<float>(acc / static_cast<double>(steps * batch * g->P));
   return SPX_OK;
 }

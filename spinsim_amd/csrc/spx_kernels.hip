@@ -1161,8 +1161,7 @@ static int prepare_dense(spx_ham* h, double scale) {
   }
   if (path != 1 || !flipflop_only || L < 13 || L > 18) return SPX_OK;
   if (!h->d_coef) SPX_HIP(hipMalloc(&h->d_coef, 32 * 31 / 2 * sizeof(double)));
-  SPX_HIP(hipMemcpyAsync(h->d_coef, anti.data(), np * sizeof(double), hipMemcpyHostToDevice, h->ctx->stream));
-  SPX_HIP(hipStreamSynchronize(h->ctx->stream));
+  SPX_TRY(copy_to_device(h->ctx, h->d_coef, anti.data(), np * sizeof(double)));
   h->dense_ok = true;
   return SPX_OK;
 }
@@ -1199,12 +1198,9 @@ int ensure_diag(spx_ham* h) {
       SPX_HIP(hipMalloc(&h->d_tab, tbytes));
       h->tab_cap = tbytes;
     }
-    if (!tab.empty())
-      SPX_HIP(hipMemcpyAsync(h->d_tab, tab.data(), tab.size() * sizeof(double),
-                             hipMemcpyHostToDevice, c->stream));
+    SPX_TRY(copy_to_device(c, h->d_tab, tab.data(), tab.size() * sizeof(double)));
     k_build_diag<<<blocks_for(n), kBlock, 0, c->stream>>>(h->d_diag, n, h->d_tab, nf, np);
     SPX_HIP(hipGetLastError());
-    SPX_HIP(hipStreamSynchronize(c->stream));  // `tab` is pageable and local
     h->diag_valid = true;
     h->diag_hz = h->hz;
     h->diag_jz = h->jz;
@@ -1343,9 +1339,7 @@ static int prepare_terms(spx_ham* h, double scale) {
     h->terms_cap = bytes;
   }
   if (!h->h_terms.empty()) {
-    SPX_HIP(hipMemcpyAsync(h->d_terms, h->h_terms.data(), h->h_terms.size() * sizeof(FlipTerm),
-                           hipMemcpyHostToDevice, c->stream));
-    SPX_HIP(hipStreamSynchronize(c->stream));  // pageable source, rewritten on the next change
+    SPX_TRY(copy_to_device(c, h->d_terms, h->h_terms.data(), h->h_terms.size() * sizeof(FlipTerm)));
   }
   static_assert(sizeof(FlipTerm) == 32, "flip term layout");
   return SPX_OK;

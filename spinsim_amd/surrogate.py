@@ -143,3 +143,58 @@ def variance_reduction(y, g, beta) -> np.ndarray:
     vy = y.var(axis=0, ddof=1)
     vr = (y - beta * g).var(axis=0, ddof=1)
     return np.where(vr > 0, vy / np.where(vr > 0, vr, 1.0), np.inf)
+
+
+def timing(ctx: spx.Context, L: int = 12, graphs: int = 16384, epochs: int = 40, points: int = 4,
+           draws: int = 1_000_000, seed: int = 20240726, eager: bool = False) -> dict:
+    """The surrogate's kernels alone, at the control variate's shapes:
+    training on `graphs` dipolar geometries of L spins (drawn inside libspx;
+    the targets are a fixed smooth function of each geometry, since the
+    kernels' cost does not depend on the values), then predictions on
+    `draws` further geometries. No exact runs, so the kernel trace and the
+    counters cover the network only. eager: train with one
+    spx_gnn_train_step call per minibatch (its batch uploaded from the host,
+    every launch issued by the host) instead of the epochs' replayed steps."""
+    import time
+    J, h = spx.dipolar_graphs(seed, 0, graphs, L)
+    t = np.linspace(0.25, 1.0, points)
+    Y = np.cos(np.abs(J).sum(axis=(1, 2))[:, None] * t[None, :] / L) + 0.1 * h.std(axis=1)[:, None]
+    model = Model(ctx, J, h, Y, seed=11)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    if eager:
+        T = model.normalise(Y)
+        rng = np.random.default_rng(1)
+        hist, lr = [], 2e-3
+        for _ in range(epochs):
+            order = rng.permutation(graphs)
+            losses = [model.net.train_step(J[order[s:s + 128]], h[order[s:s + 128]], T[order[s:s + 128]], lr)
+                      for s in range(0, graphs - 127, 128)]
+            hist.append(float(np.mean(losses)))
+            lr *= 0.95
+    else:
+        hist = model.train(J, h, Y, epochs=epochs, batch=128, lr=2e-3, seed=1, decay=0.95)
+    ctx.synchronize()
+    t_train = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = model.predict_dipolar(seed, 10_000_000, draws, L)
+    ctx.synchronize()
+    t_pred = time.perf_counter() - t0
+    return {"workload": f"surrogate alone: training on {graphs} L={L} geometries ({epochs} epochs, batches of 128, "
+                        f"{points} curve points{', one host-issued step per batch' if eager else ''}), then {draws} "
+                        f"predictions on further geometries",
+            "train_s": round(t_train, 3), "steps": epochs * (graphs // 128), "loss_first": round(hist[0], 4),
+            "loss_last": round(hist[-1], 4), "predict_s": round(t_pred, 3),
+            "predictions_per_s": round(draws / t_pred, 1), "prediction_mean": [round(float(v), 5) for v in g.mean(axis=0)]}
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    args = sys.argv[1:]
+    if not args or args[0] != "timing":
+        raise SystemExit("usage: python -m spinsim_amd.surrogate timing [graphs] [epochs] [draws] [eager]")
+    kw = {"eager": "eager" in args[1:]}
+    for key, value in zip(("graphs", "epochs", "draws"), [a for a in args[1:] if a != "eager"]):
+        kw[key] = int(value)
+    print(json.dumps(timing(spx.Context(0), **kw)), flush=True)
