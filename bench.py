@@ -225,7 +225,7 @@ def batched_leg(spx, ctx, count=4096, seed=20240726):
     if not np.isfinite(v).all():
         raise SystemExit("non-finite batched ensemble output")
     return {"workload": f"{count} realisations, L uniform in 8..12, XY-8 (finite pulses, Omega=50, tau=0.5) x 16, "
-                        "corr:x + norm per block, one persistent launch per L on concurrent streams",
+                        "corr:x + norm per block, one persistent launch for every size",
             "realisations": count, "s": round(dt, 4), "realisations_per_s": round(count / dt, 1),
             "mean_L": round(float(np.mean(Ls)), 2)}
 

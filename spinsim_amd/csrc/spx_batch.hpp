@@ -127,4 +127,11 @@ int launch_resident(spx_ctx* c, int L, const DevOp* ops, int nops, const DevPuls
                     const double* weights, const double* psi0, double* out, int width,
                     const int* obs_codes, const int* obs_spins, int nobs, int reference_axis,
                     int* queue, int nblocks, hipStream_t stream);
+// Realisations of several sizes (L <= 12) in one persistent launch;
+// `lds` covers the largest size's layout (resident_lds_bytes).
+int launch_resident_all(spx_ctx* c, const DevOp* ops, int nops, const DevPulse* pulses, const DevEvolve* evo,
+                        const DevReal* reals, const int* order, int nreal, const double* weights,
+                        const double* psi0, double* out, int width, const int* obs_codes, const int* obs_spins,
+                        int nobs, int reference_axis, int* queue, int nblocks, size_t lds, hipStream_t stream);
+size_t resident_lds_bytes(int L);
 }  // namespace spx

@@ -2,9 +2,11 @@
 // one pulse program (LocalRunner::run + Sequence::run, the reference's
 // src/ensemble/EnsembleRunner.cpp:27-88 and src/sequence/Sequence.cpp:35-106).
 //
-// A workgroup pulls realisation indices from a queue (ordered longest
-// first on the host) and runs the whole program of one realisation with its
-// state and reference vectors in registers: 512 threads, thread t owning
+// A workgroup pulls realisation indices from a queue (ordered by estimated
+// work, largest first, on the host) and runs the whole program of one
+// realisation with its state and reference vectors in registers; a batch of
+// one size runs the walker instantiated for that L, a batch of several sizes
+// one kernel that runs each realisation with the walker of its own L: 512 threads, thread t owning
 // amplitudes k = t + 512 i. Each Chebyshev order stages T_{n-1} in LDS for
 // the partner gathers, the state's and the reference's side by side up to
 // L = 11 so that both advance under one pair of barriers (at L = 12, 64 KiB
@@ -545,16 +547,28 @@ __device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double*
   }
 }
 
-// One persistent kernel per L: its registers are sized for that L alone.
-// LDS: 2^L amplitudes (two vectors' worth where both evolve together), the
-// reduction scratch, the evolve step's weights.
+// The LDS of a realisation of L spins: 2^L amplitudes (two vectors' worth
+// where both evolve together), the reduction scratch, the evolve step's
+// weights and pair masks.
 template <int L>
-__global__ __launch_bounds__(kBT) void k_resident(RArgs g) {
-  extern __shared__ double2 smem[];
+__device__ void run_sized(const RArgs& g, const DevReal& R, double2* smem) {
   double2* lds = smem;
   double* red = reinterpret_cast<double*>(smem + Res<L>::N * kVectors<L>);   // 2 x kBT
   double2* lw = reinterpret_cast<double2*>(red + 2 * kBT);                   // NP + L weights
   uint32_t* lm = reinterpret_cast<uint32_t*>(lw + Res<L>::NP + L);           // NP pair masks
+  run_real<L>(g, R, lds, red, lw, lm);
+}
+
+template <int... Ls>
+__device__ void run_any(std::integer_sequence<int, Ls...>, int L, const RArgs& g, const DevReal& R,
+                        double2* smem) {
+  ((L == Ls + 1 ? run_sized<Ls + 1>(g, R, smem) : void()), ...);
+}
+
+// One persistent kernel per L: its registers are sized for that L alone.
+template <int L>
+__global__ __launch_bounds__(kBT) void k_resident(RArgs g) {
+  extern __shared__ double2 smem[];
   __shared__ int next;
   for (;;) {
     __syncthreads();
@@ -562,11 +576,49 @@ __global__ __launch_bounds__(kBT) void k_resident(RArgs g) {
     __syncthreads();
     const int slot = next;
     if (slot >= g.nreal) return;
-    run_real<L>(g, g.reals[g.order[slot]], lds, red, lw, lm);
+    run_sized<L>(g, g.reals[g.order[slot]], smem);
+  }
+}
+
+// Every size in one persistent kernel: a workgroup pulls the next
+// realisation of one queue (the host orders it by estimated work, largest
+// first, across sizes) and runs the walker of its L. Its registers are those
+// of the largest walker; from L = 8 up every walker already needs more than
+// a second workgroup per CU would leave it, so the larger sizes lose nothing.
+__global__ __launch_bounds__(kBT) void k_resident_all(RArgs g) {
+  extern __shared__ double2 smem[];
+  __shared__ int next;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) next = atomicAdd(g.queue, 1);
+    __syncthreads();
+    const int slot = next;
+    if (slot >= g.nreal) return;
+    const DevReal& R = g.reals[g.order[slot]];
+    run_any(std::make_integer_sequence<int, 12>{}, R.nspins, g, R, smem);
   }
 }
 
 }  // namespace
+
+size_t resident_lds_bytes(int L) {
+  const int vectors = L <= 11 ? 2 : 1;   // kVectors<L>
+  return sizeof(double2) * (static_cast<size_t>(vectors) << L) + sizeof(double) * 2 * kBT +
+         sizeof(double2) * static_cast<size_t>(L * (L - 1) / 2 + L) +
+         sizeof(uint32_t) * static_cast<size_t>(L * (L - 1) / 2);
+}
+
+int launch_resident_all(spx_ctx* c, const DevOp* ops, int nops, const DevPulse* pulses, const DevEvolve* evo,
+                        const DevReal* reals, const int* order, int nreal, const double* weights,
+                        const double* psi0, double* out, int width, const int* obs_codes, const int* obs_spins,
+                        int nobs, int reference_axis, int* queue, int nblocks, size_t lds, hipStream_t stream) {
+  (void)c;
+  RArgs g{ops, nops, pulses, evo, reals, order, nreal, weights, psi0, out, width,
+          obs_codes, obs_spins, nobs, reference_axis, queue};
+  k_resident_all<<<nblocks, kBT, lds, stream>>>(g);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
 
 int launch_resident(spx_ctx* c, int L, const DevOp* ops, int nops, const DevPulse* pulses,
                     const DevEvolve* evo, const DevReal* reals, const int* order, int nreal,
@@ -575,10 +627,7 @@ int launch_resident(spx_ctx* c, int L, const DevOp* ops, int nops, const DevPuls
                     int* queue, int nblocks, hipStream_t stream) {
   RArgs g{ops, nops, pulses, evo, reals, order, nreal, weights, psi0, out, width,
           obs_codes, obs_spins, nobs, reference_axis, queue};
-  const int vectors = L <= 11 ? 2 : 1;   // kVectors<L>
-  const size_t lds = sizeof(double2) * (static_cast<size_t>(vectors) << L) + sizeof(double) * 2 * kBT +
-                     sizeof(double2) * static_cast<size_t>(L * (L - 1) / 2 + L) +
-                     sizeof(uint32_t) * static_cast<size_t>(L * (L - 1) / 2);
+  const size_t lds = resident_lds_bytes(L);
   switch (L) {
     case 1: k_resident<1><<<nblocks, kBT, lds, stream>>>(g); break;
     case 2: k_resident<2><<<nblocks, kBT, lds, stream>>>(g); break;
