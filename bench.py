@@ -299,9 +299,11 @@ def main():
                     help="realisations stepped together, one launch per order (1..16; 1 = a single state)")
     ap.add_argument("--graphs", action="store_true", help="replay each step as a captured HIP graph")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
-                    help="2: a set of 4..8 runs as two halves on two streams (the default); 1: one stream")
+                    help="2: a set of 4..16 runs as two halves on two streams (the default); 1: one stream")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="L",
                     help="only time the in-place scale kernel on 2^L amplitudes")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the 4096-realisation uneven-L batch (its JSON object alone)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -312,6 +314,9 @@ def main():
     from spinsim_amd import spx
     if args.copy_probe:
         print(json.dumps(copy_probe(spx, args.copy_probe)), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps(batched_leg(spx, spx.Context(0))), flush=True)
         return
     L = args.nspins
     n = 1 << L
