@@ -6,12 +6,14 @@
 // work, largest first, on the host) and runs the whole program of one
 // realisation with its state and reference vectors in registers; a batch of
 // one size runs the walker instantiated for that L, a batch of several sizes
-// one kernel that runs each realisation with the walker of its own L: 512 threads, thread t owning
-// amplitudes k = t + 512 i. Each Chebyshev order stages T_{n-1} in LDS for
-// the partner gathers, the state's and the reference's side by side up to
-// L = 11 so that both advance under one pair of barriers (at L = 12, 64 KiB
-// for one vector, they take turns). An evolve step's pair and field weights
-// and the pair masks are staged in LDS once per step. The weights are per
+// one kernel that runs each realisation with the walker of its own L: 512
+// threads, thread t owning amplitudes k = t + 512 i. Up to L = 10 each
+// Chebyshev order stages T_{n-1} of the state and of the reference side by
+// side in LDS for the partner gathers, under one pair of barriers. At L = 11
+// and 12 the off-diagonal part goes through Walsh-Hadamard transforms
+// instead (about 2.3x less LDS traffic than the gathers there), each vector
+// held by 8 amplitudes per lane. An evolve step's pair and field weights and
+// the pair masks are staged in LDS once per step. The weights are per
 // realisation and per evolve step. All reductions are fixed-order block
 // trees, so every realisation's numbers are independent of which workgroup
 // ran it and of timing.
@@ -203,66 +205,12 @@ __device__ __forceinline__ void stage_weights(double2* lw, uint32_t* lm, const d
   }
 }
 
-// nsteps Chebyshev steps of one vector (Propagator::applyOneStep /
-// evolve, src/core/Propagator.cpp:46-100). Returns false on a zero norm.
-template <int L>
-__device__ bool evolve_vec(double2* v, double2* lds, double* red, double2* lw, uint32_t* lm, uint32_t t,
-                           const double* dg, const double* __restrict__ w, const DevEvolve& e, const DevOp& op) {
-  constexpr int A = Res<L>::A;
-  const bool active = t < static_cast<uint32_t>(Res<L>::N);
-  stage_weights<L>(lw, lm, w, e, t);
-  const double* a = w + e.coef_off;
-  const double scale = -2.0 / e.bound;
-  double2 tprev[A], tcur[A], acc[A];
-  for (int step = 0; step < op.nsteps; ++step) {
-#pragma unroll
-    for (int i = 0; i < A; ++i) {
-      tcur[i] = v[i];
-      v[i] = make_double2(v[i].x * a[0], v[i].y * a[0]);
-    }
-    double wre = op.mode == 0 ? 0.0 : -1.0, wim = op.mode == 0 ? -1.0 : 0.0;
-    for (int n = 1; n <= e.order; ++n) {
-      stage<L>(lds, tcur, t);
-      if (active) apply_h_lds<L>(lds, tcur, acc, t, dg, scale, lw, lm, e.has_field != 0);
-      const double cr = 2.0 * a[n] * wre, ci = 2.0 * a[n] * wim;
-#pragma unroll
-      for (int i = 0; i < A; ++i) {
-        double2 tn;
-        if (n == 1) tn = make_double2(0.5 * acc[i].x, 0.5 * acc[i].y);
-        else tn = make_double2(acc[i].x - tprev[i].x, acc[i].y - tprev[i].y);
-        v[i].x += cr * tn.x - ci * tn.y;
-        v[i].y += cr * tn.y + ci * tn.x;
-        tprev[i] = tcur[i];
-        tcur[i] = tn;
-      }
-      if (op.mode == 0) {
-        const double pr = wre;
-        wre = wim;
-        wim = -pr;
-      } else {
-        wre = -wre;
-        wim = -wim;
-      }
-    }
-    if (op.autonorm) {
-      double s = 0.0;
-      if (active)
-#pragma unroll
-        for (int i = 0; i < A; ++i) s += v[i].x * v[i].x + v[i].y * v[i].y;
-      const double nrm = sqrt(block_sum(red, s, 0.0).x);
-      if (nrm == 0.0) return false;
-      const double inv = 1.0 / nrm;
-#pragma unroll
-      for (int i = 0; i < A; ++i) v[i] = make_double2(v[i].x * inv, v[i].y * inv);
-    }
-  }
-  return true;
-}
-
-// evolve_vec for the state and the reference together: one staging barrier
-// pair per order for both (T_{n-1} of the state in lds[0, N), of the
-// reference in lds[N, 2N)), the step's weights read from LDS. Each vector
-// sees exactly evolve_vec's arithmetic. Returns false on a zero norm.
+// nsteps Chebyshev steps (Propagator::applyOneStep / evolve,
+// src/core/Propagator.cpp:46-100) of the state and the reference together
+// (L <= 10): one staging barrier pair per order for both (T_{n-1} of the
+// state in lds[0, N), of the reference in lds[N, 2N)), the step's weights
+// read from LDS; T_1 = acc / 2, T_n = acc - T_{n-2}, psi += c_n T_n.
+// Returns false on a zero norm.
 template <int L>
 __device__ bool evolve_both(double2* st, double2* rf, double2* lds, double* red, double2* lw, uint32_t* lm,
                             uint32_t t,
@@ -347,6 +295,307 @@ __device__ bool evolve_both(double2* st, double2* rf, double2* lds, double* red,
     }
   }
   return true;
+}
+
+// ---------------------------------------------- L = 11, 12: Walsh-Hadamard apply
+// The off-diagonal part through the identity of spx_hadamard.hip,
+//   H_off x = W A W x + S W B W S^+ x,
+// with W the unnormalised Walsh-Hadamard transform on all L bits, A and B
+// its diagonals from the pair and field weights (times 2^-L) and
+// S = i^popcount. The partner gathers (L(L-1)/2 reads of 16 B per amplitude
+// and order: 1056 B at L = 12, 880 B at L = 11) become four L-bit
+// transforms of 8 values per lane: register stages and three LDS transposes
+// each way (384 B of LDS traffic per amplitude). A vector is held by
+// 2^LB lanes (LB = L - 3): the whole workgroup at L = 12 (state and
+// reference take turns), half of it at L = 11 (the state in threads
+// 0..255, the reference in 256..511, both through the same code and
+// barriers). Layouts of the index c for lane u, register j:
+//   N  c = u | j << LB       (k = u + 2^LB j)
+//   L1 c = j | u << 3        (registers = bits 0-2)
+//   L2 registers = bits 3-5, L3 registers = bits 6-8 (the transform domain;
+//   at L = 11 register bit 2 there is index bit 8, already transformed)
+// and the LDS slot c ^ ((c >> 3) & 7) on bits 0-2, so that each layout's
+// eight consecutive lanes hit eight distinct 16-byte slots.
+namespace wres {
+constexpr unsigned kA = 8;
+template <int LB>
+__host__ __device__ constexpr unsigned c_n(unsigned u, unsigned j) { return u | (j << LB); }
+__host__ __device__ constexpr unsigned c_l1(unsigned u, unsigned j) { return j | (u << 3); }
+__host__ __device__ constexpr unsigned c_l2(unsigned u, unsigned j) { return (u & 7u) | (j << 3) | ((u >> 3) << 6); }
+__host__ __device__ constexpr unsigned c_l3(unsigned u, unsigned j) { return (u & 63u) | (j << 6) | ((u >> 6) << 9); }
+__host__ __device__ constexpr unsigned slot(unsigned c) { return (c & ~7u) | ((c ^ (c >> 3)) & 7u); }
+// per-lane bases plus compile-time offsets (immediate LDS offsets)
+__host__ __device__ constexpr unsigned b_l1(unsigned u, unsigned k) { return (u << 3) + ((k ^ u) & 7u); }
+__host__ __device__ constexpr unsigned b_l2(unsigned u, unsigned k) { return ((u >> 3) << 6) + ((u ^ k) & 7u); }
+__host__ __device__ constexpr unsigned b_l3(unsigned u) { return (u & 56u) + ((u >> 6) << 9) + ((u ^ (u >> 3)) & 7u); }
+template <int LB>
+constexpr bool slots_ok() {
+  for (unsigned u = 0; u < (1u << LB); ++u)
+    for (unsigned j = 0; j < kA; ++j) {
+      if (slot(c_n<LB>(u, j)) != slot(u) + (j << LB)) return false;
+      if (slot(c_l1(u, j)) != b_l1(u, j)) return false;
+      if (slot(c_l2(u, j)) != b_l2(u, j) + (j << 3)) return false;
+      if (slot(c_l3(u, j)) != b_l3(u) + (j << 6)) return false;
+    }
+  return true;
+}
+static_assert(slots_ok<9>() && slots_ok<8>(), "slot bases");
+enum { N = 0, L1 = 1, L2 = 2, L3 = 3 };
+
+template <int LB, int LAY>
+__device__ __forceinline__ unsigned at(unsigned u, unsigned j) {
+  if constexpr (LAY == N) return slot(u) + (j << LB);
+  else if constexpr (LAY == L1) return b_l1(u, j);
+  else if constexpr (LAY == L2) return b_l2(u, j) + (j << 3);
+  else return b_l3(u) + (j << 6);
+}
+
+template <int B0, int B1>
+__device__ __forceinline__ void bfly(double2 (&v)[kA]) {
+#pragma unroll
+  for (int st = B0; st < B1; ++st)
+#pragma unroll
+    for (int j = 0; j < static_cast<int>(kA); ++j) {
+      if (j & (1 << st)) continue;
+      const double2 a = v[j], b = v[j | (1 << st)];
+      v[j] = make_double2(a.x + b.x, a.y + b.y);
+      v[j | (1 << st)] = make_double2(a.x - b.x, a.y - b.y);
+    }
+}
+
+// The lane index reaches the slot arithmetic through an empty asm, so that
+// the bases are recomputed at each transpose instead of held in registers.
+template <int LB, int FROM, int TO>
+__device__ __forceinline__ void xpose(double2* S, uint32_t u, double2 (&v)[kA]) {
+  unsigned uv = u;
+  asm volatile("" : "+v"(uv));
+#pragma unroll
+  for (unsigned j = 0; j < kA; ++j) S[at<LB, FROM>(uv, j)] = v[j];
+  __syncthreads();
+#pragma unroll
+  for (unsigned j = 0; j < kA; ++j) v[j] = S[at<LB, TO>(uv, j)];
+  __syncthreads();
+}
+
+// register stages of the transform domain: bits 6-8, or 6-7 when bit 8 is
+// among the natural layout's register bits (LB = 8)
+template <int LB>
+constexpr int kL3Stages = LB == 8 ? 2 : 3;
+
+template <int LB>
+__device__ __forceinline__ void fwd(double2* S, uint32_t u, double2 (&v)[kA]) {
+  bfly<0, 3>(v);
+  xpose<LB, N, L1>(S, u, v);
+  bfly<0, 3>(v);
+  xpose<LB, L1, L2>(S, u, v);
+  bfly<0, 3>(v);
+  xpose<LB, L2, L3>(S, u, v);
+  bfly<0, kL3Stages<LB>>(v);
+}
+template <int LB>
+__device__ __forceinline__ void inv(double2* S, uint32_t u, double2 (&v)[kA]) {
+  bfly<0, kL3Stages<LB>>(v);
+  xpose<LB, L3, L2>(S, u, v);
+  bfly<0, 3>(v);
+  xpose<LB, L2, L1>(S, u, v);
+  bfly<0, 3>(v);
+  xpose<LB, L1, N>(S, u, v);
+  bfly<0, 3>(v);
+}
+
+// v i^e (e mod 4): a swap and sign changes, exact
+__device__ __forceinline__ double2 ipow(double2 v, unsigned e) {
+  const bool odd = e & 1u, neg = e & 2u;
+  const double re = odd ? -v.y : v.x, im = odd ? v.x : v.y;
+  return make_double2(neg ? -re : re, neg ? -im : im);
+}
+}  // namespace wres
+
+// Chebyshev steps with the Walsh-Hadamard apply for the vector held by this
+// thread (lane u of 2^(L-3), 8 amplitudes k = u + 2^(L-3) j, staged in
+// S[0, 2^L)): the same recursion and accumulator updates (T_n = acc - T_{n-2}
+// is formed as scale diag T_{n-1} - T_{n-2} plus the two transform terms).
+// The diagonal at the own amplitudes and A and B at the transform-domain
+// ones are formed once per evolve op from the step's weights; the step's
+// weights must be staged in lw, lm (stage_weights) and visible.
+template <int L>
+__device__ bool evolve_wht(double2 (&v)[wres::kA], double2* S, double* red, const double2* lw, const uint32_t* lm,
+                           uint32_t u, const double* __restrict__ w, const DevEvolve& e, const DevOp& op) {
+  constexpr int LB = L - 3, A = static_cast<int>(wres::kA), NP = Res<L>::NP;
+  static_assert(L == 11 || L == 12, "the Walsh apply is laid out for L = 11 and 12");
+  // runtime loops over pairs and spins outside, the eight amplitudes inside
+  // (unrolled, so that the arrays stay in registers)
+  double ea[A], eb[A], dg[A];
+  uint32_t sw[A], kn[A];
+#pragma unroll
+  for (int j = 0; j < A; ++j) {
+    sw[j] = wres::c_l3(u, static_cast<unsigned>(j));
+    kn[j] = wres::c_n<LB>(u, static_cast<unsigned>(j));
+    ea[j] = 0.0;
+    eb[j] = 0.0;
+    dg[j] = 0.0;
+  }
+  for (int q = 0; q < NP; ++q) {
+    const double2 c = lw[q];
+    const uint32_t m = lm[q];
+    const double pa = 0.5 * (c.x + c.y), pb = 0.5 * (c.x - c.y);
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+      const double z = (__popc(sw[j] & m) & 1) ? -1.0 : 1.0;
+      ea[j] = fma(pa, z, ea[j]);
+      eb[j] = fma(pb, z, eb[j]);
+    }
+  }
+  for (int sp = 0; sp < L; ++sp) {
+    const double2 c = lw[NP + sp];
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+      const double z = ((sw[j] >> sp) & 1u) ? -1.0 : 1.0;
+      ea[j] = fma(c.x, z, ea[j]);
+      eb[j] = fma(-c.y, z, eb[j]);
+    }
+  }
+  // the sigma_z diagonal in rebuildDiagonal's order (as build_diag)
+  const double* zw = w + e.zdiag_off;
+  for (int sp = 0; sp < L; ++sp) {
+    const double c = zw[sp];
+    if (c == 0.0) continue;
+#pragma unroll
+    for (int j = 0; j < A; ++j) dg[j] += ((kn[j] >> sp) & 1u) ? c : -c;
+  }
+  for (int qa = 0, q = 0; qa < L; ++qa)
+    for (int qb = qa + 1; qb < L; ++qb, ++q) {
+      const double c = zw[L + q];
+      if (c == 0.0) continue;
+#pragma unroll
+      for (int j = 0; j < A; ++j) dg[j] += (((kn[j] >> qa) & 1u) == ((kn[j] >> qb) & 1u)) ? c : -c;
+    }
+#pragma unroll
+  for (int j = 0; j < A; ++j) {
+    ea[j] = ldexp(ea[j], -L);
+    eb[j] = ldexp(eb[j], -L);
+  }
+  const double* a = w + e.coef_off;
+  const double scale = -2.0 / e.bound;
+  const unsigned pcu = static_cast<unsigned>(__popc(u));
+  double2 tprev[A], tcur[A], x[A];
+  for (int step = 0; step < op.nsteps; ++step) {
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      tcur[i] = v[i];
+      v[i] = make_double2(v[i].x * a[0], v[i].y * a[0]);
+    }
+    double wre = op.mode == 0 ? 0.0 : -1.0, wim = op.mode == 0 ? -1.0 : 0.0;
+    for (int n = 1; n <= e.order; ++n) {
+#pragma unroll
+      for (int i = 0; i < A; ++i) {
+        const double f = scale * dg[i];
+        const double2 b = n >= 2 ? tprev[i] : make_double2(0.0, 0.0);
+        tprev[i] = make_double2(fma(f, tcur[i].x, -b.x), fma(f, tcur[i].y, -b.y));
+      }
+      // W A W T_{n-1}
+#pragma unroll
+      for (int i = 0; i < A; ++i) x[i] = tcur[i];
+      wres::fwd<LB>(S, u, x);
+#pragma unroll
+      for (int j = 0; j < A; ++j) x[j] = make_double2(ea[j] * x[j].x, ea[j] * x[j].y);
+      wres::inv<LB>(S, u, x);
+#pragma unroll
+      for (int i = 0; i < A; ++i) tprev[i] = make_double2(tprev[i].x + x[i].x, tprev[i].y + x[i].y);
+      // S W B W S^+ T_{n-1}
+#pragma unroll
+      for (int i = 0; i < A; ++i) x[i] = wres::ipow(tcur[i], 0u - (pcu + static_cast<unsigned>(__popc(i))));
+      wres::fwd<LB>(S, u, x);
+#pragma unroll
+      for (int j = 0; j < A; ++j) x[j] = make_double2(eb[j] * x[j].x, eb[j] * x[j].y);
+      wres::inv<LB>(S, u, x);
+      const double cr = 2.0 * a[n] * wre, ci = 2.0 * a[n] * wim;
+#pragma unroll
+      for (int i = 0; i < A; ++i) {
+        const double2 sv = wres::ipow(x[i], pcu + static_cast<unsigned>(__popc(i)));
+        double2 tn = make_double2(tprev[i].x + sv.x, tprev[i].y + sv.y);
+        if (n == 1) tn = make_double2(0.5 * tn.x, 0.5 * tn.y);
+        v[i].x += cr * tn.x - ci * tn.y;
+        v[i].y += cr * tn.y + ci * tn.x;
+        tprev[i] = tcur[i];
+        tcur[i] = tn;
+      }
+      if (op.mode == 0) {
+        const double pr = wre;
+        wre = wim;
+        wim = -pr;
+      } else {
+        wre = -wre;
+        wim = -wim;
+      }
+    }
+    if (op.autonorm) {
+      // each half's own norm (at L = 11 the halves hold different vectors)
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < A; ++i) s += v[i].x * v[i].x + v[i].y * v[i].y;
+      const bool upper = LB == 8 && threadIdx.x >= (1u << LB);
+      const double2 nn = block_sum(red, upper ? 0.0 : s, upper ? s : 0.0);
+      const double nrm = sqrt(upper ? nn.y : nn.x);
+      if (nn.x == 0.0 || (LB == 8 && nn.y == 0.0)) return false;
+      const double iv = 1.0 / nrm;
+#pragma unroll
+      for (int i = 0; i < A; ++i) v[i] = make_double2(v[i].x * iv, v[i].y * iv);
+    }
+  }
+  return true;
+}
+
+// Both vectors of an evolve op through evolve_wht. L = 12: the state, then
+// the reference, each held by all 512 threads (their walker layout is the
+// transform's natural one). L = 11: the walker's layout (4 amplitudes of
+// each vector per thread) is moved through LDS to 8 amplitudes of one vector
+// per thread and back.
+template <int L>
+__device__ bool evolve_pair_wht(double2* st, double2* rf, double2* lds, double* red, double2* lw, uint32_t* lm,
+                                uint32_t t, const double* __restrict__ w, const DevEvolve& e, const DevOp& op) {
+  stage_weights<L>(lw, lm, w, e, t);
+  __syncthreads();
+  if constexpr (L == 12) {
+    double2 v[wres::kA];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = st[i];
+    if (!evolve_wht<12>(v, lds, red, lw, lm, t, w, e, op)) return false;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      st[i] = v[i];
+      v[i] = rf[i];
+    }
+    if (!evolve_wht<12>(v, lds, red, lw, lm, t, w, e, op)) return false;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rf[i] = v[i];
+    return true;
+  } else {
+    constexpr int N = Res<L>::N, A = Res<L>::A;   // 2048, 4
+    static_assert(L == 11 && A == 4, "half-workgroup vectors are laid out for L = 11");
+    const uint32_t h = t >> 8, u = t & 255u;
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      lds[t + kBT * i] = st[i];
+      lds[N + t + kBT * i] = rf[i];
+    }
+    __syncthreads();
+    double2 v[wres::kA];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = lds[h * N + u + 256 * j];
+    __syncthreads();
+    const bool ok = evolve_wht<11>(v, lds + h * N, red, lw, lm, u, w, e, op);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lds[h * N + u + 256 * j] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      st[i] = lds[t + kBT * i];
+      rf[i] = lds[N + t + kBT * i];
+    }
+    __syncthreads();
+    return ok;
+  }
 }
 
 // SpinRotation::apply (src/core/Gate.cpp:60-85), targets in ascending order.
@@ -511,13 +760,13 @@ __device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double*
     if (op.kind == OP_EVOLVE) {
       const DevEvolve e = g.evo[R.evolve_off + op.arg];
       if (e.order == 0 || op.nsteps == 0) continue;
-      build_diag<L>(dg, t, g.w + e.zdiag_off);
       bool ok;
-      if constexpr (kBoth<L>)
+      if constexpr (L >= 11) {
+        ok = evolve_pair_wht<L>(st, rf, lds, red, lw, lm, t, g.w, e, op);
+      } else {
+        build_diag<L>(dg, t, g.w + e.zdiag_off);
         ok = evolve_both<L>(st, rf, lds, red, lw, lm, t, dg, g.w, e, op);
-      else
-        ok = evolve_vec<L>(st, lds, red, lw, lm, t, dg, g.w, e, op) &&
-             evolve_vec<L>(rf, lds, red, lw, lm, t, dg, g.w, e, op);
+      }
       if (!ok) {
         if (t == 0) atomicExch(g.queue + 1, 1);  // zero norm under autonormalisation
         return;
