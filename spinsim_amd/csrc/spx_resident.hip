@@ -94,12 +94,11 @@ struct Res {
   static constexpr int NP = L * (L - 1) / 2;
 };
 
-// State and reference evolve in one order loop up to L = 11 (at L = 12 the
-// registers of both vectors' recursions do not fit beside each other).
+// LDS amplitudes of a realisation of L spins: up to L = 10 the state's and
+// the reference's T_{n-1} side by side (for the gathers); at L = 11 and 12
+// two transpose buffers of 2^12 amplitudes (for the transforms).
 template <int L>
-constexpr bool kBoth = L <= 11;
-template <int L>
-constexpr int kVectors = kBoth<L> ? 2 : 1;
+constexpr int kLdsAmps = L >= 11 ? 8192 : 2 * Res<L>::N;
 
 // bit `b` of amplitude k = t + 512 i
 template <int L>
@@ -363,8 +362,13 @@ __device__ __forceinline__ void bfly(double2 (&v)[kA]) {
     }
 }
 
-// The lane index reaches the slot arithmetic through an empty asm, so that
-// the bases are recomputed at each transpose instead of held in registers.
+// One transpose through LDS buffer S. Consecutive transposes alternate
+// between two buffers, so one barrier each suffices: a thread reaches the
+// next transpose's barrier only after its reads of this one, and the
+// transpose after that (the only one to write S again) starts after that
+// barrier. The lane index reaches the slot arithmetic through an empty asm,
+// so that the bases are recomputed at each transpose instead of held in
+// registers.
 template <int LB, int FROM, int TO>
 __device__ __forceinline__ void xpose(double2* S, uint32_t u, double2 (&v)[kA]) {
   unsigned uv = u;
@@ -374,7 +378,6 @@ __device__ __forceinline__ void xpose(double2* S, uint32_t u, double2 (&v)[kA]) 
   __syncthreads();
 #pragma unroll
   for (unsigned j = 0; j < kA; ++j) v[j] = S[at<LB, TO>(uv, j)];
-  __syncthreads();
 }
 
 // register stages of the transform domain: bits 6-8, or 6-7 when bit 8 is
@@ -382,24 +385,27 @@ __device__ __forceinline__ void xpose(double2* S, uint32_t u, double2 (&v)[kA]) 
 template <int LB>
 constexpr int kL3Stages = LB == 8 ? 2 : 3;
 
+// The forward and inverse transforms use the buffers S0, S1, S0 and S1,
+// S0, S1: every transpose alternates with the one before (a transform pair
+// is six transposes, so the next pair starts again with S0).
 template <int LB>
-__device__ __forceinline__ void fwd(double2* S, uint32_t u, double2 (&v)[kA]) {
+__device__ __forceinline__ void fwd(double2* S0, double2* S1, uint32_t u, double2 (&v)[kA]) {
   bfly<0, 3>(v);
-  xpose<LB, N, L1>(S, u, v);
+  xpose<LB, N, L1>(S0, u, v);
   bfly<0, 3>(v);
-  xpose<LB, L1, L2>(S, u, v);
+  xpose<LB, L1, L2>(S1, u, v);
   bfly<0, 3>(v);
-  xpose<LB, L2, L3>(S, u, v);
+  xpose<LB, L2, L3>(S0, u, v);
   bfly<0, kL3Stages<LB>>(v);
 }
 template <int LB>
-__device__ __forceinline__ void inv(double2* S, uint32_t u, double2 (&v)[kA]) {
+__device__ __forceinline__ void inv(double2* S0, double2* S1, uint32_t u, double2 (&v)[kA]) {
   bfly<0, kL3Stages<LB>>(v);
-  xpose<LB, L3, L2>(S, u, v);
+  xpose<LB, L3, L2>(S1, u, v);
   bfly<0, 3>(v);
-  xpose<LB, L2, L1>(S, u, v);
+  xpose<LB, L2, L1>(S0, u, v);
   bfly<0, 3>(v);
-  xpose<LB, L1, N>(S, u, v);
+  xpose<LB, L1, N>(S1, u, v);
   bfly<0, 3>(v);
 }
 
@@ -412,14 +418,15 @@ __device__ __forceinline__ double2 ipow(double2 v, unsigned e) {
 }  // namespace wres
 
 // Chebyshev steps with the Walsh-Hadamard apply for the vector held by this
-// thread (lane u of 2^(L-3), 8 amplitudes k = u + 2^(L-3) j, staged in
-// S[0, 2^L)): the same recursion and accumulator updates (T_n = acc - T_{n-2}
+// thread (lane u of 2^(L-3), 8 amplitudes k = u + 2^(L-3) j; its transposes
+// alternate between the buffers S0 and S1 of 2^L amplitudes): the same recursion and accumulator updates (T_n = acc - T_{n-2}
 // is formed as scale diag T_{n-1} - T_{n-2} plus the two transform terms).
 // The diagonal at the own amplitudes and A and B at the transform-domain
 // ones are formed once per evolve op from the step's weights; the step's
 // weights must be staged in lw, lm (stage_weights) and visible.
 template <int L>
-__device__ bool evolve_wht(double2 (&v)[wres::kA], double2* S, double* red, const double2* lw, const uint32_t* lm,
+__device__ bool evolve_wht(double2 (&v)[wres::kA], double2* S0, double2* S1, double* red, const double2* lw,
+                           const uint32_t* lm,
                            uint32_t u, const double* __restrict__ w, const DevEvolve& e, const DevOp& op) {
   constexpr int LB = L - 3, A = static_cast<int>(wres::kA), NP = Res<L>::NP;
   static_assert(L == 11 || L == 12, "the Walsh apply is laid out for L = 11 and 12");
@@ -496,19 +503,19 @@ __device__ bool evolve_wht(double2 (&v)[wres::kA], double2* S, double* red, cons
       // W A W T_{n-1}
 #pragma unroll
       for (int i = 0; i < A; ++i) x[i] = tcur[i];
-      wres::fwd<LB>(S, u, x);
+      wres::fwd<LB>(S0, S1, u, x);
 #pragma unroll
       for (int j = 0; j < A; ++j) x[j] = make_double2(ea[j] * x[j].x, ea[j] * x[j].y);
-      wres::inv<LB>(S, u, x);
+      wres::inv<LB>(S0, S1, u, x);
 #pragma unroll
       for (int i = 0; i < A; ++i) tprev[i] = make_double2(tprev[i].x + x[i].x, tprev[i].y + x[i].y);
       // S W B W S^+ T_{n-1}
 #pragma unroll
       for (int i = 0; i < A; ++i) x[i] = wres::ipow(tcur[i], 0u - (pcu + static_cast<unsigned>(__popc(i))));
-      wres::fwd<LB>(S, u, x);
+      wres::fwd<LB>(S0, S1, u, x);
 #pragma unroll
       for (int j = 0; j < A; ++j) x[j] = make_double2(eb[j] * x[j].x, eb[j] * x[j].y);
-      wres::inv<LB>(S, u, x);
+      wres::inv<LB>(S0, S1, u, x);
       const double cr = 2.0 * a[n] * wre, ci = 2.0 * a[n] * wim;
 #pragma unroll
       for (int i = 0; i < A; ++i) {
@@ -560,13 +567,13 @@ __device__ bool evolve_pair_wht(double2* st, double2* rf, double2* lds, double* 
     double2 v[wres::kA];
 #pragma unroll
     for (int i = 0; i < 8; ++i) v[i] = st[i];
-    if (!evolve_wht<12>(v, lds, red, lw, lm, t, w, e, op)) return false;
+    if (!evolve_wht<12>(v, lds, lds + 4096, red, lw, lm, t, w, e, op)) return false;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       st[i] = v[i];
       v[i] = rf[i];
     }
-    if (!evolve_wht<12>(v, lds, red, lw, lm, t, w, e, op)) return false;
+    if (!evolve_wht<12>(v, lds, lds + 4096, red, lw, lm, t, w, e, op)) return false;
 #pragma unroll
     for (int i = 0; i < 8; ++i) rf[i] = v[i];
     return true;
@@ -584,7 +591,7 @@ __device__ bool evolve_pair_wht(double2* st, double2* rf, double2* lds, double* 
 #pragma unroll
     for (int j = 0; j < 8; ++j) v[j] = lds[h * N + u + 256 * j];
     __syncthreads();
-    const bool ok = evolve_wht<11>(v, lds + h * N, red, lw, lm, u, w, e, op);
+    const bool ok = evolve_wht<11>(v, lds + h * N, lds + 2 * N + h * N, red, lw, lm, u, w, e, op);
 #pragma unroll
     for (int j = 0; j < 8; ++j) lds[h * N + u + 256 * j] = v[j];
     __syncthreads();
@@ -802,7 +809,7 @@ __device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double*
 template <int L>
 __device__ void run_sized(const RArgs& g, const DevReal& R, double2* smem) {
   double2* lds = smem;
-  double* red = reinterpret_cast<double*>(smem + Res<L>::N * kVectors<L>);   // 2 x kBT
+  double* red = reinterpret_cast<double*>(smem + kLdsAmps<L>);               // 2 x kBT
   double2* lw = reinterpret_cast<double2*>(red + 2 * kBT);                   // NP + L weights
   uint32_t* lm = reinterpret_cast<uint32_t*>(lw + Res<L>::NP + L);           // NP pair masks
   run_real<L>(g, R, lds, red, lw, lm);
@@ -851,8 +858,8 @@ __global__ __launch_bounds__(kBT) void k_resident_all(RArgs g) {
 }  // namespace
 
 size_t resident_lds_bytes(int L) {
-  const int vectors = L <= 11 ? 2 : 1;   // kVectors<L>
-  return sizeof(double2) * (static_cast<size_t>(vectors) << L) + sizeof(double) * 2 * kBT +
+  const size_t amps = L >= 11 ? 8192 : static_cast<size_t>(2) << L;   // kLdsAmps<L>
+  return sizeof(double2) * amps + sizeof(double) * 2 * kBT +
          sizeof(double2) * static_cast<size_t>(L * (L - 1) / 2 + L) +
          sizeof(uint32_t) * static_cast<size_t>(L * (L - 1) / 2);
 }
