@@ -201,16 +201,18 @@ def xy8_program(spx, blocks=16, omega=50.0, free_steps=10):
     return P
 
 
-def batched_leg(spx, ctx, count=4096, seed=20240726):
+def batched_leg(spx, ctx, count=4096, seed=20240726, sizes=None):
     """One launch of `count` realisations at uneven L in {8..12} (L from a
     hash of the index, no draws taken from the realisation stream), XY-8
-    repeated 16 times, corr:x and norm per block: realisations per second."""
+    repeated 16 times, corr:x and norm per block: realisations per second.
+    `sizes` (a list of L) replaces the hash: realisation i gets
+    sizes[i % len(sizes)]."""
     obs = [(spx.OBS_CORR_X, 0), (spx.OBS_NORM, 0)]
     b = spx.Batch(ctx, xy8_program(spx), obs, epsilon=1e-7, reference_axis=0)
     g = np.random.default_rng(seed)
     Ls = []
     for i in range(count):
-        L = 8 + ((i * 2654435761) >> 16) % 5
+        L = 8 + ((i * 2654435761) >> 16) % 5 if sizes is None else sizes[i % len(sizes)]
         d = spx.dipolar_realisation(seed, i, L)
         f = np.zeros((L, 3))
         f[:, 2] = d.hz
@@ -224,7 +226,8 @@ def batched_leg(spx, ctx, count=4096, seed=20240726):
     dt = time.perf_counter() - t0
     if not np.isfinite(v).all():
         raise SystemExit("non-finite batched ensemble output")
-    return {"workload": f"{count} realisations, L uniform in 8..12, XY-8 (finite pulses, Omega=50, tau=0.5) x 16, "
+    spread = "L uniform in 8..12" if sizes is None else "L in " + ",".join(map(str, sizes))
+    return {"workload": f"{count} realisations, {spread}, XY-8 (finite pulses, Omega=50, tau=0.5) x 16, "
                         "corr:x + norm per block, one persistent launch for every size",
             "realisations": count, "s": round(dt, 4), "realisations_per_s": round(count / dt, 1),
             "mean_L": round(float(np.mean(Ls)), 2)}
@@ -304,6 +307,10 @@ def main():
                     help="only time the in-place scale kernel on 2^L amplitudes")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the 4096-realisation uneven-L batch (its JSON object alone)")
+    ap.add_argument("--batched-count", type=int, default=4096, metavar="R",
+                    help="with --batched-only: realisations in the batch")
+    ap.add_argument("--batched-sizes", default="", metavar="L[,L...]",
+                    help="with --batched-only: cycle through these sizes instead of L = 8..12 by hash")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -316,7 +323,10 @@ def main():
         print(json.dumps(copy_probe(spx, args.copy_probe)), flush=True)
         return
     if args.batched_only:
-        print(json.dumps(batched_leg(spx, spx.Context(0))), flush=True)
+        sizes = [int(x) for x in args.batched_sizes.split(",")] if args.batched_sizes else None
+        if sizes is not None and not all(1 <= x <= 12 for x in sizes):
+            raise SystemExit("--batched-sizes takes L in 1..12 (the resident runner's range)")
+        print(json.dumps(batched_leg(spx, spx.Context(0), count=args.batched_count, sizes=sizes)), flush=True)
         return
     L = args.nspins
     n = 1 << L
