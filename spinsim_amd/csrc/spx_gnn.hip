@@ -1,9 +1,9 @@
 // The surrogate network of the control variate (include/spx_gnn.h):
-// forward, backward and Adam on the device. Every matrix product is one
-// strided GEMM kernel on BF16 MFMA (v_mfma_f32_16x16x16_bf16, FP32
-// accumulation): a wavefront owns a 16x16 tile of C, four wavefronts a
-// 32x32 block, K is walked 16 at a time. Strides give the transposed
-// operands of the backward products without copies.
+// forward, backward and Adam on the device, every matrix product on BF16
+// MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation). Predictions run layer
+// by layer through one tiled GEMM kernel; a training step runs the whole
+// forward and backward pass of a block of graphs in one kernel (k_train),
+// then one kernel that sums the blocks' gradients and applies Adam.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -47,15 +47,13 @@ struct GemmArgs {
 
 // C = A B (+ C) (+ bias), optional ReLU; A is M x K, B is K x N, bf16,
 // FP32 accumulation. A workgroup of four wavefronts owns a 64 x 64 tile of
-// C (each wavefront 32 x 32, four 16 x 16 MFMA accumulators) and walks its
-// K range 32 at a time through LDS: A as [m][k], B transposed as [n][k], so
-// every MFMA operand fragment is four consecutive bf16 of one LDS row. An
-// operand whose K (or M / N) index is contiguous in memory is loaded eight
-// elements (16 bytes) per thread; other strides element by element. When
-// the output has few tiles and K is long (the weight gradients reduce over
-// every row of the batch), K is split across workgroups into partial tiles
-// that k_gemm_reduce sums in a fixed order, so results do not depend on
-// scheduling.
+// C (each wavefront 32 x 32, four 16 x 16 MFMA accumulators) and walks K 32
+// at a time through LDS: A as [m][k], B transposed as [n][k], so every MFMA
+// operand fragment is four consecutive bf16 of one LDS row. An operand whose
+// K (or M / N) index is contiguous in memory is loaded eight elements (16
+// bytes) per thread; other strides element by element. A prediction's
+// products never split K, so it does not depend on the batch it is
+// computed in.
 constexpr int kBM = 64, kBN = 64, kBK = 32, kLP = kBK + 8;   // LDS row pitch (bf16)
 
 __device__ __forceinline__ void gemm_store(const GemmArgs& g, int row, int col, float v) {
@@ -109,15 +107,12 @@ __device__ __forceinline__ void load_tile(uint16_t (*t)[kLP], const uint16_t* __
   }
 }
 
-// part: null for a single K range (the epilogue is applied here), else
-// [gridDim.z][M][N] partial sums of K ranges of kchunk each.
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g, float* __restrict__ part, int kchunk) {
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
   __shared__ __align__(16) uint16_t As[kBM][kLP];
   __shared__ __align__(16) uint16_t Bs[kBN][kLP];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int bm = blockIdx.y * kBM, bn = blockIdx.x * kBN;
-  const int kbeg = blockIdx.z * kchunk;
-  const int kend = min(g.K, kbeg + kchunk);
+  const int kbeg = 0, kend = g.K;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int fr = lane & 15, fk = (lane >> 4) * 4;
   f32x4 acc[2][2];
@@ -153,22 +148,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g, float* __restrict__ pa
       for (int r = 0; r < 4; ++r) {
         const int row = bm + wm + i * 16 + fk + r;
         if (row >= g.M) continue;
-        if (part)
-          part[(static_cast<long>(blockIdx.z) * g.M + row) * g.N + col] = acc[i][j][r];
-        else
-          gemm_store(g, row, col, acc[i][j][r]);
+        gemm_store(g, row, col, acc[i][j][r]);
       }
     }
-}
-
-// Sum of the split-K partials in K-range order, then the epilogue.
-__global__ __launch_bounds__(256) void k_gemm_reduce(GemmArgs g, const float* __restrict__ part, int splits) {
-  const long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
-  const long mn = static_cast<long>(g.M) * g.N;
-  if (i >= mn) return;
-  float v = 0.f;
-  for (int s = 0; s < splits; ++s) v += part[s * mn + i];
-  gemm_store(g, static_cast<int>(i / g.N), static_cast<int>(i % g.N), v);
 }
 
 __global__ void k_cast(const float* __restrict__ x, uint16_t* __restrict__ y, long n) {
@@ -221,18 +203,6 @@ __global__ void k_mix_cat(const float* __restrict__ J, const float* __restrict__
   row[H + c] = to_bf16(s);
 }
 
-// dx = d[x | m][:, :H] + A^T d[x | m][:, H:] (A symmetric): the backward of
-// k_mix_cat, reading both halves of the product's rows in place.
-__global__ void k_mix_bwd(const float* __restrict__ J, const float* __restrict__ dcat, float* __restrict__ dx,
-                          int L) {
-  const int g = blockIdx.x, i = blockIdx.y, c = threadIdx.x;
-  const float* Jr = J + (static_cast<long>(g) * L + i) * L;
-  const float* dg = dcat + static_cast<long>(g) * L * 2 * H;
-  float s = 0.f;
-  for (int j = 0; j < L; ++j) s += Jr[j] * dg[j * 2 * H + H + c];
-  dx[(static_cast<long>(g) * L + i) * H + c] = dg[i * 2 * H + c] + s;
-}
-
 __global__ void k_pool(const float* __restrict__ x, float* __restrict__ p, int L) {
   const int g = blockIdx.x, c = threadIdx.x;
   float s = 0.f;
@@ -240,105 +210,600 @@ __global__ void k_pool(const float* __restrict__ x, float* __restrict__ p, int L
   p[g * H + c] = s / static_cast<float>(L);
 }
 
-__global__ void k_pool_bwd(const float* __restrict__ dp, float* __restrict__ dx, int L) {
-  const int g = blockIdx.x, i = blockIdx.y, c = threadIdx.x;
-  dx[(static_cast<long>(g) * L + i) * H + c] = dp[g * H + c] / static_cast<float>(L);
-}
+// ---------------------------------------------------------------------------
+// One training step in two launches. k_train runs the forward pass, the loss
+// and the backward pass of whole graphs: a workgroup takes a block of at most
+// kTR rows (gb graphs of L spins, zero rows after them), keeps every
+// activation of the block in LDS, and runs each product on MFMA from LDS
+// operands, so the ~40 dependent launches of the layer-by-layer path become
+// one. Its weight and bias gradients are partial sums over the block's rows,
+// written to row blockIdx.x of `part` by a launch of the first kTrainWG
+// blocks and added to it by the launches of the blocks after them;
+// k_train_reduce sums the rows in order, so every result is independent of
+// scheduling, and applies Adam. The rounding points are those of the layer-by-layer path: bf16
+// operands of every product, FP32 accumulation, FP32 pre-activations,
+// messages, pooling and gradients of the activations.
 
-// d (MSE) / dy and the loss terms (summed on the host in a fixed order)
-__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, float* __restrict__ dy,
-                      uint16_t* __restrict__ dybf, float* __restrict__ sq, long n, float scale) {
-  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float d = y[i] - t[i];
-  const float v = 2.f * d * scale;
-  dy[i] = v;
-  dybf[i] = to_bf16(v);
-  sq[i] = d * d;
-}
+constexpr int kTR = 64;             // rows of a graph block
+constexpr int kTW = 4;              // wavefronts of k_train
+constexpr int kTrainWG = 128;       // blocks per launch (the partial rows)
+// LDS row strides in bf16 elements: 4 (mod 8) dwords between rows, so the 16
+// rows of an MFMA operand fragment start on distinct bank quadruples
+constexpr int kSC = 2 * H + 8;      // [row][2H] layer inputs
+constexpr int kSH = H + 8;          // [row][H]
+constexpr int kSR = kTR + 8;        // [*][row]: K = rows (weight gradients)
+constexpr int kSG = 16 + 8;         // [*][16]: K = features or graphs
+constexpr int kSX = H + 4;          // FP32 rows [row][H]
+// LDS regions (bytes)
+constexpr int kR1 = kTR * kSH * 2 + H * kSR * 2;           // layer input; backward dz and dz^T
+constexpr int kR2 = 2 * H * kSR * 2;                       // staged weights or the layer input transposed
+constexpr int kR3 = kTR * kSX * 4;                         // activations / their gradients, FP32
+constexpr int kR4 = kTR * kSX * 4;                         // features, readout, message gradients
+constexpr int kRA = kTR * 30 * 4;                          // edge weights [graph][i][j], gb L L <= 64 L
+constexpr int kRM = (NLAYER + 1) * 32 * 4 * 8;             // ReLU masks: a ballot per 16x16 tile and row group
+constexpr int kRF = 16 * kSR * 2;                          // features transposed
+constexpr int kRS = 2 * H * 4;                             // half-column sums
+constexpr int kRB = (NLAYER + 2) * H * 4;                  // biases b0..b_NLAYER, b_out
+constexpr int kTrainLds = kR1 + kR2 + kR3 + kR4 + kRA + kRM + kRF + kRS + kRB;
+static_assert(kTR * kSC * 2 <= kR1 && 64 * kSC * 2 <= kR2 && H * kSH * 2 <= kR2, "LDS regions");
+static_assert(16 * kSH * 2 + H * kSG * 2 + 16 * H * 4 + 16 * kSH * 2 + H * kSG * 2 <= kR4, "readout region");
+static_assert(kTrainLds <= 160 * 1024, "LDS budget");
 
-// dz = dx where the pre-activation is positive, in FP32 and in bf16
-__global__ void k_relu_bwd(const float* __restrict__ dx, const float* __restrict__ z, float* __restrict__ dz,
-                           uint16_t* __restrict__ dzbf, long n) {
-  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float v = z[i] > 0.f ? dx[i] : 0.f;
-  dz[i] = v;
-  dzbf[i] = to_bf16(v);
-}
+struct TrainArgs {
+  const float* J;               // couplings [n][L][L], FP32
+  const float* h;               // fields [n][L]
+  const float* t;               // targets [n][P]
+  const uint32_t* order;        // graph i of the batch is order[*step G + i]; null: graph i
+  const int* step;
+  int G, L, P, gb;              // batch graphs, spins, outputs, graphs per block
+  int blk0;                     // block of workgroup 0: a launch covers blocks blk0 .. blk0 + gridDim.x - 1
+  float inv_sj, inv_sh, scale;  // 1/s_J, 1/s_h, 1/(G P)
+  const float* p;               // FP32 parameters (the biases are read here)
+  const uint16_t* pbf;          // bf16 parameters
+  const uint16_t* pbfT;         // bf16 W_l^T [2H][H] (l = 1..3) and W_out^T [H][16 ceil(P / 16)]
+  long nparam;
+  float* part;                  // [gridDim.x][nparam] gradient partials
+  uint16_t* catg;               // [gridDim.x][NLAYER][kTR][2H] layer inputs for the backward
+  float* sq;                    // [G][P] squared errors
+};
 
-// Column sums of a rows x cols matrix: a block owns 64 columns; its 16 row
-// lanes sum rows r = lane (mod 16) (coalesced 256-byte rows), then a fixed
-// tree over the lanes, so the result does not depend on scheduling.
-__global__ __launch_bounds__(1024) void k_colsum(const float* __restrict__ x, float* __restrict__ out, long rows,
-                                                 int cols) {
-  __shared__ float part[16][64];
-  const int cl = threadIdx.x & 63, lane = threadIdx.x >> 6;
-  const int c = blockIdx.x * 64 + cl;
-  float s = 0.f;
-  if (c < cols) {
-#pragma unroll 4
-    for (long r = lane; r < rows; r += 16) s += x[r * cols + c];
+// Parameter layout: W0 [H][F], b0, then W_l [H][2H], b_l for l = 1..NLAYER,
+// then W_out [P][H], b_out; the transposed copies W_l^T [2H][H], then
+// W_out^T [H][Pp] with Pp = 16 ceil(P / 16) (rows zero-padded).
+__host__ __device__ constexpr long off_w(int l) {
+  return l == 0 ? 0 : H * F + H + static_cast<long>(l - 1) * (2 * H * H + H);
+}
+__host__ __device__ constexpr long off_b(int l, int P) {
+  return off_w(l) + (l == 0 ? H * F : l <= NLAYER ? 2 * H * H : static_cast<long>(P) * H);
+}
+__host__ __device__ constexpr long off_wt(int l) { return static_cast<long>(l - 1) * 2 * H * H; }
+
+// C = A Bt^T over the wave's tiles: A [M][K] (row stride sa) and Bt [N][K]
+// (stride sb), bf16 in LDS. Wave w takes the N tiles w, w + kTW, ... and the
+// first mt (<= MT) M tiles; epi(i, r, row, col, v) is called for every
+// element in lockstep across the wave (uniform control), v plus cbias[col]
+// when a column bias is given. K = k <= KMAX, a
+// multiple of 16, walked in unrolled runs of four k-steps (so the next
+// fragments load while the MFMAs run).
+template <int MT, int KMAX, class Epi>
+__device__ __forceinline__ void tile_gemm(const uint16_t* A, int sa, const uint16_t* Bt, int sb, int mt, int nt,
+                                          int k, Epi epi, const float* cbias = nullptr) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int fr = lane & 15, fk = (lane >> 4) * 4;
+  for (int tn = wave; tn < nt; tn += kTW) {
+    // the column's bias in a register: read in the epilogue, it would be
+    // re-read after every store (the compiler cannot tell the arrays apart)
+    const float cb = cbias ? cbias[tn * 16 + fr] : 0.f;
+    f32x4 acc[MT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    constexpr int KC = KMAX < 64 ? KMAX : 64;   // k-steps unrolled four at a time
+#pragma unroll 1
+    for (int kc = 0; kc < KMAX; kc += KC) {
+      if (kc >= k) break;
+#pragma unroll
+      for (int k0 = kc; k0 < kc + KC; k0 += 16) {
+        if (k0 >= k) break;
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(Bt + (tn * 16 + fr) * sb + k0 + fk);
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+          if (i >= mt) break;
+          const bf16x4 x = *reinterpret_cast<const bf16x4*>(A + (i * 16 + fr) * sa + k0 + fk);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, b, acc[i], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      if (i >= mt) break;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) epi(i, r, i * 16 + fk + r, tn * 16 + fr, cbias ? acc[i][r] + cb : acc[i][r]);
+    }
   }
-  part[lane][cl] = s;
-  __syncthreads();
-  for (int w = 8; w > 0; w >>= 1) {
-    if (lane < w) part[lane][cl] += part[lane + w][cl];
+}
+
+// The same with B given as [K][N] (row stride sb): each lane gathers its
+// four consecutive k of one column with 16-bit reads (the 16 columns of a
+// fragment are consecutive halfwords, the four k groups 16 banks apart at
+// the strides used here).
+template <int MT, int K, class Epi>
+__device__ __forceinline__ void tile_gemm_kn(const uint16_t* A, int sa, const uint16_t* Bkn, int sb, int nt, Epi epi) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int fr = lane & 15, fk = (lane >> 4) * 4;
+  for (int tn = wave; tn < nt; tn += kTW) {
+    f32x4 acc[MT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      const uint16_t* bp = Bkn + (k0 + fk) * sb + tn * 16 + fr;
+      bf16x4 b;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = static_cast<short>(bp[q * sb]);
+#pragma unroll
+      for (int i = 0; i < MT; ++i) {
+        const bf16x4 x = *reinterpret_cast<const bf16x4*>(A + (i * 16 + fr) * sa + k0 + fk);
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, b, acc[i], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) epi(i, r, i * 16 + fk + r, tn * 16 + fr, acc[i][r]);
+  }
+}
+
+// A staging of bf16 rows from global memory into LDS in two halves: load()
+// issues every 16-byte load of the thread into registers, store() writes
+// them to LDS. Issued right after a barrier, the loads run under the product
+// that precedes the staging's use. nrows rows of per 16-byte pieces
+// (nrows * per <= 2048); rows at or beyond `rows` are zero.
+struct Staging {
+  uint4 v[8];
+  __device__ __forceinline__ void load(const uint16_t* __restrict__ src, long ss, int rows, int nrows, int per) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = threadIdx.x + q * kTW * 64, r = e / per, c = (e % per) * 8;
+      v[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (e < nrows * per && r < rows) v[q] = *reinterpret_cast<const uint4*>(src + r * ss + c);
+    }
+  }
+  __device__ __forceinline__ void store(uint16_t* dst, int sd, int nrows, int per) const {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = threadIdx.x + q * kTW * 64, r = e / per, c = (e % per) * 8;
+      if (e < nrows * per) *reinterpret_cast<uint4*>(dst + r * sd + c) = v[q];
+    }
+  }
+};
+
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+  return static_cast<uint32_t>(to_bf16(lo)) | (static_cast<uint32_t>(to_bf16(hi)) << 16);
+}
+
+// 32 FP32 values as bf16 into 64 bytes (16-byte aligned)
+__device__ __forceinline__ void store_bf16x32(uint16_t* dst, const float* v) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    reinterpret_cast<uint4*>(dst)[q] = make_uint4(pack2(v[8 * q], v[8 * q + 1]), pack2(v[8 * q + 2], v[8 * q + 3]),
+                                                  pack2(v[8 * q + 4], v[8 * q + 5]), pack2(v[8 * q + 6], v[8 * q + 7]));
+}
+
+// Row-chunk passes: lane q4 (0..3) of a row takes its columns 16 k + 4 q4 + e
+// (k < 8, e < 4) as v[4 k + e], so the four lanes of a row touch 64
+// contiguous bytes at each k (FP32 rows of 128 columns).
+__device__ __forceinline__ void load_row32(float* v, const float* row, int q4) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float4 x = *reinterpret_cast<const float4*>(row + 16 * k + 4 * q4);
+    v[4 * k] = x.x;
+    v[4 * k + 1] = x.y;
+    v[4 * k + 2] = x.z;
+    v[4 * k + 3] = x.w;
+  }
+}
+
+__device__ __forceinline__ void store_row32(float* row, const float* v, int q4) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    *reinterpret_cast<float4*>(row + 16 * k + 4 * q4) = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+}
+
+__device__ __forceinline__ void store_row32_bf16(uint16_t* row, const float* v, int q4) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    *reinterpret_cast<uint2*>(row + 16 * k + 4 * q4) = make_uint2(pack2(v[4 * k], v[4 * k + 1]),
+                                                                  pack2(v[4 * k + 2], v[4 * k + 3]));
+}
+
+// ReLU masks: one ballot per 16 x 16 tile of a 64 x 128 activation and row
+// group r = row & 3; bit ((row & 15) >> 2) * 16 + (col & 15). The 16 bits of
+// row `row` in the tile holding column col.
+__device__ __forceinline__ uint32_t mask_bits16(const unsigned long long* mk, int row, int col) {
+  const unsigned long long w = mk[((row >> 4) * 8 + (col >> 4)) * 4 + (row & 3)];
+  return static_cast<uint32_t>(w >> (((row & 15) >> 2) * 16)) & 0xffffu;
+}
+
+__global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint16_t* R1 = reinterpret_cast<uint16_t*>(smem);
+  uint16_t* R2 = reinterpret_cast<uint16_t*>(smem + kR1);
+  float* R3 = reinterpret_cast<float*>(smem + kR1 + kR2);
+  float* R4 = reinterpret_cast<float*>(smem + kR1 + kR2 + kR3);
+  float* sA = reinterpret_cast<float*>(smem + kR1 + kR2 + kR3 + kR4);
+  unsigned long long* mk = reinterpret_cast<unsigned long long*>(smem + kR1 + kR2 + kR3 + kR4 + kRA);
+  uint16_t* featT = reinterpret_cast<uint16_t*>(smem + kR1 + kR2 + kR3 + kR4 + kRA + kRM);
+  float* half_sum = reinterpret_cast<float*>(smem + kR1 + kR2 + kR3 + kR4 + kRA + kRM + kRF);   // [2][H]
+  float* bias = half_sum + 2 * H;                                                                // [NLAYER + 2][H]
+  // sub-regions
+  uint16_t* cat = R1;                                           // [kTR][kSC]
+  uint16_t* dzr = R1;                                           // [kTR][kSH]
+  uint16_t* dzT = R1 + kTR * kSH;                               // [H][kSR]
+  uint16_t* feat = reinterpret_cast<uint16_t*>(R4);             // [kTR][kSG]
+  uint16_t* poolr = reinterpret_cast<uint16_t*>(R4);            // [16][kSH]
+  uint16_t* poolT = poolr + 16 * kSH;                           // [H][kSG]
+  float* dyf = reinterpret_cast<float*>(poolT + H * kSG);       // [16][H]
+  uint16_t* dyr = reinterpret_cast<uint16_t*>(dyf + 16 * H);    // [16][sp]
+  uint16_t* dyT = dyr + 16 * kSH;                               // [H][kSG]
+  const int tid = threadIdx.x;
+  const int L = a.L, P = a.P, Pt = (P + 15) / 16, sp = Pt * 16 + 8;
+  // row-chunk mapping of the elementwise passes: row tr, lane q4 of the row
+  const int tr = tid >> 2, q4 = tid & 3;
+  // column mapping: column cc, rows ch .. ch + 31
+  const int cc = tid & (H - 1), ch = (tid >> 7) * 32;
+  float* part = a.part + static_cast<long>(blockIdx.x) * a.nparam;
+  uint16_t* catg = a.catg + static_cast<long>(blockIdx.x) * NLAYER * kTR * 2 * H;
+  const long base = a.order ? static_cast<long>(*a.step) * a.G : 0;
+  const bool first = a.blk0 == 0;
+  auto put = [&](long i, float v) {
+    if (first) part[i] = v;
+    else part[i] += v;
+  };
+  {
+    const int blk = a.blk0 + static_cast<int>(blockIdx.x);
+    const int g0 = blk * a.gb, ng = min(a.gb, a.G - g0), R = ng * L;
+    auto src_of = [&](int gl) -> long {
+      const long gi = g0 + gl;
+      return a.order ? static_cast<long>(a.order[base + gi]) : gi;
+    };
+    // dz = dx where the layer's pre-activation was positive: bf16 [row][c]
+    // (row-chunk mapping) when want_rows, bf16 [c][row] and the bias
+    // gradient (column mapping); dx in R3.
+    auto relu_bwd = [&](const unsigned long long* ml, bool want_rows, long bias_off) {
+      if (want_rows) {
+        float v[32];
+        load_row32(v, R3 + tr * kSX, q4);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const uint32_t m = tr < R ? mask_bits16(ml, tr, 16 * k) >> (4 * q4) : 0u;
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (!((m >> e) & 1u)) v[4 * k + e] = 0.f;
+        }
+        store_row32_bf16(dzr + tr * kSH, v, q4);
+      }
+      // loads first, unconditionally: no read waits on another
+      float v[32], s = 0.f;
+      uint32_t mb[32];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        v[q] = R3[(ch + q) * kSX + cc];
+        mb[q] = mask_bits16(ml, ch + q, cc);
+      }
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        if (!(ch + q < R && ((mb[q] >> (cc & 15)) & 1u))) v[q] = 0.f;
+        s += v[q];
+      }
+      store_bf16x32(dzT + cc * kSR + ch, v);
+      half_sum[(ch >> 5) * H + cc] = s;
+      __syncthreads();
+      if (tid < H) put(bias_off + tid, half_sum[tid] + half_sum[H + tid]);
+    };
+    Staging st;
+    st.load(a.pbf + off_w(0), F, H, H, F / 8);    // W0 [c][f]
+    for (int e = tid; e < (NLAYER + 2) * H; e += kTW * 64) {
+      const int l = e / H, c = e % H;
+      bias[e] = l <= NLAYER ? a.p[off_b(l, P) + c] : (c < P ? a.p[off_b(NLAYER + 1, P) + c] : 0.f);
+    }
+    // 1. edge weights and features (as k_graph_inputs)
+    float* deg = R3;
+    if (tid < kTR) {
+      const int r = tid;
+      float d = 0.f, s2 = 0.f;
+      uint16_t f[4] = {0, 0, 0, 0};
+      if (r < R) {
+        const int gl = r / L, i = r % L;
+        const long src = src_of(gl);
+        const float* Jr = a.J + (src * L + i) * L;
+        for (int j = 0; j < L; ++j) {
+          const float x = Jr[j] * a.inv_sj;
+          d += fabsf(x);
+          s2 += x * x;
+        }
+        f[0] = to_bf16(a.h[src * L + i] * a.inv_sh);
+        f[1] = to_bf16(d);
+        f[2] = to_bf16(log1pf(s2));
+        f[3] = to_bf16(1.f);
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const uint16_t v = k < 4 ? f[k] : uint16_t(0);
+        feat[r * kSG + k] = v;
+        featT[k * kSR + r] = v;
+      }
+      deg[r] = d;
+    }
+    __syncthreads();
+    for (int e = tid; e < R * L; e += kTW * 64) {
+      const int r = e / L, j = e % L, gl = r / L, i = r % L;
+      const float* Jr = a.J + (src_of(gl) * L + i) * L;
+      sA[e] = Jr[j] * a.inv_sj / sqrtf((1.f + deg[r]) * (1.f + deg[gl * L + j]));
+    }
+    st.store(R2, kSG, H, F / 8);
+    __syncthreads();
+    // 2. layer 0: x = relu(feat W0^T + b0)
+    st.load(a.pbf + off_w(1), 2 * H, 64, 64, 2 * H / 8);
+    {
+      tile_gemm<4, F>(feat, kSG, R2, kSG, 4, H / 16, F, [&](int i, int r, int row, int col, float z) {
+        const unsigned long long w = __ballot(z > 0.f);
+        if ((threadIdx.x & 63) == 0) mk[(i * 8 + (col >> 4)) * 4 + r] = w;
+        R3[row * kSX + col] = row < R ? fmaxf(z, 0.f) : 0.f;
+      }, bias);
+    }
+    __syncthreads();
+    // 3. layers 1..NLAYER: cat = [x | A x] in bf16, x = relu(cat W_l^T + b_l)
+    for (int l = 1; l <= NLAYER; ++l) {
+      {
+        float x[32], m[32];
+        load_row32(x, R3 + tr * kSX, q4);
+#pragma unroll
+        for (int q = 0; q < 32; ++q) m[q] = 0.f;
+        if (tr < R) {
+          const int gl = tr / L, i = tr % L;
+          const float* Ar = sA + (gl * L + i) * L;
+          int j = 0;
+          for (; j + 1 < L; j += 2) {
+            const float w0 = Ar[j], w1 = Ar[j + 1];
+            float y0[32], y1[32];
+            load_row32(y0, R3 + (gl * L + j) * kSX, q4);
+            load_row32(y1, R3 + (gl * L + j + 1) * kSX, q4);
+#pragma unroll
+            for (int q = 0; q < 32; ++q) m[q] = (m[q] + w0 * y0[q]) + w1 * y1[q];
+          }
+          if (j < L) {
+            const float w = Ar[j];
+            float y[32];
+            load_row32(y, R3 + (gl * L + j) * kSX, q4);
+#pragma unroll
+            for (int q = 0; q < 32; ++q) m[q] += w * y[q];
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 32; ++q) x[q] = 0.f;
+        }
+        uint16_t* cg = catg + static_cast<long>(l - 1) * kTR * 2 * H + tr * 2 * H;
+        store_row32_bf16(cat + tr * kSC, x, q4);
+        store_row32_bf16(cat + tr * kSC + H, m, q4);
+        store_row32_bf16(cg, x, q4);
+        store_row32_bf16(cg + H, m, q4);
+      }
+      st.store(R2, kSC, 64, 2 * H / 8);
+      const float* b = bias + l * H;
+      for (int half = 0; half < 2; ++half) {
+        __syncthreads();
+        if (half == 0)
+          st.load(a.pbf + off_w(l) + 64L * 2 * H, 2 * H, 64, 64, 2 * H / 8);
+        else if (l < NLAYER)
+          st.load(a.pbf + off_w(l + 1), 2 * H, 64, 64, 2 * H / 8);
+        else
+          st.load(a.pbf + off_w(NLAYER + 1), H, P, Pt * 16, H / 8);    // W_out [q][c]
+        tile_gemm<4, 2 * H>(cat, kSC, R2, kSC, 4, 4, 2 * H, [&](int i, int r, int row, int c0, float z) {
+          const int col = half * 64 + c0;
+          const unsigned long long w = __ballot(z > 0.f);
+          if ((threadIdx.x & 63) == 0) mk[((l * 32) + i * 8 + (col >> 4)) * 4 + r] = w;
+          R3[row * kSX + col] = row < R ? fmaxf(z, 0.f) : 0.f;
+        }, b + half * 64);
+        __syncthreads();
+        if (half == 0) st.store(R2, kSC, 64, 2 * H / 8);
+      }
+    }
+    // 4. readout, loss and its gradient; the block's targets into R1 (free
+    // until the backward pass) while the pooling runs
+    float* tg = reinterpret_cast<float*>(R1);                     // [16][H]
+    for (int e = tid; e < ng * P; e += kTW * 64) tg[(e / P) * H + e % P] = a.t[src_of(e / P) * P + e % P];
+    {
+      const int gl = tid >> 4, c0 = (tid & 15) * 8;
+      float s[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[q] = 0.f;
+      if (gl < ng) {
+        for (int i = 0; i < L; ++i) {
+          const float4* xr = reinterpret_cast<const float4*>(R3 + (gl * L + i) * kSX + c0);
+          const float4 u = xr[0], w = xr[1];
+          s[0] += u.x; s[1] += u.y; s[2] += u.z; s[3] += u.w;
+          s[4] += w.x; s[5] += w.y; s[6] += w.z; s[7] += w.w;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint16_t v = gl < ng ? to_bf16(s[q] / static_cast<float>(L)) : uint16_t(0);
+        poolr[gl * kSH + c0 + q] = v;
+        poolT[(c0 + q) * kSG + gl] = v;
+      }
+    }
+    st.store(R2, kSH, Pt * 16, H / 8);
+    __syncthreads();
+    st.load(a.pbfT + off_wt(NLAYER + 1), Pt * 16, H, H, Pt * 2);   // W_out^T [c][q], rows padded
+    {
+      tile_gemm<1, H>(poolr, kSH, R2, kSH, 1, Pt, H, [&](int, int, int gl, int q, float y) {
+        float dv = 0.f;
+        if (gl < ng && q < P) {
+          const float d = y - tg[gl * H + q];
+          a.sq[static_cast<long>(g0 + gl) * P + q] = d * d;
+          dv = 2.f * d * a.scale;
+        }
+        dyf[gl * H + q] = dv;
+        const uint16_t db = to_bf16(dv);
+        dyr[gl * sp + q] = db;
+        dyT[q * kSG + gl] = db;
+      }, bias + (NLAYER + 1) * H);
+    }
+    __syncthreads();
+    for (int q = tid; q < P; q += kTW * 64) {
+      float s = 0.f;
+      for (int gl = 0; gl < ng; ++gl) s += dyf[gl * H + q];
+      put(off_b(NLAYER + 1, P) + q, s);
+    }
+    tile_gemm<8, 16>(dyT, kSG, poolT, kSG, Pt, H / 16, 16, [&](int, int, int q, int c, float v) {
+      if (q < P) put(off_w(NLAYER + 1) + static_cast<long>(q) * H + c, v);
+    });
+    st.store(R2, sp, H, Pt * 2);
+    __syncthreads();
+    st.load(catg + static_cast<long>(NLAYER - 1) * kTR * 2 * H, 2 * H, kTR, kTR, 2 * H / 8);
+    tile_gemm<1, 128>(dyr, sp, R2, sp, 1, H / 16, Pt * 16, [&](int, int, int gl, int c, float v) {
+      if (gl < ng) {
+        const float dp = v / static_cast<float>(L);
+        for (int i = 0; i < L; ++i) R3[(gl * L + i) * kSX + c] = dp;
+      }
+    });
+    for (int e = R * kSX + tid; e < kTR * kSX; e += kTW * 64) R3[e] = 0.f;
+    __syncthreads();
+    // 5. layers NLAYER..1: dz, the bias and weight gradients, dx of the layer below
+    for (int l = NLAYER; l >= 1; --l) {
+      relu_bwd(mk + l * 32 * 4, true, off_b(l, P));
+      st.store(R2, kSC, kTR, 2 * H / 8);               // the layer input, [row][j]
+      __syncthreads();
+      st.load(a.pbfT + off_wt(l) + static_cast<long>(H) * H, H, H, H, H / 8);
+      tile_gemm_kn<8, kTR>(dzT, kSR, R2, kSC, 2 * H / 16, [&](int, int, int c, int j, float v) {
+        put(off_w(l) + static_cast<long>(c) * 2 * H + j, v);
+      });
+      // d[x | m] = dz W_l, the message half first
+      __syncthreads();
+      st.store(R2, kSH, H, H / 8);
+      __syncthreads();
+      st.load(a.pbfT + off_wt(l), H, H, H, H / 8);
+      tile_gemm<4, H>(dzr, kSH, R2, kSH, 4, H / 16, H, [&](int, int, int r, int c, float v) {
+        R4[r * kSX + c] = v;
+      });
+      __syncthreads();
+      st.store(R2, kSH, H, H / 8);
+      __syncthreads();
+      if (l > 1) st.load(catg + static_cast<long>(l - 2) * kTR * 2 * H, 2 * H, kTR, kTR, 2 * H / 8);
+      tile_gemm<4, H>(dzr, kSH, R2, kSH, 4, H / 16, H, [&](int, int, int r, int c, float v) {
+        R3[r * kSX + c] = v;
+      });
+      __syncthreads();
+      // dx = d[x] + A^T d[m] (A symmetric), rows beyond the block zero
+      {
+        float v[32];
+        load_row32(v, R3 + tr * kSX, q4);
+        if (tr < R) {
+          const int gl = tr / L, i = tr % L;
+          const float* Ar = sA + (gl * L + i) * L;
+          float s[32];
+#pragma unroll
+          for (int q = 0; q < 32; ++q) s[q] = 0.f;
+          int j = 0;
+          for (; j + 1 < L; j += 2) {
+            const float w0 = Ar[j], w1 = Ar[j + 1];
+            float y0[32], y1[32];
+            load_row32(y0, R4 + (gl * L + j) * kSX, q4);
+            load_row32(y1, R4 + (gl * L + j + 1) * kSX, q4);
+#pragma unroll
+            for (int q = 0; q < 32; ++q) s[q] = (s[q] + w0 * y0[q]) + w1 * y1[q];
+          }
+          if (j < L) {
+            const float w = Ar[j];
+            float y[32];
+            load_row32(y, R4 + (gl * L + j) * kSX, q4);
+#pragma unroll
+            for (int q = 0; q < 32; ++q) s[q] += w * y[q];
+          }
+#pragma unroll
+          for (int q = 0; q < 32; ++q) v[q] += s[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 32; ++q) v[q] = 0.f;
+        }
+        store_row32(R3 + tr * kSX, v, q4);
+      }
+      __syncthreads();
+    }
+    // 6. layer 0
+    relu_bwd(mk, false, off_b(0, P));
+    tile_gemm<8, kTR>(dzT, kSR, featT, kSR, 8, 1, kTR, [&](int, int, int c, int f, float v) {
+      put(off_w(0) + static_cast<long>(c) * F + f, v);
+    });
     __syncthreads();
   }
-  if (lane == 0 && c < cols) out[c] = part[0][cl];
 }
 
-__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, long n, float lr, float b1t, float b2t) {
-  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  m[i] = 0.9f * m[i] + 0.1f * gi;
-  v[i] = 0.999f * v[i] + 0.001f * gi * gi;
-  const float mh = m[i] / (1.f - b1t), vh = v[i] / (1.f - b2t);
-  p[i] -= lr * mh / (sqrtf(vh) + 1e-8f);
+// Sum of the workgroups' gradient partials in workgroup order, then (adam)
+// Adam with bias corrections (b1t, b2t), or bt[2 step], bt[2 step + 1] when
+// bt is given; writes the bf16 parameters and the transposed bf16 weights.
+struct ReduceArgs {
+  const float* part;
+  int nwg;                      // partial rows; 0: no gradient (the bf16 copies only)
+  long nparam;
+  float* grad;
+  float *p, *m, *v;
+  int adam;
+  float lr, b1t, b2t;
+  const float* bt;
+  const int* step;
+  uint16_t *pbf, *pbfT;
+  int P;
+};
+
+__global__ __launch_bounds__(256) void k_train_reduce(ReduceArgs a) {
+  const long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= a.nparam) return;
+  float pv = a.p[i];
+  if (a.nwg > 0) {
+    // rows added in order, eight loads in flight at a time
+    float s = 0.f;
+    int b = 0;
+    for (; b + 8 <= a.nwg; b += 8) {
+      float x[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x[q] = a.part[static_cast<long>(b + q) * a.nparam + i];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s += x[q];
+    }
+    for (; b < a.nwg; ++b) s += a.part[static_cast<long>(b) * a.nparam + i];
+    a.grad[i] = s;
+    if (a.adam) {
+      float b1t = a.b1t, b2t = a.b2t;
+      if (a.bt) {
+        const int st = *a.step;
+        b1t = a.bt[2 * st];
+        b2t = a.bt[2 * st + 1];
+      }
+      const float mi = 0.9f * a.m[i] + 0.1f * s;
+      const float vi = 0.999f * a.v[i] + 0.001f * s * s;
+      a.m[i] = mi;
+      a.v[i] = vi;
+      const float mh = mi / (1.f - b1t), vh = vi / (1.f - b2t);
+      pv -= a.lr * mh / (sqrtf(vh) + 1e-8f);
+      a.p[i] = pv;
+    }
+  }
+  const uint16_t pb = to_bf16(pv);
+  a.pbf[i] = pb;
+  const long o = i - off_w(1);
+  if (o >= 0 && o < NLAYER * (2L * H * H + H)) {
+    const long l = o / (2L * H * H + H), w = o % (2L * H * H + H);
+    if (w < 2L * H * H) a.pbfT[off_wt(static_cast<int>(l) + 1) + (w % (2 * H)) * H + w / (2 * H)] = pb;
+  }
+  const long ow = i - off_w(NLAYER + 1);
+  if (ow >= 0 && ow < static_cast<long>(a.P) * H)
+    a.pbfT[off_wt(NLAYER + 1) + (ow % H) * ((a.P + 15) / 16 * 16) + ow / H] = pb;
 }
 
-// Minibatch rows of the resident training set (graph i of step s's batch is
-// graph order[s batch + i] of the set: couplings, fields, targets), in the
-// replayable form of an epoch's step (one HIP graph for every step but the
-// first): the step index lives on the device, so the minibatch rows and
-// Adam's bias corrections of step s are read at launch, not baked into the
-// launch arguments.
-__global__ void k_gather_step(const float* __restrict__ sJ, const float* __restrict__ sh,
-                              const float* __restrict__ st, const uint32_t* __restrict__ order,
-                              const int* __restrict__ step, int batch, int L, int P, float* __restrict__ J,
-                              float* __restrict__ h, float* __restrict__ t) {
-  const int i = blockIdx.x;
-  const long src = order[static_cast<long>(*step) * batch + i];
-  for (int q = threadIdx.x; q < L * L; q += blockDim.x) J[static_cast<long>(i) * L * L + q] = sJ[src * L * L + q];
-  for (int q = threadIdx.x; q < L; q += blockDim.x) h[static_cast<long>(i) * L + q] = sh[src * L + q];
-  for (int q = threadIdx.x; q < P; q += blockDim.x) t[static_cast<long>(i) * P + q] = st[src * P + q];
-}
-
-// k_adam with this step's (b1^t, b2^t) from bt[2 step], bt[2 step + 1]
-__global__ void k_adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                            float* __restrict__ v, long n, float lr, const float* __restrict__ bt,
-                            const int* __restrict__ step) {
-  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int s = *step;
-  const float b1t = bt[2 * s], b2t = bt[2 * s + 1];
-  const float gi = g[i];
-  m[i] = 0.9f * m[i] + 0.1f * gi;
-  v[i] = 0.999f * v[i] + 0.001f * gi * gi;
-  const float mh = m[i] / (1.f - b1t), vh = v[i] / (1.f - b2t);
-  p[i] -= lr * mh / (sqrtf(vh) + 1e-8f);
-}
-
-__global__ void k_step_next(int* step) {
-  if (threadIdx.x == 0) *step += 1;
-}
-
-// acc += sum(sq[0..n)) in a fixed order (one block, strided partial sums,
-// then a tree), so the epoch loss does not depend on scheduling.
-__global__ __launch_bounds__(256) void k_loss_acc(const float* __restrict__ sq, long n, double* __restrict__ acc) {
+// acc += sum(sq[0..n)) in a fixed order, then the step counter moves on
+__global__ __launch_bounds__(256) void k_loss_step(const float* __restrict__ sq, long n, double* __restrict__ acc,
+                                                   int* __restrict__ step) {
   __shared__ double part[256];
   double s = 0.0;
   for (long i = threadIdx.x; i < n; i += 256) s += sq[i];
@@ -348,7 +813,10 @@ __global__ __launch_bounds__(256) void k_loss_acc(const float* __restrict__ sq, 
     if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *acc += part[0];
+  if (threadIdx.x == 0) {
+    *acc += part[0];
+    *step += 1;
+  }
 }
 
 inline unsigned nb(long n, int b = 256) { return static_cast<unsigned>((n + b - 1) / b); }
@@ -370,6 +838,8 @@ struct spx_gnn {
   float* m = nullptr;
   float* v = nullptr;
   uint16_t* pbf = nullptr; // bf16 copy of the parameters
+  uint16_t* pbfT = nullptr; // bf16 W_l^T (l = 1..3) and W_out^T, for the backward products
+  bool bf_stale = true;    // pbf / pbfT not yet written from the masters
   long step = 0;
   // activations (sized for the largest batch seen)
   long cap_rows = 0, cap_graphs = 0, cap_L = 0;
@@ -392,9 +862,10 @@ struct spx_gnn {
   size_t pin_graphs = 0, pin_L = 0;
   float *pin_J[2] = {}, *pin_h[2] = {}, *pin_y[2] = {};
   hipEvent_t ev[2] = {};
-  // split-K partial tiles of the products
-  float* gemm_part = nullptr;
-  size_t gemm_part_cap = 0;
+  // the training kernel's gradient partials and saved layer inputs
+  float* tpart = nullptr;
+  uint16_t* tcat = nullptr;
+  size_t tpart_cap = 0, tcat_cap = 0;
 };
 
 namespace {
@@ -409,36 +880,9 @@ size_t bsize(const spx_gnn* g, int l) { return l <= NLAYER ? H : g->P; }
 
 int gemm(spx_gnn* net, const uint16_t* a, long sam, long sak, const uint16_t* b, long sbk, long sbn, float* out,
          long scm, long scn, int M, int N, int K, const float* bias, float* preact, int relu, int acc) {
-  spx_ctx* c = net->ctx;
   GemmArgs g{a, sam, sak, b, sbk, sbn, out, scm, scn, M, N, K, bias, preact, relu, acc};
-  const int tiles = ((N + kBN - 1) / kBN) * ((M + kBM - 1) / kBM);
-  // Split K when the output alone would leave most CUs idle and K is long:
-  // that is the weight gradients, whose K is the batch's rows. The forward
-  // products (K = 16, 128, 256) never split, so a prediction does not
-  // depend on the batch it is computed in.
-  int splits = 1;
-  if (tiles < 128 && K >= 512) splits = std::min({32, K / (2 * kBK), std::max(1, 256 / tiles)});
-  int kchunk = (K + splits - 1) / splits;
-  kchunk = (kchunk + kBK - 1) / kBK * kBK;
-  splits = (K + kchunk - 1) / kchunk;
-  if (splits <= 1) {
-    k_gemm<<<dim3(static_cast<unsigned>((N + kBN - 1) / kBN), static_cast<unsigned>((M + kBM - 1) / kBM)), 256, 0,
-             c->stream>>>(g, nullptr, K > 0 ? K : 1);
-    SPX_HIP(hipGetLastError());
-    return SPX_OK;
-  }
-  const size_t need = static_cast<size_t>(splits) * M * N;
-  if (need > net->gemm_part_cap) {
-    SPX_HIP(hipStreamSynchronize(c->stream));
-    if (net->gemm_part) SPX_HIP(hipFree(net->gemm_part));
-    net->gemm_part = nullptr;
-    net->gemm_part_cap = 0;
-    SPX_HIP(hipMalloc(&net->gemm_part, need * sizeof(float)));
-    net->gemm_part_cap = need;
-  }
-  k_gemm<<<dim3(static_cast<unsigned>((N + kBN - 1) / kBN), static_cast<unsigned>((M + kBM - 1) / kBM),
-                static_cast<unsigned>(splits)), 256, 0, c->stream>>>(g, net->gemm_part, kchunk);
-  k_gemm_reduce<<<nb(static_cast<long>(M) * N), 256, 0, c->stream>>>(g, net->gemm_part, splits);
+  k_gemm<<<dim3(static_cast<unsigned>((N + kBN - 1) / kBN), static_cast<unsigned>((M + kBM - 1) / kBM)), 256, 0,
+           net->ctx->stream>>>(g);
   SPX_HIP(hipGetLastError());
   return SPX_OK;
 }
@@ -527,45 +971,103 @@ int forward(spx_gnn* g, long G, long L) {
   return SPX_OK;
 }
 
-int backward(spx_gnn* g, long G, long L, float* loss) {
-  spx_ctx* c = g->ctx;
-  const long R = G * L;
-  const int P = g->P;
-  k_mse<<<nb(G * P), 256, 0, c->stream>>>(g->y, g->tgt, g->dy, g->dybf, g->sq, G * P,
-                                          1.f / static_cast<float>(G * P));
-  float* gr = g->grad;
-  // readout
-  SPX_TRY(gemm(g, g->dybf, 1, P, g->poolbf, H, 1, gr + g->off_w[NLAYER + 1], H, 1, P, H, static_cast<int>(G),
-               nullptr, nullptr, 0, 0));
-  k_colsum<<<nb(P, 64), 1024, 0, c->stream>>>(g->dy, gr + g->off_b[NLAYER + 1], G, P);
-  SPX_TRY(gemm(g, g->dybf, P, 1, g->pbf + g->off_w[NLAYER + 1], H, 1, g->dpool, H, 1, static_cast<int>(G), H, P,
-               nullptr, nullptr, 0, 0));
-  k_pool_bwd<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(g->dpool, g->dx,
-                                                                                           static_cast<int>(L));
-  for (int l = NLAYER; l >= 0; --l) {
-    k_relu_bwd<<<nb(R * H), 256, 0, c->stream>>>(g->dx, g->z[l], g->dz, g->dzbf, R * H);
-    k_colsum<<<nb(H, 64), 1024, 0, c->stream>>>(g->dz, gr + g->off_b[l], R, H);
-    if (l == 0) {
-      SPX_TRY(gemm(g, g->dzbf, 1, H, g->feat, F, 1, gr + g->off_w[0], F, 1, H, F, static_cast<int>(R), nullptr,
-                   nullptr, 0, 0));
-      break;
-    }
-    SPX_TRY(gemm(g, g->dzbf, 1, H, g->cat[l], 2 * H, 1, gr + g->off_w[l], 2 * H, 1, H, 2 * H, static_cast<int>(R),
-                 nullptr, nullptr, 0, 0));
-    // d[x | m] = dz W_l; dx_{l-1} = first half + J^T (second half), J symmetric
-    SPX_TRY(gemm(g, g->dzbf, H, 1, g->pbf + g->off_w[l], 2 * H, 1, g->dcat, 2 * H, 1, static_cast<int>(R), 2 * H,
-                 H, nullptr, nullptr, 0, 0));
-    k_mix_bwd<<<dim3(static_cast<unsigned>(G), static_cast<unsigned>(L)), H, 0, c->stream>>>(
-        g->J, g->dcat, g->dx, static_cast<int>(L));
-  }
+int graphs_per_block(long L) { return static_cast<int>(std::min<long>(16, kTR / L)); }
+
+ReduceArgs reduce_args(const spx_gnn* g, int nwg) {
+  ReduceArgs r{};
+  r.part = g->tpart;
+  r.nwg = nwg;
+  r.nparam = static_cast<long>(g->nparam);
+  r.grad = g->grad;
+  r.p = g->p;
+  r.m = g->m;
+  r.v = g->v;
+  r.pbf = g->pbf;
+  r.pbfT = g->pbfT;
+  r.P = g->P;
+  return r;
+}
+
+// The bf16 parameters and transposed weights from the masters, when a
+// training step has not written them since the masters last changed.
+int refresh_bf16(spx_gnn* g) {
+  if (!g->bf_stale) return SPX_OK;
+  k_train_reduce<<<nb(static_cast<long>(g->nparam)), 256, 0, g->ctx->stream>>>(reduce_args(g, 0));
   SPX_HIP(hipGetLastError());
-  if (loss) {
-    std::vector<float> sq(G * P);
-    SPX_TRY(copy_to_host(c, sq.data(), g->sq, sq.size() * sizeof(float)));
-    double s = 0.0;
-    for (float v : sq) s += v;
-    *loss = static_cast<float>(s / static_cast<double>(G * P));
+  g->bf_stale = false;
+  return SPX_OK;
+}
+
+// One forward + backward pass over a batch of G graphs of L spins (k_train),
+// its gradient partials summed in workgroup order (k_train_reduce), with an
+// Adam step when adam != 0 (bias corrections b1t, b2t, or read from bt at the
+// device step counter when bt is given). The batch is J, h, t directly, or
+// rows order[*step G + i] of them.
+int train_pass(spx_gnn* g, long G, long L, const float* J, const float* h, const float* t, const uint32_t* order,
+               const int* step, int adam, float lr, float b1t, float b2t, const float* bt) {
+  spx_ctx* c = g->ctx;
+  const int gb = graphs_per_block(L);
+  const int nblk = static_cast<int>((G + gb - 1) / gb);
+  const int nwg = std::min(nblk, kTrainWG);
+  const size_t need_part = static_cast<size_t>(nwg) * g->nparam;
+  const size_t need_cat = static_cast<size_t>(nwg) * NLAYER * kTR * 2 * H;
+  if (need_part > g->tpart_cap || need_cat > g->tcat_cap) {
+    SPX_HIP(hipStreamSynchronize(c->stream));
+    for (void** q : {reinterpret_cast<void**>(&g->tpart), reinterpret_cast<void**>(&g->tcat)}) {
+      if (*q) SPX_HIP(hipFree(*q));
+      *q = nullptr;
+    }
+    g->tpart_cap = g->tcat_cap = 0;
+    SPX_HIP(hipMalloc(&g->tpart, need_part * sizeof(float)));
+    SPX_HIP(hipMalloc(&g->tcat, need_cat * sizeof(uint16_t)));
+    g->tpart_cap = need_part;
+    g->tcat_cap = need_cat;
   }
+  SPX_TRY(refresh_bf16(g));
+  TrainArgs a{};
+  a.J = J;
+  a.h = h;
+  a.t = t;
+  a.order = order;
+  a.step = step;
+  a.G = static_cast<int>(G);
+  a.L = static_cast<int>(L);
+  a.P = g->P;
+  a.gb = gb;
+  a.inv_sj = static_cast<float>(1.0 / g->sj);
+  a.inv_sh = static_cast<float>(1.0 / g->sh);
+  a.scale = 1.f / static_cast<float>(G * g->P);
+  a.p = g->p;
+  a.pbf = g->pbf;
+  a.pbfT = g->pbfT;
+  a.nparam = static_cast<long>(g->nparam);
+  a.part = g->tpart;
+  a.catg = g->tcat;
+  a.sq = g->sq;
+  for (int b0 = 0; b0 < nblk; b0 += kTrainWG) {
+    a.blk0 = b0;
+    k_train<<<static_cast<unsigned>(std::min(kTrainWG, nblk - b0)), kTW * 64, kTrainLds, c->stream>>>(a);
+    SPX_HIP(hipGetLastError());
+  }
+  ReduceArgs r = reduce_args(g, nwg);
+  r.adam = adam;
+  r.lr = lr;
+  r.b1t = b1t;
+  r.b2t = b2t;
+  r.bt = bt;
+  r.step = step;
+  k_train_reduce<<<nb(static_cast<long>(g->nparam)), 256, 0, c->stream>>>(r);
+  SPX_HIP(hipGetLastError());
+  return SPX_OK;
+}
+
+// The batch's mean squared error from the per-point terms, summed in order.
+int batch_loss(spx_gnn* g, long G, float* loss) {
+  std::vector<float> sq(G * g->P);
+  SPX_TRY(copy_to_host(g->ctx, sq.data(), g->sq, sq.size() * sizeof(float)));
+  double s = 0.0;
+  for (float v : sq) s += v;
+  *loss = static_cast<float>(s / static_cast<double>(G * g->P));
   return SPX_OK;
 }
 
@@ -612,6 +1114,14 @@ int spx_gnn_create(spx_ctx* ctx, size_t outputs, uint64_t seed, double scale_h, 
     off += bsize(g, l);
   }
   g->nparam = off;
+  // the device kernels compute the same offsets from the layout
+  for (int l = 0; l <= NLAYER + 1; ++l)
+    if (static_cast<long>(g->off_w[l]) != off_w(l) || static_cast<long>(g->off_b[l]) != off_b(l, g->P)) {
+      delete g;
+      return fail(SPX_EINVAL, "parameter layout mismatch");
+    }
+  // W_out^T rows padded to a multiple of 16 outputs (zero padding)
+  const long offT = off_wt(NLAYER + 1) + static_cast<long>(H) * ((g->P + 15) / 16 * 16);
   // He-uniform weights, zero biases, from a seeded host stream
   std::vector<float> p(off, 0.f);
   std::mt19937_64 rng(seed);
@@ -627,11 +1137,13 @@ int spx_gnn_create(spx_ctx* ctx, size_t outputs, uint64_t seed, double scale_h, 
   int rc = SPX_OK;
   if (hipMalloc(&g->p, off * sizeof(float)) != hipSuccess || hipMalloc(&g->grad, off * sizeof(float)) != hipSuccess ||
       hipMalloc(&g->m, off * sizeof(float)) != hipSuccess || hipMalloc(&g->v, off * sizeof(float)) != hipSuccess ||
-      hipMalloc(&g->pbf, off * sizeof(uint16_t)) != hipSuccess)
+      hipMalloc(&g->pbf, off * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc(&g->pbfT, static_cast<size_t>(offT) * sizeof(uint16_t)) != hipSuccess)
     rc = fail(SPX_ENOMEM, "hipMalloc of network parameters failed");
   else if (copy_to_device(g->ctx, g->p, p.data(), off * sizeof(float)) != SPX_OK ||
            hipMemset(g->m, 0, off * sizeof(float)) != hipSuccess ||
-           hipMemset(g->v, 0, off * sizeof(float)) != hipSuccess)
+           hipMemset(g->v, 0, off * sizeof(float)) != hipSuccess ||
+           hipMemset(g->pbfT, 0, static_cast<size_t>(offT) * sizeof(uint16_t)) != hipSuccess)
     rc = fail(SPX_EDEVICE, "initialising network parameters failed");
   for (int b = 0; b < 2 && rc == SPX_OK; ++b)
     if (hipEventCreateWithFlags(&g->ev[b], hipEventDisableTiming) != hipSuccess)
@@ -653,7 +1165,8 @@ int spx_gnn_destroy(spx_gnn* g) {
                   static_cast<void*>(g->dy), static_cast<void*>(g->sq), static_cast<void*>(g->dpool),
                   static_cast<void*>(g->poolbf), static_cast<void*>(g->dybf), static_cast<void*>(g->dzbf),
                   static_cast<void*>(g->dx), static_cast<void*>(g->dz), static_cast<void*>(g->dcat),
-                  static_cast<void*>(g->gemm_part), static_cast<void*>(g->d_step), static_cast<void*>(g->d_bt)})
+                  static_cast<void*>(g->pbfT), static_cast<void*>(g->tpart), static_cast<void*>(g->tcat),
+                  static_cast<void*>(g->d_step), static_cast<void*>(g->d_bt)})
     if (q) (void)hipFree(q);
   for (int l = 0; l <= NLAYER; ++l)
     for (void* q : {static_cast<void*>(g->z[l]), static_cast<void*>(g->x[l]), static_cast<void*>(g->cat[l])})
@@ -683,6 +1196,7 @@ int spx_gnn_get_params(const spx_gnn* g, float* host) {
 
 int spx_gnn_set_params(spx_gnn* g, const float* host) {
   if (!g || !host) return fail(SPX_EINVAL, "null argument");
+  g->bf_stale = true;
   return copy_to_device(g->ctx, g->p, host, g->nparam * sizeof(float));
 }
 
@@ -733,21 +1247,25 @@ int spx_gnn_gradient(spx_gnn* g, size_t G, size_t L, const double* J, const doub
   SPX_TRY(check_batch(g, G, L));
   if (!target) return fail(SPX_EINVAL, "null targets");
   SPX_TRY(load_batch(g, static_cast<long>(G), static_cast<long>(L), J, h, target));
-  SPX_TRY(forward(g, static_cast<long>(G), static_cast<long>(L)));
-  SPX_TRY(backward(g, static_cast<long>(G), static_cast<long>(L), loss));
+  SPX_TRY(train_pass(g, static_cast<long>(G), static_cast<long>(L), g->Jin, g->hin, g->tgt, nullptr, nullptr, 0, 0.f,
+                     0.f, 0.f, nullptr));
+  if (loss) SPX_TRY(batch_loss(g, static_cast<long>(G), loss));
   if (grad) SPX_TRY(copy_to_host(g->ctx, grad, g->grad, g->nparam * sizeof(float)));
   return SPX_OK;
 }
 
 int spx_gnn_train_step(spx_gnn* g, size_t G, size_t L, const double* J, const double* h, const float* target,
                        float lr, float* loss) {
-  SPX_TRY(spx_gnn_gradient(g, G, L, J, h, target, loss, nullptr));
-  ++g->step;
-  const float b1t = static_cast<float>(std::pow(0.9, static_cast<double>(g->step)));
-  const float b2t = static_cast<float>(std::pow(0.999, static_cast<double>(g->step)));
-  k_adam<<<nb(static_cast<long>(g->nparam)), 256, 0, g->ctx->stream>>>(g->p, g->grad, g->m, g->v,
-                                                                       static_cast<long>(g->nparam), lr, b1t, b2t);
-  SPX_HIP(hipGetLastError());
+  SPX_TRY(check_batch(g, G, L));
+  if (!target) return fail(SPX_EINVAL, "null targets");
+  SPX_TRY(load_batch(g, static_cast<long>(G), static_cast<long>(L), J, h, target));
+  const long t = g->step + 1;
+  const float b1t = static_cast<float>(std::pow(0.9, static_cast<double>(t)));
+  const float b2t = static_cast<float>(std::pow(0.999, static_cast<double>(t)));
+  SPX_TRY(train_pass(g, static_cast<long>(G), static_cast<long>(L), g->Jin, g->hin, g->tgt, nullptr, nullptr, 1, lr,
+                     b1t, b2t, nullptr));
+  g->step = t;
+  if (loss) SPX_TRY(batch_loss(g, static_cast<long>(G), loss));
   return SPX_OK;
 }
 
@@ -814,15 +1332,8 @@ int spx_gnn_train_epoch(spx_gnn* g, size_t batch, const uint32_t* order, float l
   SPX_TRY(copy_to_device(c, g->d_bt, bt.data(), bt.size() * sizeof(float)));
   SPX_HIP(hipMemsetAsync(g->d_step, 0, sizeof(int), c->stream));
   auto one_step = [&]() -> int {
-    k_gather_step<<<static_cast<unsigned>(G), 128, 0, c->stream>>>(
-        g->set_J, g->set_h, g->set_t, g->set_idx, g->d_step, static_cast<int>(G), static_cast<int>(L), g->P, g->Jin,
-        g->hin, g->tgt);
-    SPX_TRY(forward(g, G, L));
-    SPX_TRY(backward(g, G, L, nullptr));
-    k_loss_acc<<<1, 256, 0, c->stream>>>(g->sq, G * g->P, g->loss_acc);
-    k_adam_step<<<nb(static_cast<long>(g->nparam)), 256, 0, c->stream>>>(
-        g->p, g->grad, g->m, g->v, static_cast<long>(g->nparam), lr, g->d_bt, g->d_step);
-    k_step_next<<<1, 64, 0, c->stream>>>(g->d_step);
+    SPX_TRY(train_pass(g, G, L, g->set_J, g->set_h, g->set_t, g->set_idx, g->d_step, 1, lr, 0.f, 0.f, g->d_bt));
+    k_loss_step<<<1, 256, 0, c->stream>>>(g->sq, G * g->P, g->loss_acc, g->d_step);
     SPX_HIP(hipGetLastError());
     return SPX_OK;
   };
