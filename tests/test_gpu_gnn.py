@@ -116,6 +116,29 @@ def test_resident_epoch_equals_step_by_step_training(ctx):
         a.train_epoch(np.arange(n, dtype=np.uint32), n + 1, 1e-3)
 
 
+@pytest.mark.parametrize("G,L,P", [(3, 1, 5), (9, 30, 3), (700, 2, 40), (2100, 12, 4)])
+def test_gnn_gradient_block_shapes(ctx, G, L, P):
+    """The training kernel's edge shapes against the BF16-rounded oracle:
+    one spin per graph (16 graphs a block), 30 spins (2 a block), outputs
+    not a multiple of 8 and more than one 16-wide tile, and a batch of 420
+    blocks (four launches adding into the same rows of partial sums).
+    Bitwise repeatable."""
+    banner(f"gnn gradient G={G} L={L} P={P}")
+    sur = spx.Surrogate(ctx, P, seed=G + L, scale_h=0.5, scale_j=1.0)
+    p = sur.get_params().astype(np.float64)
+    J, h = random_graphs(G, L, G)
+    t = np.random.default_rng(L).normal(scale=0.3, size=(G, P))
+    loss, g = sur.gradient(J, h, t)
+    lb, gb = N.loss_and_grad(p, J, h, t, P, 0.5, 1.0, use_bf16=True)
+    assert abs(loss - lb) <= 2e-3 * lb
+    errb = block_errors(g.astype(np.float64), gb, P)
+    banner("grad rel err vs bf16 oracle: " + " ".join(f"{k}={v:.1e}" for k, v in errb.items()))
+    for k, e in errb.items():
+        assert e < 1e-2, (k, e)
+    loss2, g2 = sur.gradient(J, h, t)
+    assert loss2 == loss and np.array_equal(g, g2)
+
+
 def test_gnn_argument_errors(ctx):
     banner("gnn argument errors")
     with pytest.raises(spx.InvalidArgument):
