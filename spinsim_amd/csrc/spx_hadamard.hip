@@ -48,6 +48,7 @@
 // ulps of the amplitude scale).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <utility>
 
 #include "spx_common.hpp"
@@ -75,7 +76,26 @@ struct WhtArgs {
   double pr[kMaxMany], pi[kMaxMany], qr[kMaxMany], qi[kMaxMany];
   int defer[kMaxMany];          // accumulator batching (OrderWeights::defer)
   int next[kMaxMany];           // pass 3 also leaves the row transforms of its output
+  int xcd;                      // each vector's workgroups on one XCD (vector count a multiple of 8)
 };
+
+// Vector-per-XCD placement. The dispatcher hands the workgroups of these
+// launches (linear id w, x fastest) to XCD w mod 8 in turn, so when the
+// vector count is a multiple of 8 the item of workgroup w is taken as vector
+// (w mod 8) + 8 floor(w / 8 / per), item (w / 8) mod per, where per is a
+// vector's workgroup count: all of a vector's workgroups, in every pass, run
+// on one XCD, whose L2 keeps what one pass writes for the next. Off, or for
+// any other placement by the dispatcher, every item still runs exactly once.
+__device__ __forceinline__ void xcd_item(int on, uint32_t per, uint32_t& v, uint32_t& k) {
+  if (!on) {
+    v = blockIdx.y;
+    k = blockIdx.x;
+    return;
+  }
+  const uint32_t w = blockIdx.x + gridDim.x * blockIdx.y, q = w >> 3;
+  v = (w & 7u) + 8u * (q / per);
+  k = q % per;
+}
 
 // q += (cr + i ci) t, the one form every accumulator update takes
 __device__ __forceinline__ void caxpy(double2& q, double cr, double ci, double2 t) {
@@ -163,8 +183,10 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_fwd(WhtArgs a) {
   constexpr uint32_t T = 1u << kTB, NT = kRowThreads, H = NT / 2, M = T / NT;
   extern __shared__ double2 smem[];
   const uint32_t t = threadIdx.x;
-  const int v = blockIdx.y;
-  const size_t base = static_cast<size_t>(blockIdx.x) << kTB;
+  uint32_t vv, tile;
+  xcd_item(a.xcd, gridDim.x, vv, tile);
+  const int v = static_cast<int>(vv);
+  const size_t base = static_cast<size_t>(tile) << kTB;
   const double2* __restrict__ in = a.in[v];
   double2 x[M];
 #pragma unroll
@@ -269,14 +291,23 @@ __global__ __launch_bounds__(1 << G) __attribute__((amdgpu_waves_per_eu(G <= 8 ?
   constexpr int LT = G - 3;   // register bits of the forward's last round
   extern __shared__ double2 smem[];
   const uint32_t t = threadIdx.x, c = t & 7u, u = t >> 3;
-  const int v = static_cast<int>(blockIdx.y >> 1);
-  double2* __restrict__ X = (blockIdx.y & 1u) ? a.w[v] : a.u[v];
-  const uint32_t col = blockIdx.x * kCols + c;
+  // items: (vector, V half, column group); blockIdx.y = 2 vector + half
+  uint32_t vv, item;
+  if (a.xcd) {
+    xcd_item(1, 2u * gridDim.x, vv, item);
+  } else {
+    vv = blockIdx.y >> 1;
+    item = (blockIdx.y & 1u) * gridDim.x + blockIdx.x;
+  }
+  const int v = static_cast<int>(vv);
+  const bool vhalf = item >= gridDim.x;
+  double2* __restrict__ X = vhalf ? a.w[v] : a.u[v];
+  const uint32_t col = (vhalf ? item - gridDim.x : item) * kCols + c;
   double2 x[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) x[r] = X[(static_cast<size_t>(round_index<0>(u, r)) << kTB) | col];
   // A (or B) at the elements of the forward's last layout: rows u | r << LT
-  const double* __restrict__ tab = (blockIdx.y & 1u) ? a.eb[v] : a.e[v];
+  const double* __restrict__ tab = vhalf ? a.eb[v] : a.e[v];
   double e[8];
   {
     double part = tab[col];
@@ -313,8 +344,10 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   constexpr uint32_t T = 1u << kTB, NT = kRowThreads, H = NT / 2, M = T / NT;
   extern __shared__ double2 smem[];
   const uint32_t t = threadIdx.x;
-  const int v = blockIdx.y;
-  const size_t base = static_cast<size_t>(blockIdx.x) << kTB;
+  uint32_t vv, tile;
+  xcd_item(a.xcd, gridDim.x, vv, tile);
+  const int v = static_cast<int>(vv);
+  const size_t base = static_cast<size_t>(tile) << kTB;
   // read here, rewritten in place at the end when the next order is chained
   const double2* U = a.u[v];
   const double2* V = a.w[v];
@@ -322,7 +355,7 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   double2* __restrict__ out = a.out[v];
   double2* __restrict__ psi = a.psi[v];
   const double* __restrict__ dlo = a.dz[v];
-  const double* __restrict__ drow = a.dz[v] + T + static_cast<size_t>(blockIdx.x) * (kTB + 1);
+  const double* __restrict__ drow = a.dz[v] + T + static_cast<size_t>(tile) * (kTB + 1);
   const double sdiag = a.sdiag[v], a0 = a.a0[v], c1r = a.c1r[v], c1i = a.c1i[v], cr = a.cr[v], ci = a.ci[v];
   const int defer = KIND == K_ORDERN ? a.defer[v] : 0;   // uniform over the workgroup
   double2 xu[M], xw[M], xv[M], prev[M], pv[M];
@@ -466,6 +499,9 @@ void launch_bwd(hipStream_t st, const WhtArgs& a, dim3 grid) {
 
 }  // namespace
 
+// Largest size whose launches place each vector on one XCD by default.
+constexpr size_t kWhtXcdMaxSpins = 17;
+
 bool hadamard_eligible(size_t L) { return L >= kHadamardMinSpins && L <= kHadamardMaxSpins; }
 
 // The factors of A and B (see the file comment) from the weights at the
@@ -607,6 +643,16 @@ int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
   const size_t L = h0->nspins, n = static_cast<size_t>(1) << L;
   if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard launch out of range");
   WhtArgs a{};
+  {
+    // placement of each vector on one XCD: SPX_WHT_XCD = 0 off, 1 on, else
+    // on up to kWhtXcdMaxSpins
+    static const int forced = [] {
+      const char* e = std::getenv("SPX_WHT_XCD");
+      return e ? std::atoi(e) : -1;
+    }();
+    const bool want = forced >= 0 ? forced != 0 : L <= kWhtXcdMaxSpins;
+    a.xcd = want && nvec % 8 == 0 ? 1 : 0;
+  }
   for (unsigned i = 0; i < nvec; ++i) {
     const spx_ham* h = b.ham[i];
     if (!h || h->nspins != L || !h->wht_ok || !h->d_wht || b.slot[i] < 0 || b.slot[i] >= h->wht_slots)
