@@ -98,7 +98,7 @@ struct Res {
 // the reference's T_{n-1} side by side (for the gathers); at L = 11 and 12
 // two transpose buffers of 2^12 amplitudes (for the transforms).
 template <int L>
-constexpr int kLdsAmps = L >= 11 ? 8192 : 2 * Res<L>::N;
+constexpr int kLdsAmps = L >= 11 ? 8192 : L == 10 ? 4096 : 2 * Res<L>::N;
 
 // bit `b` of amplitude k = t + 512 i
 template <int L>
@@ -369,15 +369,19 @@ __device__ __forceinline__ void bfly(double2 (&v)[kA]) {
 // barrier. The lane index reaches the slot arithmetic through an empty asm,
 // so that the bases are recomputed at each transpose instead of held in
 // registers.
+// Threads that hold no vector (active false, whole wavefronts) take part in
+// the barrier only.
 template <int LB, int FROM, int TO>
-__device__ __forceinline__ void xpose(double2* S, uint32_t u, double2 (&v)[kA]) {
+__device__ __forceinline__ void xpose(double2* S, uint32_t u, double2 (&v)[kA], bool active) {
   unsigned uv = u;
   asm volatile("" : "+v"(uv));
+  if (active)
 #pragma unroll
-  for (unsigned j = 0; j < kA; ++j) S[at<LB, FROM>(uv, j)] = v[j];
+    for (unsigned j = 0; j < kA; ++j) S[at<LB, FROM>(uv, j)] = v[j];
   __syncthreads();
+  if (active)
 #pragma unroll
-  for (unsigned j = 0; j < kA; ++j) v[j] = S[at<LB, TO>(uv, j)];
+    for (unsigned j = 0; j < kA; ++j) v[j] = S[at<LB, TO>(uv, j)];
 }
 
 // register stages of the transform domain: bits 6-8, or 6-7 when bit 8 is
@@ -388,25 +392,28 @@ constexpr int kL3Stages = LB == 8 ? 2 : 3;
 // The forward and inverse transforms use the buffers S0, S1, S0 and S1,
 // S0, S1: every transpose alternates with the one before (a transform pair
 // is six transposes, so the next pair starts again with S0).
-template <int LB>
-__device__ __forceinline__ void fwd(double2* S0, double2* S1, uint32_t u, double2 (&v)[kA]) {
-  bfly<0, 3>(v);
-  xpose<LB, N, L1>(S0, u, v);
-  bfly<0, 3>(v);
-  xpose<LB, L1, L2>(S1, u, v);
-  bfly<0, 3>(v);
-  xpose<LB, L2, L3>(S0, u, v);
-  bfly<0, kL3Stages<LB>>(v);
+// NST: register stages of the natural layout (2 when its register bit 2 is
+// not an index bit of the transform, as for two L = 10 vectors held as one
+// 11-bit vector).
+template <int LB, int NST = 3>
+__device__ __forceinline__ void fwd(double2* S0, double2* S1, uint32_t u, double2 (&v)[kA], bool active = true) {
+  if (active) bfly<0, NST>(v);
+  xpose<LB, N, L1>(S0, u, v, active);
+  if (active) bfly<0, 3>(v);
+  xpose<LB, L1, L2>(S1, u, v, active);
+  if (active) bfly<0, 3>(v);
+  xpose<LB, L2, L3>(S0, u, v, active);
+  if (active) bfly<0, kL3Stages<LB>>(v);
 }
-template <int LB>
-__device__ __forceinline__ void inv(double2* S0, double2* S1, uint32_t u, double2 (&v)[kA]) {
-  bfly<0, kL3Stages<LB>>(v);
-  xpose<LB, L3, L2>(S1, u, v);
-  bfly<0, 3>(v);
-  xpose<LB, L2, L1>(S0, u, v);
-  bfly<0, 3>(v);
-  xpose<LB, L1, N>(S1, u, v);
-  bfly<0, 3>(v);
+template <int LB, int NST = 3>
+__device__ __forceinline__ void inv(double2* S0, double2* S1, uint32_t u, double2 (&v)[kA], bool active = true) {
+  if (active) bfly<0, kL3Stages<LB>>(v);
+  xpose<LB, L3, L2>(S1, u, v, active);
+  if (active) bfly<0, 3>(v);
+  xpose<LB, L2, L1>(S0, u, v, active);
+  if (active) bfly<0, 3>(v);
+  xpose<LB, L1, N>(S1, u, v, active);
+  if (active) bfly<0, NST>(v);
 }
 
 // v i^e (e mod 4): a swap and sign changes, exact
@@ -424,12 +431,22 @@ __device__ __forceinline__ double2 ipow(double2 v, unsigned e) {
 // The diagonal at the own amplitudes and A and B at the transform-domain
 // ones are formed once per evolve op from the step's weights; the step's
 // weights must be staged in lw, lm (stage_weights) and visible.
-template <int L>
+// LS < L (LS = 10, L = 11): the lane holds two LS-spin vectors as one
+// L-bit vector, index bit LS telling them apart (register bit 2 of the
+// natural layout), and only threads 0..255 hold it; the transforms skip that
+// bit, the tables never see it (their masks cover spins 0..LS-1), and
+// S = i^popcount gains a constant i on the second vector, which cancels in
+// S W B W S^+.
+template <int L, int LS = L>
 __device__ bool evolve_wht(double2 (&v)[wres::kA], double2* S0, double2* S1, double* red, const double2* lw,
                            const uint32_t* lm,
                            uint32_t u, const double* __restrict__ w, const DevEvolve& e, const DevOp& op) {
-  constexpr int LB = L - 3, A = static_cast<int>(wres::kA), NP = Res<L>::NP;
-  static_assert(L == 11 || L == 12, "the Walsh apply is laid out for L = 11 and 12");
+  constexpr int LB = L - 3, A = static_cast<int>(wres::kA), NP = Res<LS>::NP;
+  constexpr bool kTwo = LS < L;
+  constexpr int NST = kTwo ? 2 : 3;
+  static_assert((L == 11 || L == 12) && (LS == L || (L == 11 && LS == 10)),
+                "the Walsh apply is laid out for L = 11 and 12 (two L = 10 vectors as one of 11 bits)");
+  const bool active = !kTwo || threadIdx.x < (1u << LB);
   // runtime loops over pairs and spins outside, the eight amplitudes inside
   // (unrolled, so that the arrays stay in registers)
   double ea[A], eb[A], dg[A];
@@ -453,7 +470,7 @@ __device__ bool evolve_wht(double2 (&v)[wres::kA], double2* S0, double2* S1, dou
       eb[j] = fma(pb, z, eb[j]);
     }
   }
-  for (int sp = 0; sp < L; ++sp) {
+  for (int sp = 0; sp < LS; ++sp) {
     const double2 c = lw[NP + sp];
 #pragma unroll
     for (int j = 0; j < A; ++j) {
@@ -464,69 +481,77 @@ __device__ bool evolve_wht(double2 (&v)[wres::kA], double2* S0, double2* S1, dou
   }
   // the sigma_z diagonal in rebuildDiagonal's order (as build_diag)
   const double* zw = w + e.zdiag_off;
-  for (int sp = 0; sp < L; ++sp) {
+  for (int sp = 0; sp < LS; ++sp) {
     const double c = zw[sp];
     if (c == 0.0) continue;
 #pragma unroll
     for (int j = 0; j < A; ++j) dg[j] += ((kn[j] >> sp) & 1u) ? c : -c;
   }
-  for (int qa = 0, q = 0; qa < L; ++qa)
-    for (int qb = qa + 1; qb < L; ++qb, ++q) {
-      const double c = zw[L + q];
+  for (int qa = 0, q = 0; qa < LS; ++qa)
+    for (int qb = qa + 1; qb < LS; ++qb, ++q) {
+      const double c = zw[LS + q];
       if (c == 0.0) continue;
 #pragma unroll
       for (int j = 0; j < A; ++j) dg[j] += (((kn[j] >> qa) & 1u) == ((kn[j] >> qb) & 1u)) ? c : -c;
     }
 #pragma unroll
   for (int j = 0; j < A; ++j) {
-    ea[j] = ldexp(ea[j], -L);
-    eb[j] = ldexp(eb[j], -L);
+    ea[j] = ldexp(ea[j], -LS);
+    eb[j] = ldexp(eb[j], -LS);
   }
   const double* a = w + e.coef_off;
   const double scale = -2.0 / e.bound;
   const unsigned pcu = static_cast<unsigned>(__popc(u));
   double2 tprev[A], tcur[A], x[A];
   for (int step = 0; step < op.nsteps; ++step) {
+    if (active)
 #pragma unroll
-    for (int i = 0; i < A; ++i) {
-      tcur[i] = v[i];
-      v[i] = make_double2(v[i].x * a[0], v[i].y * a[0]);
-    }
+      for (int i = 0; i < A; ++i) {
+        tcur[i] = v[i];
+        v[i] = make_double2(v[i].x * a[0], v[i].y * a[0]);
+      }
     double wre = op.mode == 0 ? 0.0 : -1.0, wim = op.mode == 0 ? -1.0 : 0.0;
     for (int n = 1; n <= e.order; ++n) {
-#pragma unroll
-      for (int i = 0; i < A; ++i) {
-        const double f = scale * dg[i];
-        const double2 b = n >= 2 ? tprev[i] : make_double2(0.0, 0.0);
-        tprev[i] = make_double2(fma(f, tcur[i].x, -b.x), fma(f, tcur[i].y, -b.y));
-      }
       // W A W T_{n-1}
+      if (active) {
 #pragma unroll
-      for (int i = 0; i < A; ++i) x[i] = tcur[i];
-      wres::fwd<LB>(S0, S1, u, x);
+        for (int i = 0; i < A; ++i) {
+          const double f = scale * dg[i];
+          const double2 b = n >= 2 ? tprev[i] : make_double2(0.0, 0.0);
+          tprev[i] = make_double2(fma(f, tcur[i].x, -b.x), fma(f, tcur[i].y, -b.y));
+        }
 #pragma unroll
-      for (int j = 0; j < A; ++j) x[j] = make_double2(ea[j] * x[j].x, ea[j] * x[j].y);
-      wres::inv<LB>(S0, S1, u, x);
-#pragma unroll
-      for (int i = 0; i < A; ++i) tprev[i] = make_double2(tprev[i].x + x[i].x, tprev[i].y + x[i].y);
-      // S W B W S^+ T_{n-1}
-#pragma unroll
-      for (int i = 0; i < A; ++i) x[i] = wres::ipow(tcur[i], 0u - (pcu + static_cast<unsigned>(__popc(i))));
-      wres::fwd<LB>(S0, S1, u, x);
-#pragma unroll
-      for (int j = 0; j < A; ++j) x[j] = make_double2(eb[j] * x[j].x, eb[j] * x[j].y);
-      wres::inv<LB>(S0, S1, u, x);
-      const double cr = 2.0 * a[n] * wre, ci = 2.0 * a[n] * wim;
-#pragma unroll
-      for (int i = 0; i < A; ++i) {
-        const double2 sv = wres::ipow(x[i], pcu + static_cast<unsigned>(__popc(i)));
-        double2 tn = make_double2(tprev[i].x + sv.x, tprev[i].y + sv.y);
-        if (n == 1) tn = make_double2(0.5 * tn.x, 0.5 * tn.y);
-        v[i].x += cr * tn.x - ci * tn.y;
-        v[i].y += cr * tn.y + ci * tn.x;
-        tprev[i] = tcur[i];
-        tcur[i] = tn;
+        for (int i = 0; i < A; ++i) x[i] = tcur[i];
       }
+      wres::fwd<LB, NST>(S0, S1, u, x, active);
+      if (active)
+#pragma unroll
+        for (int j = 0; j < A; ++j) x[j] = make_double2(ea[j] * x[j].x, ea[j] * x[j].y);
+      wres::inv<LB, NST>(S0, S1, u, x, active);
+      // S W B W S^+ T_{n-1}
+      if (active) {
+#pragma unroll
+        for (int i = 0; i < A; ++i) tprev[i] = make_double2(tprev[i].x + x[i].x, tprev[i].y + x[i].y);
+#pragma unroll
+        for (int i = 0; i < A; ++i) x[i] = wres::ipow(tcur[i], 0u - (pcu + static_cast<unsigned>(__popc(i))));
+      }
+      wres::fwd<LB, NST>(S0, S1, u, x, active);
+      if (active)
+#pragma unroll
+        for (int j = 0; j < A; ++j) x[j] = make_double2(eb[j] * x[j].x, eb[j] * x[j].y);
+      wres::inv<LB, NST>(S0, S1, u, x, active);
+      const double cr = 2.0 * a[n] * wre, ci = 2.0 * a[n] * wim;
+      if (active)
+#pragma unroll
+        for (int i = 0; i < A; ++i) {
+          const double2 sv = wres::ipow(x[i], pcu + static_cast<unsigned>(__popc(i)));
+          double2 tn = make_double2(tprev[i].x + sv.x, tprev[i].y + sv.y);
+          if (n == 1) tn = make_double2(0.5 * tn.x, 0.5 * tn.y);
+          v[i].x += cr * tn.x - ci * tn.y;
+          v[i].y += cr * tn.y + ci * tn.x;
+          tprev[i] = tcur[i];
+          tcur[i] = tn;
+        }
       if (op.mode == 0) {
         const double pr = wre;
         wre = wim;
@@ -537,17 +562,26 @@ __device__ bool evolve_wht(double2 (&v)[wres::kA], double2* S0, double2* S1, dou
       }
     }
     if (op.autonorm) {
-      // each half's own norm (at L = 11 the halves hold different vectors)
-      double s = 0.0;
+      // each vector's own norm: at L = 11 the halves hold different vectors,
+      // with two L = 10 vectors registers 0..3 and 4..7
+      double s0 = 0.0, s1 = 0.0;
+      if (active)
 #pragma unroll
-      for (int i = 0; i < A; ++i) s += v[i].x * v[i].x + v[i].y * v[i].y;
-      const bool upper = LB == 8 && threadIdx.x >= (1u << LB);
-      const double2 nn = block_sum(red, upper ? 0.0 : s, upper ? s : 0.0);
-      const double nrm = sqrt(upper ? nn.y : nn.x);
-      if (nn.x == 0.0 || (LB == 8 && nn.y == 0.0)) return false;
-      const double iv = 1.0 / nrm;
+        for (int i = 0; i < A; ++i) {
+          const double q = v[i].x * v[i].x + v[i].y * v[i].y;
+          if (kTwo && i >= A / 2) s1 += q;
+          else s0 += q;
+        }
+      const bool upper = !kTwo && LB == 8 && threadIdx.x >= (1u << LB);
+      const double2 nn = block_sum(red, upper ? 0.0 : s0, upper ? s0 : s1);
+      if (nn.x == 0.0 || ((kTwo || LB == 8) && nn.y == 0.0)) return false;
+      const double iv0 = 1.0 / sqrt(upper ? nn.y : nn.x), iv1 = 1.0 / sqrt(nn.y);
+      if (active)
 #pragma unroll
-      for (int i = 0; i < A; ++i) v[i] = make_double2(v[i].x * iv, v[i].y * iv);
+        for (int i = 0; i < A; ++i) {
+          const double iv = kTwo && i >= A / 2 ? iv1 : iv0;
+          v[i] = make_double2(v[i].x * iv, v[i].y * iv);
+        }
     }
   }
   return true;
@@ -577,6 +611,36 @@ __device__ bool evolve_pair_wht(double2* st, double2* rf, double2* lds, double* 
 #pragma unroll
     for (int i = 0; i < 8; ++i) rf[i] = v[i];
     return true;
+  } else if constexpr (L == 10) {
+    // state and reference as one 11-bit vector (index bit 10: the
+    // reference) in threads 0..255, 8 amplitudes each; threads 256..511 join
+    // the barriers only
+    constexpr int N = Res<L>::N, A = Res<L>::A;   // 1024, 2
+    const uint32_t u = t & 255u;
+    const bool holder = t < 256u;
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      lds[t + kBT * i] = st[i];
+      lds[N + t + kBT * i] = rf[i];
+    }
+    __syncthreads();
+    double2 v[wres::kA];
+    if (holder)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = lds[u + 256 * j];
+    __syncthreads();
+    const bool ok = evolve_wht<11, 10>(v, lds, lds + 2 * N, red, lw, lm, u, w, e, op);
+    if (holder)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) lds[u + 256 * j] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+      st[i] = lds[t + kBT * i];
+      rf[i] = lds[N + t + kBT * i];
+    }
+    __syncthreads();
+    return ok;
   } else {
     constexpr int N = Res<L>::N, A = Res<L>::A;   // 2048, 4
     static_assert(L == 11 && A == 4, "half-workgroup vectors are laid out for L = 11");
@@ -768,7 +832,7 @@ __device__ void run_real(const RArgs& g, const DevReal& R, double2* lds, double*
       const DevEvolve e = g.evo[R.evolve_off + op.arg];
       if (e.order == 0 || op.nsteps == 0) continue;
       bool ok;
-      if constexpr (L >= 11) {
+      if constexpr (L >= 10) {
         ok = evolve_pair_wht<L>(st, rf, lds, red, lw, lm, t, g.w, e, op);
       } else {
         build_diag<L>(dg, t, g.w + e.zdiag_off);
@@ -858,7 +922,7 @@ __global__ __launch_bounds__(kBT) void k_resident_all(RArgs g) {
 }  // namespace
 
 size_t resident_lds_bytes(int L) {
-  const size_t amps = L >= 11 ? 8192 : static_cast<size_t>(2) << L;   // kLdsAmps<L>
+  const size_t amps = L >= 11 ? 8192 : L == 10 ? 4096 : static_cast<size_t>(2) << L;   // kLdsAmps<L>
   return sizeof(double2) * amps + sizeof(double) * 2 * kBT +
          sizeof(double2) * static_cast<size_t>(L * (L - 1) / 2 + L) +
          sizeof(uint32_t) * static_cast<size_t>(L * (L - 1) / 2);

@@ -440,6 +440,38 @@ def test_resident_batch_uneven_sizes_finite_pulses_match_oracle(ctx):
     assert np.array_equal(gv, gv2)
 
 
+def test_resident_batch_autonormalised_steps_match_oracle(ctx):
+    """Evolve steps with autonormalisation in the resident runner, in
+    imaginary and real time, at every walker (L = 8..12: pair gathers up to
+    9, two vectors held as one at 10, half-workgroup vectors at 11, the
+    whole workgroup at 12): rows within 1e-10 of the oracle's."""
+    banner("resident batch, autonormalised imaginary and real time at L = 8..12")
+    obs = ((O.OBS_CORR_X, 0), (O.OBS_NORM, 0), (O.OBS_ENERGY, 0))
+
+    def program(P):
+        P.measure()
+        P.evolve(0.1, 4, mode=1, autonorm=True)
+        P.measure()
+        P.evolve(0.05, 3, mode=0, autonorm=True)
+        P.measure()
+        return P
+
+    b = spx.Batch(ctx, program(spx.Program()), obs, epsilon=1e-7, reference_axis=0)
+    want = []
+    Ls = list(range(8, 13))
+    for i, L in enumerate(Ls):
+        fields, c, psi0 = realisation_inputs(L, 20240726, 40 + i)
+        b.add(L, fields, c.lo, c.hi, c.jx, c.jy, c.jz, psi0)
+        cfg = O.Config(L=L, program=program(O.Program()), observables=obs, field_disorder=0.5,
+                       min_separation=0.5, angular=True)
+        want.append(O.run_realization(cfg, 40 + i)[1])
+    _, gv = b.run()
+    for i, L in enumerate(Ls):
+        err = float(np.max(np.abs(gv[i] - want[i])))
+        assert err < 1e-10, (L, err)
+        assert np.allclose(gv[i][1:, 2:4], 1.0, atol=1e-12), L   # both norms after each step
+
+
 def test_resident_batch_full_xy8_program_matches_oracle(ctx):
     """The uneven-size batched workload's whole pulse program: XY-8 with
     finite pulses repeated 16 times (128 pulses), a measure before and after
