@@ -333,6 +333,11 @@ int batch_run_streaming(spx_batch* b, size_t first, size_t count, double* times,
     const char* e = std::getenv("SPX_BATCH_WORKERS");
     return e ? std::atoi(e) : 8;
   }();
+  static const size_t group_max = [] {
+    const char* e = std::getenv("SPX_BATCH_GROUP");
+    const int g = e ? std::atoi(e) : static_cast<int>(kGroup);
+    return static_cast<size_t>(std::max(1, std::min(g, static_cast<int>(kGroup))));
+  }();
   std::vector<size_t> idx(count);
   for (size_t i = 0; i < count; ++i) idx[i] = first + i;
   std::stable_sort(idx.begin(), idx.end(),
@@ -340,7 +345,7 @@ int batch_run_streaming(spx_batch* b, size_t first, size_t count, double* times,
   std::vector<std::pair<size_t, size_t>> groups;   // [begin, end) into idx
   for (size_t g = 0; g < count;) {
     size_t e = g + 1;
-    while (e < count && e - g < kGroup && b->items[idx[e]].nspins == b->items[idx[g]].nspins) ++e;
+    while (e < count && e - g < group_max && b->items[idx[e]].nspins == b->items[idx[g]].nspins) ++e;
     groups.emplace_back(g, e);
     g = e;
   }
@@ -483,7 +488,80 @@ int batch_run_resident(spx_batch* b, double* times, double* values) {
     ops.push_back(d);
   }
 
-  // per-realisation expansions and weights, built on the host
+  // per-realisation expansions and weights, built on host threads: each
+  // realisation's tables with offsets relative to its own block of weights
+  // (an evolve op whose fields, mode and tau equal an earlier one's shares
+  // that one's tables), then concatenated in index order, so the tables do
+  // not depend on the threads
+  struct Tables {
+    std::vector<DevEvolve> evo;   // one per evolve op, then the measure weights
+    std::vector<double> w;
+    double cost = 0.0;
+  };
+  std::vector<Tables> tables(n);
+  {
+    std::atomic<size_t> next{0};
+    std::atomic<int> status{SPX_OK};
+    std::string msg;
+    auto build = [&](size_t i) -> int {
+      const Realisation& r = b->items[i];
+      Tables& T = tables[i];
+      std::vector<std::vector<double>> seen_f;   // fields of the distinct evolve ops so far
+      std::vector<const ProgOp*> seen_op;
+      std::vector<size_t> seen_at;
+      for (const ProgOp* op : evolves) {
+        std::vector<double> tmp;
+        const std::vector<double>& f = step_fields(*op, r.fields, r.nspins, tmp);
+        size_t k = 0;
+        for (; k < seen_op.size(); ++k)
+          if (seen_op[k]->mode == op->mode && seen_op[k]->tau == op->tau && seen_f[k] == f) break;
+        if (k < seen_op.size()) {
+          const DevEvolve e = T.evo[seen_at[k]];
+          T.cost += static_cast<double>(e.order) * static_cast<double>(op->nsteps);
+          T.evo.push_back(e);
+          continue;
+        }
+        DevEvolve e{};
+        const double sb = bound_of(f, r);
+        std::vector<double> coeffs;
+        double eb = 1.0;
+        SPX_TRY(cheb_build(op->mode, sb, op->tau, b->epsilon, 6000, coeffs, eb));
+        e.bound = eb;
+        e.order = static_cast<int>(coeffs.size()) - 1;
+        e.coef_off = static_cast<int>(T.w.size());
+        T.w.insert(T.w.end(), coeffs.begin(), coeffs.end());
+        append_weights(T.w, f, r, e.order > 0 ? -2.0 / eb : 1.0, e);
+        T.cost += static_cast<double>(e.order) * static_cast<double>(op->nsteps);
+        seen_f.push_back(f);
+        seen_op.push_back(op);
+        seen_at.push_back(T.evo.size());
+        T.evo.push_back(e);
+      }
+      DevEvolve me{};
+      append_weights(T.w, r.fields, r, 1.0, me);
+      T.evo.push_back(me);
+      return SPX_OK;
+    };
+    auto work = [&] {
+      for (;;) {
+        const size_t i = next.fetch_add(1);
+        if (i >= n || status.load() != SPX_OK) return;
+        const int rc = build(i);
+        if (rc != SPX_OK) {
+          int ok = SPX_OK;
+          if (status.compare_exchange_strong(ok, rc)) msg = spx_last_error();
+          return;
+        }
+      }
+    };
+    const size_t threads = std::min<size_t>(std::max<size_t>(1, n / 64),
+                                            std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (status.load() != SPX_OK) return fail(status.load(), msg);
+  }
   std::vector<DevEvolve> evo;
   std::vector<DevReal> reals(n);
   std::vector<double> w, psi;
@@ -492,33 +570,27 @@ int batch_run_resident(spx_batch* b, double* times, double* values) {
   for (size_t i = 0; i < n; ++i) {
     const Realisation& r = b->items[i];
     DevReal& R = reals[i];
+    const Tables& T = tables[i];
+    if (w.size() + T.w.size() > static_cast<size_t>(0x7fffffff)) return fail(SPX_ENOMEM, "batch tables too large");
+    const int base = static_cast<int>(w.size());
     R.nspins = static_cast<int>(r.nspins);
     R.evolve_off = static_cast<int>(evo.size());
-    for (const ProgOp* op : evolves) {
-      std::vector<double> tmp;
-      const std::vector<double>& f = step_fields(*op, r.fields, r.nspins, tmp);
-      DevEvolve e{};
-      const double sb = bound_of(f, r);
-      std::vector<double> coeffs;
-      double eb = 1.0;
-      SPX_TRY(cheb_build(op->mode, sb, op->tau, b->epsilon, 6000, coeffs, eb));
-      e.bound = eb;
-      e.order = static_cast<int>(coeffs.size()) - 1;
-      e.coef_off = static_cast<int>(w.size());
-      w.insert(w.end(), coeffs.begin(), coeffs.end());
-      append_weights(w, f, r, e.order > 0 ? -2.0 / eb : 1.0, e);
-      cost[i] += static_cast<double>(e.order) * static_cast<double>(op->nsteps);
+    for (DevEvolve e : T.evo) {
+      e.coef_off += base;
+      e.pair_off += base;
+      e.field_off += base;
+      e.zdiag_off += base;
       evo.push_back(e);
     }
-    DevEvolve me{};
-    append_weights(w, r.fields, r, 1.0, me);
-    R.measure_off = static_cast<int>(evo.size());
-    evo.push_back(me);
+    R.measure_off = static_cast<int>(evo.size()) - 1;
+    w.insert(w.end(), T.w.begin(), T.w.end());
+    cost[i] = T.cost;
     R.psi_off = psi_off;
     psi_off += static_cast<long long>(static_cast<size_t>(2) << r.nspins);
     R.out_off = static_cast<long long>(i * points * b->width);
     cost[i] *= static_cast<double>(1u << r.nspins);
   }
+  tables.clear();
   // initial states into one buffer, drawn or copied on host threads
   psi.resize(static_cast<size_t>(psi_off));
   {
