@@ -105,7 +105,7 @@ def test_control_variate_at_l16_against_plain_monte_carlo(ctx):
     of the worst point's variance, and the control variate takes less device
     time."""
     res = E.control_variate(ctx, L=16, target=5e-3, draws=1_000_000, n_train=8192, n_val=256, epochs=30,
-                            log=banner)
+                            decay=0.95, log=banner)
     line = res.line
     banner(f"plain {line['plain']}; cv {line['control_variate']}; speedup {line['speedup']}; "
            f"VR {line['worst_point_variance_reduction']}; |z| {line['max_abs_z_cv_vs_plain']}")
