@@ -292,8 +292,19 @@ def main(argv):
         print(json.dumps(training_set(ctx, log=log)), flush=True)
     elif which == "control-variate":
         print(json.dumps(control_variate(ctx, log=log).line), flush=True)
+    elif which == "xy16-curves":
+        # exact XY-16 curves alone (the control variate's exact runs):
+        # L and count from the command line, after a warm-up of 16
+        L = int(argv[2]) if len(argv) > 2 else 16
+        count = int(argv[3]) if len(argv) > 3 else 256
+        P = xy16_program()
+        exact_curves(ctx, P, L, 0, 16)
+        _, dt = exact_curves(ctx, P, L, 1000, count)
+        print(json.dumps({"workload": f"{count} exact XY-16 curves at L={L}", "s": round(dt, 3),
+                          "realisations_per_s": round(count / dt, 1)}), flush=True)
     else:
-        raise SystemExit("usage: python -m spinsim_amd.ensembles training-set | control-variate")
+        raise SystemExit("usage: python -m spinsim_amd.ensembles training-set | control-variate | "
+                         "xy16-curves [L] [count]")
 
 
 if __name__ == "__main__":
