@@ -65,6 +65,21 @@ def test_uneven_orders_bitwise_equal_to_single_evolves(ctx, dense):
     assert all(n.startswith(prefix) for n in names), names
 
 
+@pytest.mark.parametrize("L,count", [(16, 8), (15, 16)])
+def test_xcd_placed_launches_bitwise_equal_to_single_evolves(ctx, L, count):
+    """Launches of 8 or 16 vectors place each vector's workgroups on one XCD
+    (up to L = 17); the launches narrow below 8 as the shorter expansions
+    finish. Every state equals its single evolve (one vector, no placement)
+    bit for bit, transverse fields included."""
+    banner(f"evolve_set L={L} dense=2, {count} states (XCD placement)")
+    scales = [1.0, 3.0, 0.5, 1.0, 2.0, 1e-6, 1.0, 0.7]
+    specs = [(k, scales[k % 8], k % 3 == 0) for k in range(count)]
+    orders, names = check_set(ctx, L, specs, 2, nsteps=2)
+    banner(f"orders {orders}")
+    assert len(set(orders)) >= 3
+    assert all(n.startswith("k_wht_") for n in names), names
+
+
 def test_chained_steps_bitwise_equal_to_single_evolves(ctx):
     """Several steps of one launch group on the Hadamard path: each state's
     last order leaves the row transforms of the finished state for the next
