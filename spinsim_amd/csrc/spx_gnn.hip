@@ -225,7 +225,8 @@ __global__ void k_pool(const float* __restrict__ x, float* __restrict__ p, int L
 // messages, pooling and gradients of the activations.
 
 constexpr int kTR = 64;             // rows of a graph block
-constexpr int kTW = 4;              // wavefronts of k_train
+constexpr int kTW = 8;              // wavefronts of k_train (two per SIMD)
+constexpr int kTT = kTW * 64;       // its threads
 constexpr int kTrainWG = 128;       // blocks per launch (the partial rows)
 // LDS row strides in bf16 elements: 4 (mod 8) dwords between rows, so the 16
 // rows of an MFMA operand fragment start on distinct bank quadruples
@@ -242,7 +243,8 @@ constexpr int kR4 = kTR * kSX * 4;                         // features, readout,
 constexpr int kRA = kTR * 30 * 4;                          // edge weights [graph][i][j], gb L L <= 64 L
 constexpr int kRM = (NLAYER + 1) * 32 * 4 * 8;             // ReLU masks: a ballot per 16x16 tile and row group
 constexpr int kRF = 16 * kSR * 2;                          // features transposed
-constexpr int kRS = 2 * H * 4;                             // half-column sums
+constexpr int kCG = kTT / H;                               // row groups of the column passes
+constexpr int kRS = kCG * H * 4;                           // their partial column sums
 constexpr int kRB = (NLAYER + 2) * H * 4;                  // biases b0..b_NLAYER, b_out
 constexpr int kTrainLds = kR1 + kR2 + kR3 + kR4 + kRA + kRM + kRF + kRS + kRB;
 static_assert(kTR * kSC * 2 <= kR1 && 64 * kSC * 2 <= kR2 && H * kSH * 2 <= kR2, "LDS regions");
@@ -278,25 +280,28 @@ __host__ __device__ constexpr long off_b(int l, int P) {
 }
 __host__ __device__ constexpr long off_wt(int l) { return static_cast<long>(l - 1) * 2 * H * H; }
 
-// C = A Bt^T over the wave's tiles: A [M][K] (row stride sa) and Bt [N][K]
-// (stride sb), bf16 in LDS. Wave w takes the N tiles w, w + kTW, ... and the
-// first mt (<= MT) M tiles; epi(i, r, row, col, v) is called for every
-// element in lockstep across the wave (uniform control), v plus cbias[col]
-// when a column bias is given. K = k <= KMAX, a
-// multiple of 16, walked in unrolled runs of four k-steps (so the next
-// fragments load while the MFMAs run).
-template <int MT, int KMAX, class Epi>
+// C = A Bt^T: A [M][K] (row stride sa) and Bt [N][K] (stride sb), bf16 in
+// LDS, mt M tiles and nt N tiles of 16. The work items (a group of up to MG
+// M tiles, one N tile) go to the waves in turn, so that small products
+// still occupy every wave; epi(i, r, row, col, v) (i: the M tile) is called
+// for every element in lockstep across the wave (uniform control), v plus
+// cbias[col] when a column bias is given. K = k <= KMAX, a multiple of 16,
+// walked in unrolled runs of four k-steps (so the next fragments load
+// while the MFMAs run).
+template <int MG, int KMAX, class Epi>
 __device__ __forceinline__ void tile_gemm(const uint16_t* A, int sa, const uint16_t* Bt, int sb, int mt, int nt,
                                           int k, Epi epi, const float* cbias = nullptr) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int fr = lane & 15, fk = (lane >> 4) * 4;
-  for (int tn = wave; tn < nt; tn += kTW) {
+  const int items = (mt + MG - 1) / MG * nt;
+  for (int it = wave; it < items; it += kTW) {
+    const int tn = it % nt, m0 = it / nt * MG;
     // the column's bias in a register: read in the epilogue, it would be
     // re-read after every store (the compiler cannot tell the arrays apart)
     const float cb = cbias ? cbias[tn * 16 + fr] : 0.f;
-    f32x4 acc[MT];
+    f32x4 acc[MG];
 #pragma unroll
-    for (int i = 0; i < MT; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < MG; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
     constexpr int KC = KMAX < 64 ? KMAX : 64;   // k-steps unrolled four at a time
 #pragma unroll 1
     for (int kc = 0; kc < KMAX; kc += KC) {
@@ -306,34 +311,38 @@ __device__ __forceinline__ void tile_gemm(const uint16_t* A, int sa, const uint1
         if (k0 >= k) break;
         const bf16x4 b = *reinterpret_cast<const bf16x4*>(Bt + (tn * 16 + fr) * sb + k0 + fk);
 #pragma unroll
-        for (int i = 0; i < MT; ++i) {
-          if (i >= mt) break;
-          const bf16x4 x = *reinterpret_cast<const bf16x4*>(A + (i * 16 + fr) * sa + k0 + fk);
+        for (int i = 0; i < MG; ++i) {
+          if (m0 + i >= mt) break;
+          const bf16x4 x = *reinterpret_cast<const bf16x4*>(A + ((m0 + i) * 16 + fr) * sa + k0 + fk);
           acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, b, acc[i], 0, 0, 0);
         }
       }
     }
 #pragma unroll
-    for (int i = 0; i < MT; ++i) {
-      if (i >= mt) break;
+    for (int i = 0; i < MG; ++i) {
+      if (m0 + i >= mt) break;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) epi(i, r, i * 16 + fk + r, tn * 16 + fr, cbias ? acc[i][r] + cb : acc[i][r]);
+      for (int r = 0; r < 4; ++r)
+        epi(m0 + i, r, (m0 + i) * 16 + fk + r, tn * 16 + fr, cbias ? acc[i][r] + cb : acc[i][r]);
     }
   }
 }
 
-// The same with B given as [K][N] (row stride sb): each lane gathers its
-// four consecutive k of one column with 16-bit reads (the 16 columns of a
-// fragment are consecutive halfwords, the four k groups 16 banks apart at
-// the strides used here).
-template <int MT, int K, class Epi>
-__device__ __forceinline__ void tile_gemm_kn(const uint16_t* A, int sa, const uint16_t* Bkn, int sb, int nt, Epi epi) {
+// The same with B given as [K][N] (row stride sb) and a compile-time K:
+// each lane gathers its four consecutive k of one column with 16-bit reads
+// (the 16 columns of a fragment are consecutive halfwords, the four k groups
+// 16 banks apart at the strides used here).
+template <int MG, int K, class Epi>
+__device__ __forceinline__ void tile_gemm_kn(const uint16_t* A, int sa, const uint16_t* Bkn, int sb, int mt, int nt,
+                                             Epi epi) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int fr = lane & 15, fk = (lane >> 4) * 4;
-  for (int tn = wave; tn < nt; tn += kTW) {
-    f32x4 acc[MT];
+  const int items = (mt + MG - 1) / MG * nt;
+  for (int it = wave; it < items; it += kTW) {
+    const int tn = it % nt, m0 = it / nt * MG;
+    f32x4 acc[MG];
 #pragma unroll
-    for (int i = 0; i < MT; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < MG; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k0 = 0; k0 < K; k0 += 16) {
       const uint16_t* bp = Bkn + (k0 + fk) * sb + tn * 16 + fr;
@@ -341,15 +350,18 @@ __device__ __forceinline__ void tile_gemm_kn(const uint16_t* A, int sa, const ui
 #pragma unroll
       for (int q = 0; q < 4; ++q) b[q] = static_cast<short>(bp[q * sb]);
 #pragma unroll
-      for (int i = 0; i < MT; ++i) {
-        const bf16x4 x = *reinterpret_cast<const bf16x4*>(A + (i * 16 + fr) * sa + k0 + fk);
+      for (int i = 0; i < MG; ++i) {
+        if (m0 + i >= mt) break;
+        const bf16x4 x = *reinterpret_cast<const bf16x4*>(A + ((m0 + i) * 16 + fr) * sa + k0 + fk);
         acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, b, acc[i], 0, 0, 0);
       }
     }
 #pragma unroll
-    for (int i = 0; i < MT; ++i)
+    for (int i = 0; i < MG; ++i) {
+      if (m0 + i >= mt) break;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) epi(i, r, i * 16 + fk + r, tn * 16 + fr, acc[i][r]);
+      for (int r = 0; r < 4; ++r) epi(m0 + i, r, (m0 + i) * 16 + fk + r, tn * 16 + fr, acc[i][r]);
+    }
   }
 }
 
@@ -358,11 +370,12 @@ __device__ __forceinline__ void tile_gemm_kn(const uint16_t* A, int sa, const ui
 // them to LDS. Issued right after a barrier, the loads run under the product
 // that precedes the staging's use. nrows rows of per 16-byte pieces
 // (nrows * per <= 2048); rows at or beyond `rows` are zero.
+constexpr int kStageIt = 2048 / kTT;   // 16-byte pieces per thread
 struct Staging {
-  uint4 v[8];
+  uint4 v[kStageIt];
   __device__ __forceinline__ void load(const uint16_t* __restrict__ src, long ss, int rows, int nrows, int per) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < kStageIt; ++q) {
       const int e = threadIdx.x + q * kTW * 64, r = e / per, c = (e % per) * 8;
       v[q] = make_uint4(0u, 0u, 0u, 0u);
       if (e < nrows * per && r < rows) v[q] = *reinterpret_cast<const uint4*>(src + r * ss + c);
@@ -370,7 +383,7 @@ struct Staging {
   }
   __device__ __forceinline__ void store(uint16_t* dst, int sd, int nrows, int per) const {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < kStageIt; ++q) {
       const int e = threadIdx.x + q * kTW * 64, r = e / per, c = (e % per) * 8;
       if (e < nrows * per) *reinterpret_cast<uint4*>(dst + r * sd + c) = v[q];
     }
@@ -381,21 +394,20 @@ __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
   return static_cast<uint32_t>(to_bf16(lo)) | (static_cast<uint32_t>(to_bf16(hi)) << 16);
 }
 
-// 32 FP32 values as bf16 into 64 bytes (16-byte aligned)
-__device__ __forceinline__ void store_bf16x32(uint16_t* dst, const float* v) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    reinterpret_cast<uint4*>(dst)[q] = make_uint4(pack2(v[8 * q], v[8 * q + 1]), pack2(v[8 * q + 2], v[8 * q + 3]),
-                                                  pack2(v[8 * q + 4], v[8 * q + 5]), pack2(v[8 * q + 6], v[8 * q + 7]));
-}
+// Elementwise passes over the block's 64 x 128 FP32 activations.
+// Row-chunk mapping: kTPR threads per row, lane q of a row takes the kRV
+// columns 4 kTPR k + 4 q + e (k < kRV / 4, e < 4) as v[4 k + e], so the
+// row's lanes touch 16 kTPR contiguous bytes at each k. Column mapping:
+// thread t takes column t mod H and the kCR rows from kCR (t / H).
+constexpr int kTPR = kTT / kTR, kRV = H / kTPR, kCR = kTR / kCG;
+static_assert(kRV % 4 == 0 && kCR % 8 == 0 && kTT % H == 0, "elementwise mappings");
 
-// Row-chunk passes: lane q4 (0..3) of a row takes its columns 16 k + 4 q4 + e
-// (k < 8, e < 4) as v[4 k + e], so the four lanes of a row touch 64
-// contiguous bytes at each k (FP32 rows of 128 columns).
-__device__ __forceinline__ void load_row32(float* v, const float* row, int q4) {
+__device__ __forceinline__ int row_col(int k, int q) { return 4 * kTPR * k + 4 * q; }
+
+__device__ __forceinline__ void load_row(float* v, const float* row, int q) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const float4 x = *reinterpret_cast<const float4*>(row + 16 * k + 4 * q4);
+  for (int k = 0; k < kRV / 4; ++k) {
+    const float4 x = *reinterpret_cast<const float4*>(row + row_col(k, q));
     v[4 * k] = x.x;
     v[4 * k + 1] = x.y;
     v[4 * k + 2] = x.z;
@@ -403,17 +415,25 @@ __device__ __forceinline__ void load_row32(float* v, const float* row, int q4) {
   }
 }
 
-__device__ __forceinline__ void store_row32(float* row, const float* v, int q4) {
+__device__ __forceinline__ void store_row(float* row, const float* v, int q) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    *reinterpret_cast<float4*>(row + 16 * k + 4 * q4) = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+  for (int k = 0; k < kRV / 4; ++k)
+    *reinterpret_cast<float4*>(row + row_col(k, q)) = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 }
 
-__device__ __forceinline__ void store_row32_bf16(uint16_t* row, const float* v, int q4) {
+__device__ __forceinline__ void store_row_bf16(uint16_t* row, const float* v, int q) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    *reinterpret_cast<uint2*>(row + 16 * k + 4 * q4) = make_uint2(pack2(v[4 * k], v[4 * k + 1]),
-                                                                  pack2(v[4 * k + 2], v[4 * k + 3]));
+  for (int k = 0; k < kRV / 4; ++k)
+    *reinterpret_cast<uint2*>(row + row_col(k, q)) = make_uint2(pack2(v[4 * k], v[4 * k + 1]),
+                                                                pack2(v[4 * k + 2], v[4 * k + 3]));
+}
+
+// kCR FP32 values as bf16 into 2 kCR bytes (16-byte aligned)
+__device__ __forceinline__ void store_col_bf16(uint16_t* dst, const float* v) {
+#pragma unroll
+  for (int q = 0; q < kCR / 8; ++q)
+    reinterpret_cast<uint4*>(dst)[q] = make_uint4(pack2(v[8 * q], v[8 * q + 1]), pack2(v[8 * q + 2], v[8 * q + 3]),
+                                                  pack2(v[8 * q + 4], v[8 * q + 5]), pack2(v[8 * q + 6], v[8 * q + 7]));
 }
 
 // ReLU masks: one ballot per 16 x 16 tile of a 64 x 128 activation and row
@@ -433,8 +453,8 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
   float* sA = reinterpret_cast<float*>(smem + kR1 + kR2 + kR3 + kR4);
   unsigned long long* mk = reinterpret_cast<unsigned long long*>(smem + kR1 + kR2 + kR3 + kR4 + kRA);
   uint16_t* featT = reinterpret_cast<uint16_t*>(smem + kR1 + kR2 + kR3 + kR4 + kRA + kRM);
-  float* half_sum = reinterpret_cast<float*>(smem + kR1 + kR2 + kR3 + kR4 + kRA + kRM + kRF);   // [2][H]
-  float* bias = half_sum + 2 * H;                                                                // [NLAYER + 2][H]
+  float* half_sum = reinterpret_cast<float*>(smem + kR1 + kR2 + kR3 + kR4 + kRA + kRM + kRF);   // [kCG][H]
+  float* bias = half_sum + kCG * H;                                                              // [NLAYER + 2][H]
   // sub-regions
   uint16_t* cat = R1;                                           // [kTR][kSC]
   uint16_t* dzr = R1;                                           // [kTR][kSH]
@@ -447,10 +467,10 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
   uint16_t* dyT = dyr + 16 * kSH;                               // [H][kSG]
   const int tid = threadIdx.x;
   const int L = a.L, P = a.P, Pt = (P + 15) / 16, sp = Pt * 16 + 8;
-  // row-chunk mapping of the elementwise passes: row tr, lane q4 of the row
-  const int tr = tid >> 2, q4 = tid & 3;
-  // column mapping: column cc, rows ch .. ch + 31
-  const int cc = tid & (H - 1), ch = (tid >> 7) * 32;
+  // row-chunk mapping of the elementwise passes: row tr, lane rq of the row
+  const int tr = tid / kTPR, rq = tid % kTPR;
+  // column mapping: column cc, rows ch .. ch + kCR - 1
+  const int cc = tid % H, ch = tid / H * kCR;
   float* part = a.part + static_cast<long>(blockIdx.x) * a.nparam;
   uint16_t* catg = a.catg + static_cast<long>(blockIdx.x) * NLAYER * kTR * 2 * H;
   const long base = a.order ? static_cast<long>(*a.step) * a.G : 0;
@@ -471,34 +491,40 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
     // gradient (column mapping); dx in R3.
     auto relu_bwd = [&](const unsigned long long* ml, bool want_rows, long bias_off) {
       if (want_rows) {
-        float v[32];
-        load_row32(v, R3 + tr * kSX, q4);
+        float v[kRV];
+        load_row(v, R3 + tr * kSX, rq);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const uint32_t m = tr < R ? mask_bits16(ml, tr, 16 * k) >> (4 * q4) : 0u;
+        for (int k = 0; k < kRV / 4; ++k) {
+          const int col = row_col(k, rq);
+          const uint32_t m = tr < R ? mask_bits16(ml, tr, col) >> (col & 15) : 0u;
 #pragma unroll
           for (int e = 0; e < 4; ++e)
             if (!((m >> e) & 1u)) v[4 * k + e] = 0.f;
         }
-        store_row32_bf16(dzr + tr * kSH, v, q4);
+        store_row_bf16(dzr + tr * kSH, v, rq);
       }
       // loads first, unconditionally: no read waits on another
-      float v[32], s = 0.f;
-      uint32_t mb[32];
+      float v[kCR], s = 0.f;
+      uint32_t mb[kCR];
 #pragma unroll
-      for (int q = 0; q < 32; ++q) {
+      for (int q = 0; q < kCR; ++q) {
         v[q] = R3[(ch + q) * kSX + cc];
         mb[q] = mask_bits16(ml, ch + q, cc);
       }
 #pragma unroll
-      for (int q = 0; q < 32; ++q) {
+      for (int q = 0; q < kCR; ++q) {
         if (!(ch + q < R && ((mb[q] >> (cc & 15)) & 1u))) v[q] = 0.f;
         s += v[q];
       }
-      store_bf16x32(dzT + cc * kSR + ch, v);
-      half_sum[(ch >> 5) * H + cc] = s;
+      store_col_bf16(dzT + cc * kSR + ch, v);
+      half_sum[(ch / kCR) * H + cc] = s;
       __syncthreads();
-      if (tid < H) put(bias_off + tid, half_sum[tid] + half_sum[H + tid]);
+      if (tid < H) {
+        float b = half_sum[tid];
+#pragma unroll
+        for (int g = 1; g < kCG; ++g) b += half_sum[g * H + tid];
+        put(bias_off + tid, b);
+      }
     };
     Staging st;
     st.load(a.pbf + off_w(0), F, H, H, F / 8);    // W0 [c][f]
@@ -555,38 +581,38 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
     // 3. layers 1..NLAYER: cat = [x | A x] in bf16, x = relu(cat W_l^T + b_l)
     for (int l = 1; l <= NLAYER; ++l) {
       {
-        float x[32], m[32];
-        load_row32(x, R3 + tr * kSX, q4);
+        float x[kRV], m[kRV];
+        load_row(x, R3 + tr * kSX, rq);
 #pragma unroll
-        for (int q = 0; q < 32; ++q) m[q] = 0.f;
+        for (int q = 0; q < kRV; ++q) m[q] = 0.f;
         if (tr < R) {
           const int gl = tr / L, i = tr % L;
           const float* Ar = sA + (gl * L + i) * L;
           int j = 0;
           for (; j + 1 < L; j += 2) {
             const float w0 = Ar[j], w1 = Ar[j + 1];
-            float y0[32], y1[32];
-            load_row32(y0, R3 + (gl * L + j) * kSX, q4);
-            load_row32(y1, R3 + (gl * L + j + 1) * kSX, q4);
+            float y0[kRV], y1[kRV];
+            load_row(y0, R3 + (gl * L + j) * kSX, rq);
+            load_row(y1, R3 + (gl * L + j + 1) * kSX, rq);
 #pragma unroll
-            for (int q = 0; q < 32; ++q) m[q] = (m[q] + w0 * y0[q]) + w1 * y1[q];
+            for (int q = 0; q < kRV; ++q) m[q] = (m[q] + w0 * y0[q]) + w1 * y1[q];
           }
           if (j < L) {
             const float w = Ar[j];
-            float y[32];
-            load_row32(y, R3 + (gl * L + j) * kSX, q4);
+            float y[kRV];
+            load_row(y, R3 + (gl * L + j) * kSX, rq);
 #pragma unroll
-            for (int q = 0; q < 32; ++q) m[q] += w * y[q];
+            for (int q = 0; q < kRV; ++q) m[q] += w * y[q];
           }
         } else {
 #pragma unroll
-          for (int q = 0; q < 32; ++q) x[q] = 0.f;
+          for (int q = 0; q < kRV; ++q) x[q] = 0.f;
         }
         uint16_t* cg = catg + static_cast<long>(l - 1) * kTR * 2 * H + tr * 2 * H;
-        store_row32_bf16(cat + tr * kSC, x, q4);
-        store_row32_bf16(cat + tr * kSC + H, m, q4);
-        store_row32_bf16(cg, x, q4);
-        store_row32_bf16(cg + H, m, q4);
+        store_row_bf16(cat + tr * kSC, x, rq);
+        store_row_bf16(cat + tr * kSC + H, m, rq);
+        store_row_bf16(cg, x, rq);
+        store_row_bf16(cg + H, m, rq);
       }
       st.store(R2, kSC, 64, 2 * H / 8);
       const float* b = bias + l * H;
@@ -598,7 +624,7 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
           st.load(a.pbf + off_w(l + 1), 2 * H, 64, 64, 2 * H / 8);
         else
           st.load(a.pbf + off_w(NLAYER + 1), H, P, Pt * 16, H / 8);    // W_out [q][c]
-        tile_gemm<4, 2 * H>(cat, kSC, R2, kSC, 4, 4, 2 * H, [&](int i, int r, int row, int c0, float z) {
+        tile_gemm<2, 2 * H>(cat, kSC, R2, kSC, 4, 4, 2 * H, [&](int i, int r, int row, int c0, float z) {
           const int col = half * 64 + c0;
           const unsigned long long w = __ballot(z > 0.f);
           if ((threadIdx.x & 63) == 0) mk[((l * 32) + i * 8 + (col >> 4)) * 4 + r] = w;
@@ -613,20 +639,24 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
     float* tg = reinterpret_cast<float*>(R1);                     // [16][H]
     for (int e = tid; e < ng * P; e += kTW * 64) tg[(e / P) * H + e % P] = a.t[src_of(e / P) * P + e % P];
     {
-      const int gl = tid >> 4, c0 = (tid & 15) * 8;
-      float s[8];
+      // 16 graphs x H columns over the threads, kPC columns each
+      constexpr int kPC = 16 * H / kTT;
+      static_assert(kPC == 4, "one float4 of a row per thread");
+      const int gl = tid / (H / kPC), c0 = tid % (H / kPC) * kPC;
+      float s[kPC];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) s[q] = 0.f;
+      for (int q = 0; q < kPC; ++q) s[q] = 0.f;
       if (gl < ng) {
         for (int i = 0; i < L; ++i) {
-          const float4* xr = reinterpret_cast<const float4*>(R3 + (gl * L + i) * kSX + c0);
-          const float4 u = xr[0], w = xr[1];
-          s[0] += u.x; s[1] += u.y; s[2] += u.z; s[3] += u.w;
-          s[4] += w.x; s[5] += w.y; s[6] += w.z; s[7] += w.w;
+          const float4 u = *reinterpret_cast<const float4*>(R3 + (gl * L + i) * kSX + c0);
+          s[0] += u.x;
+          s[1] += u.y;
+          s[2] += u.z;
+          s[3] += u.w;
         }
       }
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
+      for (int q = 0; q < kPC; ++q) {
         const uint16_t v = gl < ng ? to_bf16(s[q] / static_cast<float>(L)) : uint16_t(0);
         poolr[gl * kSH + c0 + q] = v;
         poolT[(c0 + q) * kSG + gl] = v;
@@ -675,7 +705,7 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
       st.store(R2, kSC, kTR, 2 * H / 8);               // the layer input, [row][j]
       __syncthreads();
       st.load(a.pbfT + off_wt(l) + static_cast<long>(H) * H, H, H, H, H / 8);
-      tile_gemm_kn<8, kTR>(dzT, kSR, R2, kSC, 2 * H / 16, [&](int, int, int c, int j, float v) {
+      tile_gemm_kn<8, kTR>(dzT, kSR, R2, kSC, 8, 2 * H / 16, [&](int, int, int c, int j, float v) {
         put(off_w(l) + static_cast<long>(c) * 2 * H + j, v);
       });
       // d[x | m] = dz W_l, the message half first
@@ -696,43 +726,43 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
       __syncthreads();
       // dx = d[x] + A^T d[m] (A symmetric), rows beyond the block zero
       {
-        float v[32];
-        load_row32(v, R3 + tr * kSX, q4);
+        float v[kRV];
+        load_row(v, R3 + tr * kSX, rq);
         if (tr < R) {
           const int gl = tr / L, i = tr % L;
           const float* Ar = sA + (gl * L + i) * L;
-          float s[32];
+          float s[kRV];
 #pragma unroll
-          for (int q = 0; q < 32; ++q) s[q] = 0.f;
+          for (int q = 0; q < kRV; ++q) s[q] = 0.f;
           int j = 0;
           for (; j + 1 < L; j += 2) {
             const float w0 = Ar[j], w1 = Ar[j + 1];
-            float y0[32], y1[32];
-            load_row32(y0, R4 + (gl * L + j) * kSX, q4);
-            load_row32(y1, R4 + (gl * L + j + 1) * kSX, q4);
+            float y0[kRV], y1[kRV];
+            load_row(y0, R4 + (gl * L + j) * kSX, rq);
+            load_row(y1, R4 + (gl * L + j + 1) * kSX, rq);
 #pragma unroll
-            for (int q = 0; q < 32; ++q) s[q] = (s[q] + w0 * y0[q]) + w1 * y1[q];
+            for (int q = 0; q < kRV; ++q) s[q] = (s[q] + w0 * y0[q]) + w1 * y1[q];
           }
           if (j < L) {
             const float w = Ar[j];
-            float y[32];
-            load_row32(y, R4 + (gl * L + j) * kSX, q4);
+            float y[kRV];
+            load_row(y, R4 + (gl * L + j) * kSX, rq);
 #pragma unroll
-            for (int q = 0; q < 32; ++q) s[q] += w * y[q];
+            for (int q = 0; q < kRV; ++q) s[q] += w * y[q];
           }
 #pragma unroll
-          for (int q = 0; q < 32; ++q) v[q] += s[q];
+          for (int q = 0; q < kRV; ++q) v[q] += s[q];
         } else {
 #pragma unroll
-          for (int q = 0; q < 32; ++q) v[q] = 0.f;
+          for (int q = 0; q < kRV; ++q) v[q] = 0.f;
         }
-        store_row32(R3 + tr * kSX, v, q4);
+        store_row(R3 + tr * kSX, v, rq);
       }
       __syncthreads();
     }
     // 6. layer 0
     relu_bwd(mk, false, off_b(0, P));
-    tile_gemm<8, kTR>(dzT, kSR, featT, kSR, 8, 1, kTR, [&](int, int, int c, int f, float v) {
+    tile_gemm<1, kTR>(dzT, kSR, featT, kSR, 8, 1, kTR, [&](int, int, int c, int f, float v) {
       put(off_w(0) + static_cast<long>(c) * F + f, v);
     });
     __syncthreads();
