@@ -171,7 +171,7 @@ class CVResult:
 
 
 def control_variate(ctx: spx.Context, L: int = 16, target: float = 1e-3, draws: int = 1_000_000,
-                    L_train: int = 12, n_train: int = 32768, n_val: int = 256, epochs: int = 100,
+                    L_train: int = 10, n_train: int = 65536, n_val: int = 256, epochs: int = 60,
                     decay: float | None = None, seed: int = SEED, log=lambda m: None) -> CVResult:
     """Seconds to a +-target error bar on every point of the mean XY-16 echo
     at L: plain batched Monte-Carlo against the control variate (the
@@ -179,14 +179,18 @@ def control_variate(ctx: spx.Context, L: int = 16, target: float = 1e-3, draws: 
     arms actually run to the bar.
 
     The surrogate is trained on exact curves of the same sequence at the
-    smaller L_train (the LDS-resident runner: about 4.8 s for 32768 curves);
-    the network is size-agnostic and the short-time echo is set by local
-    geometry. The learning rate falls by `decay` after every epoch, by
-    default to an eighth of its start over the epochs. The training budget
-    trades against the exact runs: on 512 held-out L = 16 curves the
-    worst point's variance reduction was 7.3 after 40 epochs on 16384
-    curves, 13.6 after 100, 19.3 after 80 epochs on 32768 and 23.7 after
-    120, and the exact runs the bar needs scale as its inverse. beta is fitted on n_val held-out exact curves at L (its error
+    smaller L_train (the LDS-resident runner: about 2.4 s for 65536 curves
+    at L = 10); the network is size-agnostic and the short-time echo is set
+    by local geometry. The learning rate falls by `decay` after every epoch,
+    by default to an eighth of its start over the epochs. The training set
+    trades against the exact runs, which scale as the inverse of the worst
+    point's variance reduction. On 1024 held-out L = 16 curves (one
+    training run each): L = 12, 32768 curves x 100 epochs 29.7 (curves
+    3.1 s, training 1.95 s); L = 11, 65536 x 60 33.8; L = 10, 65536 x 60
+    35.2 (2.4 s, 2.3 s), x 80 36.2, 98304 x 40 34.9, 131072 x 30 36.1;
+    L = 9, 131072 x 30 32.7. With fewer curves the smaller sizes fall
+    behind: L = 10, 32768 x 100 reached 17-19 on 512 held-out curves,
+    where L = 11 reached 30 and L = 12 31-37. beta is fitted on n_val held-out exact curves at L (its error
     adds a fraction of about 1 / n_val to the residual variance), whose
     variances also give both arms their first sample sizes; each arm then
     runs further realisations until its own standard error is at most

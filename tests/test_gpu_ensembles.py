@@ -104,7 +104,7 @@ def test_control_variate_at_l16_against_plain_monte_carlo(ctx):
     itself up until its own standard error does), the surrogate removes most
     of the worst point's variance, and the control variate takes less device
     time."""
-    res = E.control_variate(ctx, L=16, target=5e-3, draws=1_000_000, n_train=8192, n_val=256, epochs=30,
+    res = E.control_variate(ctx, L=16, target=5e-3, draws=1_000_000, L_train=12, n_train=8192, n_val=256, epochs=30,
                             decay=0.95, log=banner)
     line = res.line
     banner(f"plain {line['plain']}; cv {line['control_variate']}; speedup {line['speedup']}; "
