@@ -213,7 +213,8 @@ __global__ void k_pool(const float* __restrict__ x, float* __restrict__ p, int L
 // ---------------------------------------------------------------------------
 // One training step in two launches. k_train runs the forward pass, the loss
 // and the backward pass of whole graphs: a workgroup takes a block of at most
-// kTR rows (gb graphs of L spins, zero rows after them), keeps every
+// kTR rows (gb graphs of L spins, zero rows after them; the products cover
+// the M tiles that hold the block's rows), keeps every
 // activation of the block in LDS, and runs each product on MFMA from LDS
 // operands, so the ~40 dependent launches of the layer-by-layer path become
 // one. Its weight and bias gradients are partial sums over the block's rows,
@@ -482,6 +483,7 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
   {
     const int blk = a.blk0 + static_cast<int>(blockIdx.x);
     const int g0 = blk * a.gb, ng = min(a.gb, a.G - g0), R = ng * L;
+    const int mtr = (R + 15) / 16;   // M tiles holding the block's rows
     auto src_of = [&](int gl) -> long {
       const long gi = g0 + gl;
       return a.order ? static_cast<long>(a.order[base + gi]) : gi;
@@ -571,7 +573,7 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
     // 2. layer 0: x = relu(feat W0^T + b0)
     st.load(a.pbf + off_w(1), 2 * H, 64, 64, 2 * H / 8);
     {
-      tile_gemm<4, F>(feat, kSG, R2, kSG, 4, H / 16, F, [&](int i, int r, int row, int col, float z) {
+      tile_gemm<4, F>(feat, kSG, R2, kSG, mtr, H / 16, F, [&](int i, int r, int row, int col, float z) {
         const unsigned long long w = __ballot(z > 0.f);
         if ((threadIdx.x & 63) == 0) mk[(i * 8 + (col >> 4)) * 4 + r] = w;
         R3[row * kSX + col] = row < R ? fmaxf(z, 0.f) : 0.f;
@@ -624,12 +626,15 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
           st.load(a.pbf + off_w(l + 1), 2 * H, 64, 64, 2 * H / 8);
         else
           st.load(a.pbf + off_w(NLAYER + 1), H, P, Pt * 16, H / 8);    // W_out [q][c]
-        tile_gemm<2, 2 * H>(cat, kSC, R2, kSC, 4, 4, 2 * H, [&](int i, int r, int row, int c0, float z) {
+        auto epi = [&](int i, int r, int row, int c0, float z) {
           const int col = half * 64 + c0;
           const unsigned long long w = __ballot(z > 0.f);
           if ((threadIdx.x & 63) == 0) mk[((l * 32) + i * 8 + (col >> 4)) * 4 + r] = w;
           R3[row * kSX + col] = row < R ? fmaxf(z, 0.f) : 0.f;
-        }, b + half * 64);
+        };
+        // up to 32 rows: one M tile per work item, so that every wave has one
+        if (mtr > 2) tile_gemm<2, 2 * H>(cat, kSC, R2, kSC, mtr, 4, 2 * H, epi, b + half * 64);
+        else tile_gemm<1, 2 * H>(cat, kSC, R2, kSC, mtr, 4, 2 * H, epi, b + half * 64);
         __syncthreads();
         if (half == 0) st.store(R2, kSC, 64, 2 * H / 8);
       }
@@ -713,14 +718,14 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
       st.store(R2, kSH, H, H / 8);
       __syncthreads();
       st.load(a.pbfT + off_wt(l), H, H, H, H / 8);
-      tile_gemm<4, H>(dzr, kSH, R2, kSH, 4, H / 16, H, [&](int, int, int r, int c, float v) {
+      tile_gemm<4, H>(dzr, kSH, R2, kSH, mtr, H / 16, H, [&](int, int, int r, int c, float v) {
         R4[r * kSX + c] = v;
       });
       __syncthreads();
       st.store(R2, kSH, H, H / 8);
       __syncthreads();
       if (l > 1) st.load(catg + static_cast<long>(l - 2) * kTR * 2 * H, 2 * H, kTR, kTR, 2 * H / 8);
-      tile_gemm<4, H>(dzr, kSH, R2, kSH, 4, H / 16, H, [&](int, int, int r, int c, float v) {
+      tile_gemm<4, H>(dzr, kSH, R2, kSH, mtr, H / 16, H, [&](int, int, int r, int c, float v) {
         R3[r * kSX + c] = v;
       });
       __syncthreads();
@@ -762,7 +767,7 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
     }
     // 6. layer 0
     relu_bwd(mk, false, off_b(0, P));
-    tile_gemm<1, kTR>(dzT, kSR, featT, kSR, 8, 1, kTR, [&](int, int, int c, int f, float v) {
+    tile_gemm<1, kTR>(dzT, kSR, featT, kSR, 8, 1, 16 * mtr, [&](int, int, int c, int f, float v) {
       put(off_w(0) + static_cast<long>(c) * F + f, v);
     });
     __syncthreads();
@@ -1001,7 +1006,11 @@ int forward(spx_gnn* g, long G, long L) {
   return SPX_OK;
 }
 
-int graphs_per_block(long L) { return static_cast<int>(std::min<long>(16, kTR / L)); }
+// Graphs per block of k_train: whole graphs up to 32 rows (one graph when it
+// is larger), so that the forward products give every wave one M tile.
+// Measured on 32768-graph epochs (80 -> 73 us per step at L = 10 and 12)
+// against filling the 64 rows.
+int graphs_per_block(long L) { return static_cast<int>(std::max<long>(1, std::min<long>(16, 32 / L))); }
 
 ReduceArgs reduce_args(const spx_gnn* g, int nwg) {
   ReduceArgs r{};
