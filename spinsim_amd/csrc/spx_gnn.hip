@@ -329,13 +329,13 @@ __device__ __forceinline__ void tile_gemm(const uint16_t* A, int sa, const uint1
   }
 }
 
-// The same with B given as [K][N] (row stride sb) and a compile-time K:
-// each lane gathers its four consecutive k of one column with 16-bit reads
-// (the 16 columns of a fragment are consecutive halfwords, the four k groups
-// 16 banks apart at the strides used here).
-template <int MG, int K, class Epi>
+// The same with B given as [K][N] (row stride sb), K = k <= KMAX a multiple
+// of 16: each lane gathers its four consecutive k of one column with 16-bit
+// reads (the 16 columns of a fragment are consecutive halfwords, the four k
+// groups 16 banks apart at the strides used here).
+template <int MG, int KMAX, class Epi>
 __device__ __forceinline__ void tile_gemm_kn(const uint16_t* A, int sa, const uint16_t* Bkn, int sb, int mt, int nt,
-                                             Epi epi) {
+                                             int k, Epi epi) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int fr = lane & 15, fk = (lane >> 4) * 4;
   const int items = (mt + MG - 1) / MG * nt;
@@ -345,7 +345,8 @@ __device__ __forceinline__ void tile_gemm_kn(const uint16_t* A, int sa, const ui
 #pragma unroll
     for (int i = 0; i < MG; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = 0; k0 < KMAX; k0 += 16) {
+      if (k0 >= k) break;
       const uint16_t* bp = Bkn + (k0 + fk) * sb + tn * 16 + fr;
       bf16x4 b;
 #pragma unroll
@@ -710,7 +711,7 @@ __global__ __launch_bounds__(kTW * 64) void k_train(TrainArgs a) {
       st.store(R2, kSC, kTR, 2 * H / 8);               // the layer input, [row][j]
       __syncthreads();
       st.load(a.pbfT + off_wt(l) + static_cast<long>(H) * H, H, H, H, H / 8);
-      tile_gemm_kn<8, kTR>(dzT, kSR, R2, kSC, 8, 2 * H / 16, [&](int, int, int c, int j, float v) {
+      tile_gemm_kn<8, kTR>(dzT, kSR, R2, kSC, 8, 2 * H / 16, 16 * mtr, [&](int, int, int c, int j, float v) {
         put(off_w(l) + static_cast<long>(c) * 2 * H + j, v);
       });
       // d[x | m] = dz W_l, the message half first
