@@ -2,12 +2,12 @@
 """Headline benchmark: the L=18 FP64 Chebyshev propagator on one MI355X.
 
 Workload (the single-large-state configuration, whose 64 realisations run
-eight at a time): 2^18 complex FP64 amplitudes per realisation, all-to-all
+twelve at a time): 2^18 complex FP64 amplitudes per realisation, all-to-all
 secular dipolar XXZ couplings (153 pairs) from random 2D positions at unit
 density (min spacing 0.5, (1 - 3cos^2 theta)/r^3) plus Gaussian Hz disorder
 (sigma 0.5), real-time steps of dt = 0.05 at epsilon = 1e-12,
 autonormalisation off. One bench "step" is one Chebyshev time step of each of
-the `--set` realisations (default 8), stepped together: every order is one
+the `--set` realisations (default 12), stepped together: every order is one
 launch sequence for the set, each realisation with its own bound and order.
 
 Prints ONE JSON line. `value` is algorithmic GB/s of the whole timed region
@@ -37,15 +37,21 @@ HBM_PEAK_GBS = 8000.0
 # states in the set, states per launch): FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
 # (factors from the in-place scale kernel of known size), summed over the
 # set's launches and averaged over the benchmark's own steps;
-# profiles/k_dense_L18_order.md and profiles/k_wht_L18_8states_order.md.
+# profiles/k_dense_L18_order.md, profiles/k_wht_L18_8states_order.md and
+# profiles/k_wht_L18_12states_order.md.
+WHT = "k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>"
 MEASURED_TRAFFIC = {("k_dense<K_ORDERN, 18, 10>", 1, 1): 42417 * 1024,
-                    ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8, 8): 384722 * 1024,
-                    ("k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>", 8, 4): 385260 * 1024}
-# where each figure can be reproduced: the kernel-trace summary whose
-# per-dispatch means add up to `us_per_launch`, and the PMC runs behind
-# `traffic` (scaled FETCH_SIZE + WRITE_SIZE, not measured by this run)
-TRACE_PROFILE = {"k_dense<K_ORDERN, 18, 10>": "profiles/k_dense_L18_order.md",
-                 "k_wht_cols<8> + k_wht_rows_bwd<K_ORDERN>": "profiles/k_wht_L18_8states_order.md"}
+                    (WHT, 8, 8): 384722 * 1024,
+                    (WHT, 8, 4): 385260 * 1024,
+                    (WHT, 12, 12): 576850 * 1024,
+                    (WHT, 12, 6): 577387 * 1024}
+# where each figure can be reproduced, keyed by (order kernels, states in the
+# set): the kernel-trace summary whose per-dispatch means add up to
+# `us_per_launch`, and the PMC runs behind `traffic` (scaled FETCH_SIZE +
+# WRITE_SIZE, not measured by this run)
+TRACE_PROFILE = {("k_dense<K_ORDERN, 18, 10>", 1): "profiles/k_dense_L18_order.md",
+                 (WHT, 8): "profiles/k_wht_L18_8states_order.md",
+                 (WHT, 12): "profiles/k_wht_L18_12states_order.md"}
 TRAFFIC_PROFILE = TRACE_PROFILE
 BYTES_PER_AMP_ORDER = 88          # read T_{n-1}, T_{n-2}, diag; write T_n; read+write psi
 BYTES_FIRST, BYTES_SECOND = 40, 72  # order 1 (psi -> T_1), order 2 (fused psi update)
@@ -298,7 +304,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedule")
-    ap.add_argument("--set", type=int, default=8, metavar="S",
+    ap.add_argument("--set", type=int, default=12, metavar="S",
                     help="realisations stepped together, one launch per order (1..16; 1 = a single state)")
     ap.add_argument("--graphs", action="store_true", help="replay each step as a captured HIP graph")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
@@ -460,10 +466,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((kernel_name, S, S)),
-                     "traffic_from": TRAFFIC_PROFILE.get(kernel_name),
+                     "traffic_from": TRAFFIC_PROFILE.get((kernel_name, S)),
                      "kernel": kernel_name + f" (general Chebyshev order, {S} states per launch, one stream)",
                      "bytes_per_launch": kernel_bytes, "us_per_launch": round(per_order_us, 3),
-                     "profile": TRACE_PROFILE.get(kernel_name),
+                     "profile": TRACE_PROFILE.get((kernel_name, S)),
                      "copy_GBs": copy["GB/s"], "frac_of_copy": round(achieved / copy["GB/s"], 4),
                      "two_stream": two},
     }

@@ -530,17 +530,18 @@ def test_imaginary_time_relaxes_to_the_ground_state(ctx):
 
 def test_benchmarked_arrangement_matches_the_oracle(ctx):
     """The headline workload exactly as bench.py runs it: realisations
-    (20240726, 0..7) at L = 18 from bench.synthetic_system, expansions built
+    (20240726, 0..11) at L = 18 from bench.synthetic_system, expansions built
     at dt = 0.05 and eps = 1e-12, the schedule the tuner picks at the set's
     one-stream width and adopted through the tuner's cache, two halves of
-    four on two streams, three chained steps through steps_set. Every
+    six on two streams, three chained steps through steps_set. Every
     realisation agrees with the oracle's three steps amplitude by amplitude
     to 1e-12 of its largest amplitude, and the kernels are the Hadamard
     path's."""
     import bench
     L = 18
-    banner("benchmarked arrangement: oracle three steps of eight L=18 realisations (CPU, about 15 s)")
-    systems = [bench.synthetic_system(L, index=i) for i in range(8)]
+    S = 12
+    banner(f"benchmarked arrangement: oracle three steps of {S} L=18 realisations (CPU, about 25 s)")
+    systems = [bench.synthetic_system(L, index=i) for i in range(S)]
     hams, states, wants = [], [], []
     for f, lo, hi, jx, jy, jz, p0 in systems:
         x = p0.copy()
@@ -554,7 +555,7 @@ def test_benchmarked_arrangement_matches_the_oracle(ctx):
         states.append(spx.StateVector(ctx, L, p0))
     built = [spx.cheb_build(spx.REAL, h.spectral_bound(), 0.05, 1e-12) for h in hams]
     banner("benchmarked arrangement: tune, then three steps on the device")
-    hams[0].autotune(built[0][1], nvec=8)
+    hams[0].autotune(built[0][1], nvec=S)
     spx.autotune_mode(1)
     try:
         prop = spx.Propagator(ctx, L, 1e-12)
