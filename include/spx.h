@@ -129,7 +129,7 @@ int spx_autotune_mode(int mode);
 /* Fix a schedule by hand; 0 / -1 fields keep the defaults. `dense`: -1
  * default, 0 table kernels only, 1 the all-pairs gather kernel (L = 13..18,
  * flip-flop couplings only: no transverse field and jx = jy, else the table
- * kernels), 2 the Walsh-Hadamard path (L = 14..20, any couplings and
+ * kernels), 2 the Walsh-Hadamard path (L = 13..20, any couplings and
  * fields). */
 int spx_ham_set_schedule(spx_ham* h, int tile_bits, int block, int xcd, int dense, int dense_tile_bits);
 

@@ -185,7 +185,7 @@ int ham_prepare(spx_ham* h, double scale);   // diagonal + flip table at scale
 // The off-diagonal part of H through Walsh-Hadamard transforms
 // (spx_hadamard.hip): any couplings and fields, three passes per application
 // (two per Chebyshev order after the first), L in range.
-constexpr size_t kHadamardMinSpins = 14, kHadamardMaxSpins = 20;
+constexpr size_t kHadamardMinSpins = 13, kHadamardMaxSpins = 20;
 // Chebyshev weights of one vector of an order launch (see launch_order).
 // `defer` (general orders on the Hadamard path only, 0 elsewhere) batches
 // the accumulator updates: 0 psi += c T_n; 1 leave psi (the term waits);

@@ -486,6 +486,44 @@ __global__ __launch_bounds__(kRowThreads) void k_wht_rows_bwd(WhtArgs a) {
   }
 }
 
+// Pass 2 at G = 3 (L = 13): a column's eight values are the whole column
+// transform, so one thread takes one column, runs the forward stages, the
+// product with A (U) or B (V) and the inverse stages in registers, and
+// consecutive threads take consecutive columns (whole 1 KiB rows per
+// wavefront). The same arithmetic as k_wht_cols<3>, whose workgroups would
+// be eight threads.
+__global__ __launch_bounds__(256) void k_wht_cols3(WhtArgs a) {
+  constexpr uint32_t T = 1u << kTB, NT = 8;
+  const int v = static_cast<int>(blockIdx.y >> 1);
+  const bool vhalf = blockIdx.y & 1u;
+  double2* __restrict__ X = vhalf ? a.w[v] : a.u[v];
+  const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  double2 x[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) x[r] = X[(static_cast<size_t>(r) << kTB) | col];
+  const double* __restrict__ tab = vhalf ? a.eb[v] : a.e[v];
+  double e[8];
+  {
+    const double part = tab[col];
+    double top[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) top[j] = tab[T + NT + j * T + col];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      double s = part + tab[T + r];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s += ((r >> j) & 1) ? -top[j] : top[j];
+      e[r] = s;
+    }
+  }
+  butterflies<0, 3>(x);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) x[r] = make_double2(e[r] * x[r].x, e[r] * x[r].y);
+  butterflies<0, 3>(x);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) X[(static_cast<size_t>(r) << kTB) | col] = x[r];
+}
+
 template <int G>
 void launch_cols(hipStream_t st, const WhtArgs& a, unsigned nvec) {
   const dim3 grid((1u << kTB) / kCols, 2 * nvec);
@@ -511,7 +549,7 @@ int hadamard_prepare(spx_ham* h, const std::vector<double>& anti, const std::vec
                      const std::vector<double>& cx, const std::vector<double>& cy,
                      const std::vector<double>& dfield, const std::vector<double>& dpair) {
   const size_t L = h->nspins;
-  if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard path needs 14 <= L <= 20");
+  if (!hadamard_eligible(L)) return fail(SPX_EINVAL, "Hadamard path needs 13 <= L <= 20");
   const size_t T = size_t(1) << kTB, G = L - kTB, R = size_t(1) << G, TS = T + R + G * T;
   // tables already built for these weights: point at them
   std::vector<double> key;
@@ -685,6 +723,7 @@ int launch_hadamard(int kind, const HadamardLaunch& b, unsigned nvec) {
   const dim3 rows(static_cast<unsigned>(n >> kTB), nvec);
   if (!b.ready) k_wht_rows_fwd<<<rows, kRowThreads, sizeof(double2) * (2u << kTB), st>>>(a);
   switch (L - kTB) {
+    case 3: k_wht_cols3<<<dim3((1u << kTB) / 256, 2 * nvec), 256, 0, st>>>(a); break;
     case 4: launch_cols<4>(st, a, nvec); break;
     case 5: launch_cols<5>(st, a, nvec); break;
     case 6: launch_cols<6>(st, a, nvec); break;

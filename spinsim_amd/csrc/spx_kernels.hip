@@ -1062,7 +1062,7 @@ int xcd_remap(const spx_ham* h) {
 // The two specialised paths and their weights: the all-pairs gather kernel
 // (dense = 1; L = 13..18, flip-flop couplings only: no transverse field and
 // no flip-both weight at this scale) and the Hadamard transforms (dense = 2;
-// L = 14..20, any couplings and fields). SPX_DENSE=0 (or a schedule with
+// L = 13..20, any couplings and fields). SPX_DENSE=0 (or a schedule with
 // dense = 0) switches both off; without either, default_dense_path chooses.
 int dense_tile_bits(const spx_ham* h) {
   static const int env = env_int("SPX_DENSE_TB", 10);

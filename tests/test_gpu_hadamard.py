@@ -132,7 +132,7 @@ def test_l18_conservation_and_time_reversal(ctx):
     assert D.max_abs_diff(s.download(), psi0) < 1e-10
 
 
-@pytest.mark.parametrize("L", [13, 21])
+@pytest.mark.parametrize("L", [12, 21])
 def test_falls_back_outside_its_sizes(ctx, L):
     banner(f"hadamard fallback L={L}")
     fields, c = dipolar_system(L, 7)
@@ -142,7 +142,7 @@ def test_falls_back_outside_its_sizes(ctx, L):
     assert not h.kernel_name().startswith("k_wht"), h.kernel_name()
 
 
-@pytest.mark.parametrize("L,transverse,aniso", [(14, True, False), (14, False, True), (16, True, True)])
+@pytest.mark.parametrize("L,transverse,aniso", [(13, True, True), (14, True, False), (14, False, True), (16, True, True)])
 def test_transverse_fields_and_flip_both_weights(ctx, L, transverse, aniso):
     """Pulses (hx, hy) and jx != jy: the Y-string sandwich S W B W S^+ and
     the fields in A and B, against the oracle's apply and evolution."""
