@@ -292,7 +292,7 @@ def copy_probe(spx, L, reps=50):
     return out
 
 
-def main():
+def build_parser() -> argparse.ArgumentParser:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -317,7 +317,11 @@ def main():
                     help="with --batched-only: realisations in the batch")
     ap.add_argument("--batched-sizes", default="", metavar="L[,L...]",
                     help="with --batched-only: cycle through these sizes instead of L = 8..12 by hash")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = build_parser().parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(realisations would shard across GPUs with a host-side index-ordered fold)",

@@ -43,3 +43,15 @@ def test_algorithmic_bytes_per_order():
     k = 13
     assert bench.step_bytes(k, n) == (bench.BYTES_FIRST + bench.BYTES_SECOND
                                       + (k - 2) * bench.BYTES_PER_AMP_ORDER) * n
+
+
+def test_every_priced_arrangement_names_a_committed_profile():
+    """Each (order kernels, set size) the line can price names the committed
+    profile that reproduces its time and traffic, and the default set size
+    is one of them."""
+    default_set = next(a.default for a in bench.build_parser()._actions if a.dest == "set")
+    assert (bench.WHT, default_set) in bench.TRACE_PROFILE
+    assert (bench.WHT, default_set, default_set) in bench.MEASURED_TRAFFIC
+    for key, path in bench.TRACE_PROFILE.items():
+        assert os.path.isfile(os.path.join(ROOT, path)), (key, path)
+        assert (key[0], key[1], key[1]) in bench.MEASURED_TRAFFIC, key
